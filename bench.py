@@ -1,0 +1,174 @@
+#!/usr/bin/env python3
+"""Throughput of the per-timestep PIC loop on one MI355X.
+
+Workload (default): periodic 256^3 grid, 64 particles per cell per species,
+electrons (q/m -1, q -1/64, v_th 1.0) and ions (q/m 0.01, q +1/64, v_th 0.1),
+uniform B0z 0.5, dt 0.1, FP64 positions/velocities, seeded uniform Maxwellian
+(PCG32 seed 42/52) generated on the device.  One step = charge deposit (fused
+into the previous step), spectral Poisson solve, E, gather + Boris + wrap +
+re-bin of all 2^31 particles.
+
+Prints ONE JSON line on stdout (progress goes to stderr).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, published
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+WORKLOADS = {
+    # name: grid, ppc per species, precision
+    "uniform256_ppc64_fp64": dict(n=(256, 256, 256), ppc=64, precision=0),
+    "uniform128_ppc64_fp64": dict(n=(128, 128, 128), ppc=64, precision=0),
+    "uniform32_ppc16_fp64": dict(n=(32, 32, 32), ppc=16, precision=0),
+}
+
+
+def run_gpu(args):
+    import ohhelppic3d_amd as ohp
+
+    wl = WORKLOADS[args.workload]
+    n = wl["n"]
+    ppc = wl["ppc"]
+    ng = n[0] * n[1] * n[2]
+    npcl = ppc * ng
+    cfg = ohp.SimConfig(n=n, qm=[-1.0, 0.01], q_macro=[-1.0 / ppc, 1.0 / ppc],
+                        capacity=[npcl, npcl], dt=0.1, b0=(0.0, 0.0, 0.5),
+                        precision=wl["precision"], slack=args.slack, track_ids=False,
+                        tile=tuple(args.tile), chunks=args.chunks)
+    log(f"[bench] workload {args.workload}: grid {n}, {2 * npcl} particles, "
+        f"est. footprint {ohp.estimate_footprint(cfg) / 1e9:.1f} GB")
+    t0 = time.time()
+    sim = ohp.Simulation(cfg)
+    sim.seed_uniform_maxwell([npcl, npcl], [1.0, 0.1], seed=(42, 52))
+    log(f"[bench] created + loaded in {time.time() - t0:.1f} s, tile {sim.tile}, "
+        f"device bytes {sim.memory_footprint() / 1e9:.1f} GB")
+    for i in range(args.warmup):
+        sim.step(1)
+        log(f"[bench] warmup step {i + 1}/{args.warmup} done")
+    sim.synchronize()
+    sim.enable_timing(True)
+    per_step = []
+    t_start = time.perf_counter()
+    for i in range(args.steps):
+        ts = time.perf_counter()
+        sim.step(1)
+        per_step.append(time.perf_counter() - ts)
+        if (i + 1) % max(1, args.steps // 5) == 0:
+            log(f"[bench] timed step {i + 1}/{args.steps}: {per_step[-1] * 1e3:.2f} ms")
+    sim.synchronize()
+    t_total = time.perf_counter() - t_start
+    kt = sim.kernel_times()
+    diag = sim.diagnostics()
+    sim.close()
+
+    total_p = 2 * npcl
+    value = total_p * args.steps / t_total
+    push_ms, push_n = kt.get("push_fused", (0.0, 0))
+    solve_ms, solve_n = kt.get("field_solve", (0.0, 0))
+    push_avg = push_ms / max(push_n, 1)
+    sp_bytes = 48 if wl["precision"] == 0 else 28
+    alg_bytes = 2 * sp_bytes * total_p + 24 * ng + 8 * ng
+    achieved = alg_bytes / (push_avg * 1e-3) / 1e9 if push_avg > 0 else 0.0
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", f"traffic_push_fused_{args.workload}.json")
+    if os.path.exists(tfile):
+        with open(tfile) as f:
+            traffic = json.load(f).get("hbm_bytes_per_launch")
+    res = {
+        "metric": "particle-steps/s at 256^3x64ppc, FP64 vs mixed; p50/p99 ensemble latency",
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": t_total / args.steps * 1e3,
+        "ms_per_step_p50": float(np.median(per_step) * 1e3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if wl["precision"] == 0 else "f32+f64",
+        "data": "synthetic (device-seeded PCG32 uniform Maxwellian)",
+        "config": {"workload": args.workload, "grid": list(n), "ppc_per_species": ppc,
+                   "species": 2, "particles": total_p, "dt": 0.1, "b0z": 0.5,
+                   "tile": list(args.tile), "slack": args.slack},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": "push_fused", "kernel_avg_ms": push_avg,
+                     "alg_bytes_per_launch": alg_bytes},
+        "field_solve_avg_ms": solve_ms / max(solve_n, 1),
+        "energy": {"total": diag["total_energy"], "field": diag["field_energy"],
+                   "repacks": diag["repacks"]},
+    }
+    return res
+
+
+def run_cpu_baseline(seconds: float):
+    """The FP64 oracle (a CPU restatement of the reference step) on the host:
+    32^3 grid, 16 ppc per species (2^20 particles), single thread."""
+    import oracle as orc
+
+    n = (32, 32, 32)
+    ppc = 16
+    rng = orc.Pcg32(42, 52)
+    sp = []
+    for qm, vt in ((-1.0, 1.0), (0.01, 0.1)):
+        x, y, z, vx, vy, vz = orc.load_uniform_maxwell(rng, ppc * 32 ** 3, n, vt)
+        sp.append(dict(qm=qm, q=(1 if qm > 0 else -1) / ppc, x=x, y=y, z=z, vx=vx, vy=vy,
+                       vz=vz))
+    sim = orc.OracleSim(n, sp, 0.1, (0, 0, 0.5), 0)
+    sim.step(1)  # warm-up
+    times = []
+    t_end = time.perf_counter() + seconds
+    while time.perf_counter() < t_end or len(times) < 3:
+        t = time.perf_counter()
+        sim.step(1)
+        times.append(time.perf_counter() - t)
+    npart = 2 * ppc * 32 ** 3
+    med = float(np.median(times))
+    return {"value": npart / med, "unit": "particle-steps/s", "cores": 1, "kind": "port",
+            "sample": f"FP64 oracle step, 32^3 grid x 16 ppc x 2 species ({npart} particles), "
+                      f"{len(times)} steps, median step {med * 1e3:.1f} ms, 1 thread"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="uniform256_ppc64_fp64", choices=list(WORKLOADS))
+    ap.add_argument("--slack", type=float, default=0.05)
+    ap.add_argument("--tile", type=int, nargs=3, default=(8, 8, 8))
+    ap.add_argument("--chunks", type=int, default=0)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(one MI355X holds the whole 256^3 domain)", file=sys.stderr)
+        sys.exit(2)
+    res = run_gpu(args)
+    if not args.no_cpu:
+        log("[bench] timing the CPU baseline (oracle)")
+        res["cpu_baseline"] = run_cpu_baseline(args.cpu_seconds)
+    else:
+        res["cpu_baseline"] = None
+    print(json.dumps(res), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,182 @@
+/*
+ * ohpic.h -- C ABI of the MI355X (gfx950) implementation of ohhelppic3d's
+ * per-timestep loop: CIC charge scatter, periodic spectral Poisson solve,
+ * node-centred E, CIC field gather + Boris push, periodic particle boundary,
+ * and the per-step re-binning that replaces OhHelp's particle migration.
+ *
+ * Plain C: pointers, sizes and status codes only.  Every call returns
+ * OHPIC_OK (0) or a negative OHPIC_E* code; ohpic_last_error(ctx) gives the
+ * message (ctx may be NULL for errors raised before a context exists).  The
+ * reference aborts on error (error stop / MPI_Abort); a Fortran caller turns a
+ * non-zero status into `error stop msg`.
+ *
+ * Ownership: the context owns every device allocation; host buffers passed in
+ * are owned by the caller and only read or written during the call.  One
+ * context = one simulation instance = one HIP stream; a context is not
+ * thread-safe, separate contexts are independent.
+ *
+ * Units and layout follow the reference: dx = 1, eps0 = 1, periodic grid of
+ * nx*ny*nz nodes, field arrays node-centred with x fastest
+ * (values(:, ix, iy, iz) order); particle positions in [0, n) per axis.
+ */
+#ifndef OHPIC_H
+#define OHPIC_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define OHPIC_MAX_SPECIES 8
+
+/* status codes */
+#define OHPIC_OK 0
+#define OHPIC_EINVAL -1   /* bad argument */
+#define OHPIC_EHIP -2     /* HIP runtime error */
+#define OHPIC_ENOMEM -3   /* device memory / bin capacity exhausted */
+#define OHPIC_ESTATE -4   /* call not valid in the current state */
+#define OHPIC_ELOST -5    /* particles lost (overflow buffer exhausted) */
+#define OHPIC_ENODEV -6   /* no usable GPU */
+
+/* precision modes */
+#define OHPIC_FP64 0      /* FP64 positions/velocities and arithmetic */
+#define OHPIC_MIXED 1     /* int32 cell + FP32 offset, FP32 v; FP64 fields */
+
+/* CIC weight order (linear_interpolator.f90:54-57, linear_scatter.f90:41-42) */
+#define OHPIC_WEIGHTS_REFERENCE 0 /* lower node gets r = frac(x) (as written) */
+#define OHPIC_WEIGHTS_STANDARD 1  /* lower node gets 1 - r */
+
+/* field selectors for get/put */
+#define OHPIC_FIELD_RHO 0
+#define OHPIC_FIELD_PHI 1
+#define OHPIC_FIELD_EX 2
+#define OHPIC_FIELD_EY 3
+#define OHPIC_FIELD_EZ 4
+#define OHPIC_FIELD_EB 5  /* 6 components: Ex, Ey, Ez, B0x, B0y, B0z */
+
+typedef struct ohpic_ctx ohpic_ctx;
+
+typedef struct {
+    int32_t nx, ny, nz;
+    int32_t nspecies;
+    double qm[OHPIC_MAX_SPECIES];        /* charge_to_mass_ratio (parameters.f90:127) */
+    double q_macro[OHPIC_MAX_SPECIES];   /* charge_per_macro_particle (parameters.f90:140-149) */
+    int64_t capacity[OHPIC_MAX_SPECIES]; /* max particles per species */
+    double dt;
+    double b0[3];          /* uniform magnetic field */
+    int32_t precision;     /* OHPIC_FP64 or OHPIC_MIXED */
+    int32_t weight_order;  /* OHPIC_WEIGHTS_* */
+    int32_t tile[3];       /* cells per tile; 0 = default (8 if n >= 32 else 4) */
+    double slack;          /* bin capacity slack fraction; <= 0 = 0.15 */
+    int32_t chunks;        /* work items per tile; 0 = fill the device */
+    int32_t track_ids;     /* keep a 64-bit particle id beside each particle */
+    int32_t device;        /* HIP device ordinal */
+    int32_t charge_bits;   /* fixed-point fraction bits of the charge grid; 0 = derived */
+    double overflow_frac;  /* overflow list size as a fraction of capacity; <= 0 = 0.05 */
+} ohpic_config;
+
+typedef struct {
+    int64_t step;                          /* completed steps */
+    double ke_old[OHPIC_MAX_SPECIES];      /* sum 1/2 m |v^{n-1/2}|^2 of the last push */
+    double ke_new[OHPIC_MAX_SPECIES];      /* sum 1/2 m |v^{n+1/2}|^2 of the last push */
+    double field_energy;                   /* 1/2 sum rho*phi of the last solve */
+    double field_energy_e2;                /* 1/2 sum |E|^2 of the last solve */
+    double net_charge;                     /* sum rho of the last solve */
+    double total_energy;                   /* (ke_old+ke_new)/2 summed + field_energy */
+    int64_t repacks;                       /* full re-bins so far */
+    int64_t fast_particles;                /* particles that moved > 1 cell in a step */
+} ohpic_diag;
+
+/* --- context --------------------------------------------------------------- */
+int ohpic_config_default(ohpic_config *cfg);
+int ohpic_create(const ohpic_config *cfg, ohpic_ctx **out);
+int ohpic_destroy(ohpic_ctx *ctx);
+const char *ohpic_last_error(const ohpic_ctx *ctx);
+int ohpic_get_config(const ohpic_ctx *ctx, ohpic_config *out);
+
+/* --- particles (oh_particle storage, ohparticles.f90:16-78) -----------------
+ * SoA host arrays; positions must lie in [0, n).  pid may be NULL. */
+int ohpic_load_particles(ohpic_ctx *ctx, int ispec, int64_t n, const double *x,
+                         const double *y, const double *z, const double *vx,
+                         const double *vy, const double *vz, const int64_t *pid);
+int ohpic_particle_count(ohpic_ctx *ctx, int ispec, int64_t *n);
+/* Copies the species' particles out in bin order (pid gives the identity). */
+int ohpic_store_particles(ohpic_ctx *ctx, int ispec, int64_t cap, int64_t *n, double *x,
+                          double *y, double *z, double *vx, double *vy, double *vz,
+                          int64_t *pid);
+/* AoS interchange in the oh_particle record layout: x,y,z,vx,vy,vz (f64),
+ * pid (i64), nid, spec (i32; spec is 1-based as in the reference). */
+typedef struct {
+    double x, y, z, vx, vy, vz;
+    int64_t pid;
+    int32_t nid, spec;
+} ohpic_oh_particle;
+int ohpic_load_oh_particles(ohpic_ctx *ctx, int64_t n, const ohpic_oh_particle *rec);
+int ohpic_store_oh_particles(ohpic_ctx *ctx, int64_t cap, int64_t *n, ohpic_oh_particle *rec);
+
+/* Seeded loaders generated on the device.  Draw order follows the reference
+ * injector (particle_injector_with_distribution.f90:51-72): one shared PCG32
+ * stream (pcg_generator.f90:21-41), species in order, one random_fix draw per
+ * species, then per particle x, y, z (u*n) and four Box-Muller uniforms with
+ * std vth[s].  np[s] particles per species. */
+int ohpic_seed_uniform_maxwell(ohpic_ctx *ctx, uint64_t initstate, uint64_t initseq,
+                               const int64_t *np, const double *vth);
+/* Density 1 + amp*exp(-|r-c|^2/(2 sigma^2)) by rejection sampling; particle i
+ * of species s draws from the sub-stream at offset (s*2^40 + i*budget). */
+int ohpic_seed_hump_maxwell(ohpic_ctx *ctx, uint64_t initstate, uint64_t initseq,
+                            const int64_t *np, const double *vth, double amp, double sigma,
+                            uint64_t budget);
+
+/* --- the per-step operators --------------------------------------------------
+ * deposit      : rho := sum_s q_s W(x)   (linear_scatter.f90:22-55 over all
+ *                particles, ohhelppic3d.f90:311-320)
+ * field_solve  : rho -> phi -> E       (poisson_field_solver.f90:90-175)
+ * gather_push  : E(x) gather + Boris  (linear_interpolator.f90:22-69,
+ *                particle_mover_boris.f90:21-64; ohhelppic3d.f90:347-349)
+ * boundary     : periodic wrap (applied twice) + re-bin
+ *                (periodic_particle_boundary.f90:42-53; ohhelppic3d.f90:351-355)
+ * step         : nsteps of deposit -> solve -> gather_push -> boundary, fused. */
+int ohpic_deposit(ohpic_ctx *ctx);
+int ohpic_field_solve(ohpic_ctx *ctx);
+int ohpic_gather_push(ohpic_ctx *ctx);
+int ohpic_boundary(ohpic_ctx *ctx);
+int ohpic_step(ohpic_ctx *ctx, int nsteps);
+int ohpic_synchronize(ohpic_ctx *ctx);
+
+/* --- fields (t_OhField, ohfield.f90:53-72) -----------------------------------
+ * Copy between the device grid and a host array values(nelem, lb:ub) in
+ * Fortran order; component `comp` (0-based) of the host array receives the
+ * field (for OHPIC_FIELD_EB all 6 components, comp ignored).  Halo nodes
+ * outside [0, n) are filled periodically on get; put reads [0, n) only. */
+int ohpic_get_field(ohpic_ctx *ctx, int which, double *values, int nelem, const int32_t lb[3],
+                    const int32_t ub[3], int comp);
+int ohpic_put_field(ohpic_ctx *ctx, int which, const double *values, int nelem,
+                    const int32_t lb[3], const int32_t ub[3], int comp);
+
+/* --- FFT (t_MPIFFTSolver3d forward/backward, mpi_fftw3_solver.f90:209-236) --
+ * Separable 3-D real-to-real transform of an nx*ny*nz host array (x fastest):
+ * forward = R2HC on every axis (FFTW halfcomplex layout); backward = HC2R on
+ * every axis divided by nx*ny*nz.  in and out may alias. */
+int ohpic_fft3d_r2r(int device, int nx, int ny, int nz, int backward, const double *in,
+                    double *out);
+
+/* --- diagnostics ------------------------------------------------------------*/
+int ohpic_diagnostics(ohpic_ctx *ctx, ohpic_diag *out);
+/* per-cell particle counts (cell ix + nx*(iy + ny*iz)); ispec < 0 = all */
+int ohpic_cell_counts(ohpic_ctx *ctx, int ispec, int32_t *counts);
+/* Kernel timing (HIP events on the context stream).  enable != 0 starts
+ * recording; ohpic_kernel_times fills up to n (name, total_ms, launches). */
+int ohpic_enable_timing(ohpic_ctx *ctx, int enable);
+int ohpic_kernel_times(ohpic_ctx *ctx, int n, char names[][32], double *total_ms,
+                       int64_t *launches, int *count);
+/* Device memory held by the context, bytes. */
+int ohpic_memory_footprint(ohpic_ctx *ctx, int64_t *bytes);
+/* Predicted device bytes for a configuration (used for admission). */
+int ohpic_estimate_footprint(const ohpic_config *cfg, int64_t *bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* OHPIC_H */

@@ -1,0 +1,1329 @@
+// ohpic_api.cpp -- host runtime and C ABI of libohpic.so (see include/ohpic.h).
+//
+// A context owns one simulation instance: its device buffers, one HIP stream,
+// the tile bins of every species and the work list the particle kernels
+// consume.  The per-step sequence is
+//   [deposit if the charge grid is stale] -> rho from int64 grid -> 5 FFT
+//   passes (x R2HC, y R2HC, fused z forward/divide/inverse, y HC2R, x HC2R)
+//   -> node-centred E -> fused gather/push/wrap/deposit/re-bin kernel
+// followed by a check of the overflow flags and, when a bin overflowed, a
+// full re-bin (repack) of the affected species.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "../../include/ohpic.h"
+#include "ohpic_fft.hpp"
+#include "ohpic_internal.hpp"
+#include "ohpic_kernels.hpp"
+
+using namespace ohpic;
+
+namespace {
+
+std::mutex g_err_mu;
+std::string g_err;
+
+constexpr int kNumTimers = 4;
+const char *kTimerNames[kNumTimers] = {"push_fused", "field_solve", "deposit", "repack"};
+
+struct TimerSlot {
+    hipEvent_t a = nullptr, b = nullptr;
+    bool pending = false;
+    double total_ms = 0.0;
+    int64_t launches = 0;
+};
+
+struct SpeciesHost {
+    int64_t capacity = 0;  // max particles
+    int64_t alloc = 0;     // elements per buffer component
+    int64_t n = 0;         // particles loaded
+    std::vector<int64_t> base;
+    std::vector<int32_t> cap;
+};
+
+}  // namespace
+
+struct ohpic_ctx {
+    ohpic_config cfg{};
+    int device = 0;
+    hipStream_t st = nullptr;
+    int n[3]{}, T[3]{}, nt[3]{};
+    int ntiles = 0;
+    int64_t ng = 0;
+    int charge_bits = 0;
+    SpeciesHost sp[kMaxSpecies];
+    InstDev h{};                 // host mirror of the device instance record
+    InstDev *d_inst = nullptr;
+    WorkItem *d_items = nullptr;
+    int nitems = 0;
+    int nitems_alloc = 0;
+    int64_t *d_tmp64 = nullptr;  // [ntiles] scratch for repack / store
+    int32_t *d_tmp32a = nullptr; // [ntiles]
+    int32_t *d_tmp32b = nullptr; // [ntiles]
+    int64_t *d_tmp64b = nullptr; // [ntiles]
+    unsigned long long *d_cursor = nullptr;
+    int32_t *d_cellcounts = nullptr;
+    FftAxisDev ax[3]{};
+    double *d_lam[3]{};
+    unsigned long long *h_flags = nullptr;  // pinned
+    double *h_diag = nullptr;               // pinned
+    int par = 0;
+    bool rho_valid = false;
+    bool binned = true;
+    int64_t step = 0;
+    int64_t repacks = 0;
+    int64_t fast = 0;
+    int num_cus = 256;
+    bool timing = false;
+    TimerSlot timers[kNumTimers];
+    std::vector<void *> allocs;
+    int64_t bytes = 0;
+    std::string err;
+};
+
+namespace {
+
+int fail(ohpic_ctx *ctx, int code, const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    if (ctx) ctx->err = buf;
+    std::lock_guard<std::mutex> lk(g_err_mu);
+    g_err = buf;
+    return code;
+}
+
+#define HIPCHK(ctx, expr)                                                                   \
+    do {                                                                                    \
+        hipError_t e_ = (expr);                                                             \
+        if (e_ != hipSuccess)                                                               \
+            return fail(ctx, OHPIC_EHIP, "%s failed: %s (%s:%d)", #expr,                   \
+                        hipGetErrorString(e_), __FILE__, __LINE__);                         \
+    } while (0)
+
+template <typename T>
+int dalloc(ohpic_ctx *ctx, T **p, int64_t count)
+{
+    *p = nullptr;
+    if (count <= 0) count = 1;
+    const size_t bytes = (size_t)count * sizeof(T);
+    hipError_t e = hipMalloc((void **)p, bytes);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(ctx, OHPIC_ENOMEM, "hipMalloc of %zu bytes failed: %s", bytes,
+                    hipGetErrorString(e));
+    }
+    ctx->allocs.push_back((void *)*p);
+    ctx->bytes += (int64_t)bytes;
+    return OHPIC_OK;
+}
+
+int64_t round_up(int64_t v, int64_t m) { return (v + m - 1) / m * m; }
+
+int64_t species_alloc(const ohpic_config &c, int s, int ntiles)
+{
+    const double slack = c.slack > 0 ? c.slack : 0.15;
+    return (int64_t)std::ceil((double)c.capacity[s] * (1.0 + slack)) + (int64_t)ntiles * 80 + 64;
+}
+
+int64_t overflow_cap(const ohpic_config &c)
+{
+    const double f = c.overflow_frac > 0 ? c.overflow_frac : 0.05;
+    int64_t tot = 0;
+    for (int s = 0; s < c.nspecies; ++s) tot += c.capacity[s];
+    return (int64_t)std::ceil(f * (double)tot) + 4096;
+}
+
+void pick_tile(const ohpic_config &c, int T[3])
+{
+    for (int a = 0; a < 3; ++a) T[a] = c.tile[a];
+    if (T[0] <= 0 || T[1] <= 0 || T[2] <= 0) {
+        const int m = std::min(c.nx, std::min(c.ny, c.nz));
+        const int t = m >= 32 ? 8 : 4;
+        T[0] = T[1] = T[2] = t;
+    }
+}
+
+int derive_charge_bits(const ohpic_config &c, int64_t ng)
+{
+    if (c.charge_bits > 0) return c.charge_bits;
+    // (1) every contribution |q w| * 2^S < 2^51 (exact rounding in fix_round)
+    // (2) node sums up to 1024x the mean species density stay below 2^62
+    double qmax = 0.0, bound = 0.0;
+    for (int s = 0; s < c.nspecies; ++s) {
+        qmax = std::max(qmax, std::fabs(c.q_macro[s]));
+        bound += std::fabs(c.q_macro[s]) * (double)c.capacity[s] / (double)std::max<int64_t>(ng, 1);
+    }
+    if (qmax <= 0) qmax = 1.0;
+    bound = std::max(bound * 1024.0, 1.0);
+    const int s1 = 50 - (int)std::ceil(std::log2(qmax));
+    const int s2 = 62 - (int)std::ceil(std::log2(bound));
+    return std::max(8, std::min(s1, s2));
+}
+
+int64_t estimate_bytes(const ohpic_config &c)
+{
+    int T[3];
+    pick_tile(c, T);
+    const int64_t nt = (int64_t)((c.nx + T[0] - 1) / T[0]) * ((c.ny + T[1] - 1) / T[1]) *
+                       ((c.nz + T[2] - 1) / T[2]);
+    const int64_t ng = (int64_t)c.nx * c.ny * c.nz;
+    const int per_particle = (c.precision == OHPIC_MIXED ? 28 : 48) + (c.track_ids ? 8 : 0);
+    int64_t b = 0;
+    for (int s = 0; s < c.nspecies; ++s) {
+        b += 2 * species_alloc(c, s, (int)nt) * per_particle;
+        b += nt * (8 + 4 + 4 + 4);
+    }
+    b += overflow_cap(c) * (per_particle + 8);
+    b += ng * (8 * 2 + 8 * 5) + ng * 4;  // acc x2, rho, phi, E x3, cell counts
+    b += nt * (16 + 8 + 8 + 4 + 4) * 2;
+    return b;
+}
+
+int sync_upload_inst(ohpic_ctx *ctx)
+{
+    HIPCHK(ctx, hipMemcpyAsync(ctx->d_inst, &ctx->h, sizeof(InstDev), hipMemcpyHostToDevice,
+                               ctx->st));
+    return OHPIC_OK;
+}
+
+void timer_begin(ohpic_ctx *ctx, int k)
+{
+    if (!ctx->timing) return;
+    TimerSlot &t = ctx->timers[k];
+    if (!t.a) {
+        hipEventCreate(&t.a);
+        hipEventCreate(&t.b);
+    }
+    if (t.pending) {
+        hipEventSynchronize(t.b);
+        float ms = 0;
+        hipEventElapsedTime(&ms, t.a, t.b);
+        t.total_ms += ms;
+        t.launches += 1;
+        t.pending = false;
+    }
+    hipEventRecord(t.a, ctx->st);
+}
+
+void timer_end(ohpic_ctx *ctx, int k)
+{
+    if (!ctx->timing) return;
+    TimerSlot &t = ctx->timers[k];
+    hipEventRecord(t.b, ctx->st);
+    t.pending = true;
+}
+
+void timers_collect(ohpic_ctx *ctx)
+{
+    for (int k = 0; k < kNumTimers; ++k) {
+        TimerSlot &t = ctx->timers[k];
+        if (!t.pending) continue;
+        hipEventSynchronize(t.b);
+        float ms = 0;
+        hipEventElapsedTime(&ms, t.a, t.b);
+        t.total_ms += ms;
+        t.launches += 1;
+        t.pending = false;
+    }
+}
+
+int build_work_items(ohpic_ctx *ctx)
+{
+    const int nt = ctx->ntiles;
+    std::vector<int64_t> load(nt, 0);
+    int64_t maxcap = 0;
+    for (int s = 0; s < ctx->cfg.nspecies; ++s)
+        for (int t = 0; t < nt; ++t) {
+            load[t] += ctx->sp[s].cap.empty() ? 0 : ctx->sp[s].cap[t];
+            if (!ctx->sp[s].cap.empty()) maxcap = std::max<int64_t>(maxcap, ctx->sp[s].cap[t]);
+        }
+    int nch = ctx->cfg.chunks;
+    if (nch <= 0) {
+        const int64_t target = (int64_t)ctx->num_cus * 8;
+        nch = (int)std::max<int64_t>(1, (target + nt - 1) / nt);
+        // keep at least ~512 particles per chunk of the fullest bin
+        const int64_t lim = std::max<int64_t>(1, maxcap / 512);
+        nch = (int)std::min<int64_t>(nch, lim);
+    }
+    std::vector<int> order(nt);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int a, int b) { return load[a] > load[b]; });
+    std::vector<WorkItem> items;
+    items.reserve((size_t)nt * nch);
+    for (int c = 0; c < nch; ++c)
+        for (int i = 0; i < nt; ++i) items.push_back(WorkItem{0, order[i], c, nch});
+    // heavy tiles first within each chunk round: interleave so the first
+    // dispatched items are the heaviest chunks
+    std::stable_sort(items.begin(), items.end(), [&](const WorkItem &a, const WorkItem &b) {
+        return load[a.tile] > load[b.tile];
+    });
+    if ((int)items.size() > ctx->nitems_alloc) {
+        if (ctx->d_items) {
+            hipFree(ctx->d_items);
+            ctx->allocs.erase(std::find(ctx->allocs.begin(), ctx->allocs.end(),
+                                        (void *)ctx->d_items));
+            ctx->d_items = nullptr;
+        }
+        int rc = dalloc(ctx, &ctx->d_items, (int64_t)items.size());
+        if (rc) return rc;
+        ctx->nitems_alloc = (int)items.size();
+    }
+    ctx->nitems = (int)items.size();
+    HIPCHK(ctx, hipMemcpyAsync(ctx->d_items, items.data(), items.size() * sizeof(WorkItem),
+                               hipMemcpyHostToDevice, ctx->st));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->st));
+    return OHPIC_OK;
+}
+
+// Assign capacities/bases from per-bin counts and upload them.
+int set_bins(ohpic_ctx *ctx, int s, const std::vector<int32_t> &counts)
+{
+    SpeciesHost &S = ctx->sp[s];
+    const double slack = ctx->cfg.slack > 0 ? ctx->cfg.slack : 0.15;
+    const int nt = ctx->ntiles;
+    S.base.assign(nt, 0);
+    S.cap.assign(nt, 0);
+    int64_t off = 0;
+    for (int t = 0; t < nt; ++t) {
+        const int64_t c = counts[t];
+        int64_t cap = round_up((int64_t)std::ceil((double)c * (1.0 + slack)) + 16, 64);
+        if (cap > INT32_MAX) return fail(ctx, OHPIC_ENOMEM, "bin capacity exceeds int32");
+        S.base[t] = off;
+        S.cap[t] = (int32_t)cap;
+        off += cap;
+    }
+    if (off > S.alloc)
+        return fail(ctx, OHPIC_ENOMEM,
+                    "species %d: bins need %lld elements, buffer holds %lld (raise capacity)", s,
+                    (long long)off, (long long)S.alloc);
+    HIPCHK(ctx, hipMemcpyAsync(ctx->h.sp[s].bin_base, S.base.data(), nt * sizeof(int64_t),
+                               hipMemcpyHostToDevice, ctx->st));
+    HIPCHK(ctx, hipMemcpyAsync(ctx->h.sp[s].bin_cap, S.cap.data(), nt * sizeof(int32_t),
+                               hipMemcpyHostToDevice, ctx->st));
+    return OHPIC_OK;
+}
+
+int read_flags(ohpic_ctx *ctx)
+{
+    HIPCHK(ctx, hipMemcpyAsync(ctx->h_flags, ctx->h.flags, kFlagSlots * sizeof(unsigned long long),
+                               hipMemcpyDeviceToHost, ctx->st));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->st));
+    return OHPIC_OK;
+}
+
+int clear_flags(ohpic_ctx *ctx)
+{
+    HIPCHK(ctx, launch_fill_u64(ctx->h.flags, kFlagSlots, 0ull, ctx->st));
+    return OHPIC_OK;
+}
+
+// Bin a contiguous list of n particles held in buffer stg = par^1 into the
+// bins of buffer par.
+int bin_from_staging(ohpic_ctx *ctx, int s, int64_t n)
+{
+    const int nt = ctx->ntiles;
+    const int stg = ctx->par ^ 1;
+    int32_t *cnt = ctx->h.sp[s].count[ctx->par];
+    HIPCHK(ctx, launch_fill_i32(cnt, nt, 0, ctx->st));
+    int rc = clear_flags(ctx);
+    if (rc) return rc;
+    HIPCHK(ctx, launch_bin_count(ctx->d_inst, 0, s, stg, n, ctx->T, ctx->st));
+    std::vector<int32_t> counts(nt);
+    HIPCHK(ctx, hipMemcpyAsync(counts.data(), cnt, nt * sizeof(int32_t), hipMemcpyDeviceToHost,
+                               ctx->st));
+    rc = read_flags(ctx);
+    if (rc) return rc;
+    if (ctx->h_flags[kFlagRange])
+        return fail(ctx, OHPIC_EINVAL, "species %d: %llu particle positions outside [0, n)", s,
+                    (unsigned long long)ctx->h_flags[kFlagRange]);
+    rc = set_bins(ctx, s, counts);
+    if (rc) return rc;
+    HIPCHK(ctx, launch_fill_i32(cnt, nt, 0, ctx->st));
+    HIPCHK(ctx, launch_bin_scatter(ctx->d_inst, 0, s, stg, n, ctx->T, ctx->st));
+    rc = read_flags(ctx);
+    if (rc) return rc;
+    if (ctx->h_flags[kFlagOverflow] || ctx->h_flags[kFlagLost])
+        return fail(ctx, OHPIC_ENOMEM, "binning overflowed (internal error)");
+    ctx->sp[s].n = n;
+    return OHPIC_OK;
+}
+
+// Compact the bins of buffer par (counts in count[par], possibly above cap)
+// plus species-s overflow entries into a contiguous list in buffer par^1.
+// Returns the number of particles listed.
+int bins_to_staging(ohpic_ctx *ctx, int s, int64_t nov, int64_t *nlisted)
+{
+    const int nt = ctx->ntiles;
+    std::vector<int32_t> counts(nt);
+    HIPCHK(ctx, hipMemcpyAsync(counts.data(), ctx->h.sp[s].count[ctx->par], nt * sizeof(int32_t),
+                               hipMemcpyDeviceToHost, ctx->st));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->st));
+    std::vector<int64_t> off(nt);
+    int64_t tot = 0;
+    for (int t = 0; t < nt; ++t) {
+        off[t] = tot;
+        tot += std::min(counts[t], ctx->sp[s].cap[t]);
+    }
+    HIPCHK(ctx, hipMemcpyAsync(ctx->d_tmp64, off.data(), nt * sizeof(int64_t),
+                               hipMemcpyHostToDevice, ctx->st));
+    HIPCHK(ctx, launch_bins_to_list(ctx->d_inst, 0, s, ctx->par, ctx->par ^ 1, nt,
+                                    ctx->h.sp[s].bin_base, ctx->h.sp[s].bin_cap,
+                                    ctx->h.sp[s].count[ctx->par], ctx->d_tmp64, ctx->st));
+    if (nov > 0) {
+        HIPCHK(ctx, launch_fill_u64(ctx->d_cursor, 1, 0ull, ctx->st));
+        HIPCHK(ctx, launch_ovf_to_list(ctx->d_inst, 0, s, ctx->par ^ 1, nov, tot, ctx->d_cursor,
+                                       ctx->st));
+        unsigned long long added = 0;
+        HIPCHK(ctx, hipMemcpyAsync(&added, ctx->d_cursor, sizeof added, hipMemcpyDeviceToHost,
+                                   ctx->st));
+        HIPCHK(ctx, hipStreamSynchronize(ctx->st));
+        tot += (int64_t)added;
+    }
+    HIPCHK(ctx, hipStreamSynchronize(ctx->st));
+    *nlisted = tot;
+    return OHPIC_OK;
+}
+
+int repack(ohpic_ctx *ctx, int64_t nov)
+{
+    timer_begin(ctx, 3);
+    for (int s = 0; s < ctx->cfg.nspecies; ++s) {
+        int64_t n = 0;
+        int rc = bins_to_staging(ctx, s, nov, &n);
+        if (rc) return rc;
+        rc = bin_from_staging(ctx, s, n);
+        if (rc) return rc;
+    }
+    timer_end(ctx, 3);
+    int rc = clear_flags(ctx);
+    if (rc) return rc;
+    ctx->repacks += 1;
+    return build_work_items(ctx);
+}
+
+int check_after_rebin(ohpic_ctx *ctx)
+{
+    int rc = read_flags(ctx);
+    if (rc) return rc;
+    timers_collect(ctx);
+    const unsigned long long *f = ctx->h_flags;
+    ctx->fast += (int64_t)f[kFlagFast];
+    if (f[kFlagLost])
+        return fail(ctx, OHPIC_ELOST, "%llu particles lost: overflow list full (raise slack)",
+                    (unsigned long long)f[kFlagLost]);
+    if (f[kFlagRange])
+        return fail(ctx, OHPIC_ESTATE,
+                    "%llu particles moved more than two box lengths in one step",
+                    (unsigned long long)f[kFlagRange]);
+    if (f[kFlagOverflow]) return repack(ctx, (int64_t)f[kFlagOverflow]);
+    return OHPIC_OK;
+}
+
+int do_deposit(ohpic_ctx *ctx)
+{
+    timer_begin(ctx, 2);
+    HIPCHK(ctx, launch_fill_u64(ctx->h.rho_acc[ctx->par], ctx->ng, 0ull, ctx->st));
+    HIPCHK(ctx, launch_particles(kModeDeposit, ctx->cfg.precision, ctx->cfg.weight_order, ctx->T,
+                                 ctx->d_inst, ctx->d_items, ctx->nitems, ctx->par, ctx->st));
+    timer_end(ctx, 2);
+    ctx->rho_valid = true;
+    return OHPIC_OK;
+}
+
+int do_solve(ohpic_ctx *ctx, bool from_acc)
+{
+    timer_begin(ctx, 1);
+    const int nx = ctx->n[0], ny = ctx->n[1], nz = ctx->n[2];
+    double *rho = ctx->h.rho, *phi = ctx->h.phi;
+    if (from_acc)
+        HIPCHK(ctx, launch_rho_from_acc(ctx->h.rho_acc[ctx->par], rho, ctx->ng, ctx->h.fix_inv,
+                                        ctx->st));
+    const double *L0 = ctx->d_lam[0], *L1 = ctx->d_lam[1], *L2 = ctx->d_lam[2];
+    HIPCHK(ctx, fft_axis_pass(kFftR2HC, 0, rho, false, 1.0, phi, 1.0, nx, ny, nz, ctx->ax[0], L0,
+                              L1, L2, ctx->st));
+    HIPCHK(ctx, fft_axis_pass(kFftR2HC, 1, phi, false, 1.0, phi, 1.0, nx, ny, nz, ctx->ax[1], L0,
+                              L1, L2, ctx->st));
+    HIPCHK(ctx, fft_axis_pass(kFftPoisson, 2, phi, false, 1.0, phi, 1.0, nx, ny, nz, ctx->ax[2],
+                              L0, L1, L2, ctx->st));
+    HIPCHK(ctx, fft_axis_pass(kFftHC2R, 1, phi, false, 1.0, phi, 1.0, nx, ny, nz, ctx->ax[1], L0,
+                              L1, L2, ctx->st));
+    HIPCHK(ctx, fft_axis_pass(kFftHC2R, 0, phi, false, 1.0, phi, 1.0 / (double)ctx->ng, nx, ny,
+                              nz, ctx->ax[0], L0, L1, L2, ctx->st));
+    HIPCHK(ctx, launch_fill_f64(ctx->h.diag + kDiagWe, 3, 0.0, ctx->st));
+    HIPCHK(ctx, launch_efield(phi, rho, ctx->h.e[0], ctx->h.e[1], ctx->h.e[2], nx, ny, nz,
+                              ctx->h.diag, ctx->st));
+    timer_end(ctx, 1);
+    return OHPIC_OK;
+}
+
+int zero_next_counts(ohpic_ctx *ctx)
+{
+    for (int s = 0; s < ctx->cfg.nspecies; ++s)
+        HIPCHK(ctx, launch_fill_i32(ctx->h.sp[s].count[ctx->par ^ 1], ctx->ntiles, 0, ctx->st));
+    return OHPIC_OK;
+}
+
+int one_step(ohpic_ctx *ctx)
+{
+    int rc;
+    if (!ctx->rho_valid && (rc = do_deposit(ctx))) return rc;
+    if ((rc = do_solve(ctx, true))) return rc;
+    HIPCHK(ctx, launch_fill_f64(ctx->h.diag, 2 * kMaxSpecies, 0.0, ctx->st));
+    if ((rc = zero_next_counts(ctx))) return rc;
+    HIPCHK(ctx, launch_fill_u64(ctx->h.rho_acc[ctx->par ^ 1], ctx->ng, 0ull, ctx->st));
+    if ((rc = clear_flags(ctx))) return rc;
+    timer_begin(ctx, 0);
+    HIPCHK(ctx, launch_particles(kModeFused, ctx->cfg.precision, ctx->cfg.weight_order, ctx->T,
+                                 ctx->d_inst, ctx->d_items, ctx->nitems, ctx->par, ctx->st));
+    timer_end(ctx, 0);
+    ctx->par ^= 1;
+    ctx->step += 1;
+    ctx->rho_valid = true;
+    return check_after_rebin(ctx);
+}
+
+int grid_copy_out(ohpic_ctx *ctx, const double *dev, std::vector<double> &host)
+{
+    host.resize(ctx->ng);
+    HIPCHK(ctx, hipMemcpyAsync(host.data(), dev, ctx->ng * sizeof(double), hipMemcpyDeviceToHost,
+                               ctx->st));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->st));
+    return OHPIC_OK;
+}
+
+int make_axis(ohpic_ctx *ctx, int n, FftAxisDev &ax)
+{
+    ax.n = n;
+    if (!fft_factorize(n, ax.fac, &ax.nfac))
+        return fail(ctx, OHPIC_EINVAL, "axis length %d: unsupported (max %d, prime factors <= %d)",
+                    n, kMaxFftLen, kMaxRadix);
+    std::vector<double2> tw(n);
+    fft_twiddles(n, tw.data());
+    double2 *d = nullptr;
+    int rc = dalloc(ctx, &d, n);
+    if (rc) return rc;
+    HIPCHK(ctx, hipMemcpy(d, tw.data(), n * sizeof(double2), hipMemcpyHostToDevice));
+    ax.tw = d;
+    return OHPIC_OK;
+}
+
+void release(ohpic_ctx *ctx)
+{
+    if (!ctx) return;
+    hipSetDevice(ctx->device);
+    if (ctx->st) hipStreamSynchronize(ctx->st);
+    for (void *p : ctx->allocs) hipFree(p);
+    ctx->allocs.clear();
+    if (ctx->h_flags) hipHostFree(ctx->h_flags);
+    if (ctx->h_diag) hipHostFree(ctx->h_diag);
+    for (auto &t : ctx->timers) {
+        if (t.a) hipEventDestroy(t.a);
+        if (t.b) hipEventDestroy(t.b);
+    }
+    if (ctx->st) hipStreamDestroy(ctx->st);
+    delete ctx;
+}
+
+}  // namespace
+
+// ============================================================== FFT helpers
+namespace ohpic {
+
+bool fft_factorize(int n, int *fac, int *nfac)
+{
+    if (n < 1 || n > kMaxFftLen) return false;
+    int k = 0;
+    int m = n;
+    while (m % 4 == 0 && k < kMaxFactors) {
+        fac[k++] = 4;
+        m /= 4;
+    }
+    for (int p = 2; m > 1 && k < kMaxFactors; ++p) {
+        while (m % p == 0 && k < kMaxFactors) {
+            if (p > kMaxRadix) return false;
+            fac[k++] = p;
+            m /= p;
+        }
+    }
+    if (m != 1) return false;
+    *nfac = k;
+    return true;
+}
+
+void fft_twiddles(int n, double2 *tw)
+{
+    const long double two_pi = 6.283185307179586476925286766559005768L;
+    for (int k = 0; k < n; ++k) {
+        // reduce to the first octant for accuracy
+        const long double a = two_pi * (long double)k / (long double)n;
+        tw[k].x = (double)cosl(a);
+        tw[k].y = (double)-sinl(a);
+    }
+}
+
+}  // namespace ohpic
+
+// ================================================================== C ABI
+extern "C" {
+
+int ohpic_config_default(ohpic_config *cfg)
+{
+    if (!cfg) return fail(nullptr, OHPIC_EINVAL, "cfg is NULL");
+    std::memset(cfg, 0, sizeof *cfg);
+    cfg->nx = cfg->ny = cfg->nz = 32;
+    cfg->nspecies = 2;
+    cfg->qm[0] = -1.0;
+    cfg->qm[1] = 0.01;
+    cfg->q_macro[0] = -1.0 / 16;
+    cfg->q_macro[1] = 1.0 / 16;
+    cfg->capacity[0] = cfg->capacity[1] = 16LL * 32 * 32 * 32;
+    cfg->dt = 0.1;
+    cfg->b0[2] = 0.5;
+    cfg->precision = OHPIC_FP64;
+    cfg->weight_order = OHPIC_WEIGHTS_REFERENCE;
+    cfg->slack = 0.15;
+    cfg->overflow_frac = 0.05;
+    return OHPIC_OK;
+}
+
+const char *ohpic_last_error(const ohpic_ctx *ctx)
+{
+    if (ctx) return ctx->err.c_str();
+    std::lock_guard<std::mutex> lk(g_err_mu);
+    static thread_local std::string copy;
+    copy = g_err;
+    return copy.c_str();
+}
+
+int ohpic_estimate_footprint(const ohpic_config *cfg, int64_t *bytes)
+{
+    if (!cfg || !bytes) return fail(nullptr, OHPIC_EINVAL, "NULL argument");
+    *bytes = estimate_bytes(*cfg);
+    return OHPIC_OK;
+}
+
+int ohpic_create(const ohpic_config *cfg, ohpic_ctx **out)
+{
+    if (!cfg || !out) return fail(nullptr, OHPIC_EINVAL, "NULL argument");
+    *out = nullptr;
+    const ohpic_config &c = *cfg;
+    if (c.nx < 1 || c.ny < 1 || c.nz < 1)
+        return fail(nullptr, OHPIC_EINVAL, "grid sizes must be >= 1");
+    if (c.nspecies < 1 || c.nspecies > OHPIC_MAX_SPECIES)
+        return fail(nullptr, OHPIC_EINVAL, "nspecies must be in [1, %d]", OHPIC_MAX_SPECIES);
+    if (c.precision != OHPIC_FP64)
+        return fail(nullptr, OHPIC_EINVAL, "precision mode %d not available in this build",
+                    c.precision);
+    if (c.weight_order != 0 && c.weight_order != 1)
+        return fail(nullptr, OHPIC_EINVAL, "weight_order must be 0 or 1");
+    for (int s = 0; s < c.nspecies; ++s) {
+        if (c.qm[s] == 0.0) return fail(nullptr, OHPIC_EINVAL, "species %d: qm == 0", s);
+        if (c.capacity[s] < 0) return fail(nullptr, OHPIC_EINVAL, "species %d: capacity < 0", s);
+    }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
+        (void)hipGetLastError();
+        return fail(nullptr, OHPIC_ENODEV, "no HIP device available");
+    }
+    if (c.device < 0 || c.device >= ndev)
+        return fail(nullptr, OHPIC_EINVAL, "device %d out of range (%d devices)", c.device, ndev);
+
+    ohpic_ctx *ctx = new ohpic_ctx();
+    ctx->cfg = c;
+    ctx->device = c.device;
+    int rc = OHPIC_OK;
+#define CREATE_CHK(expr)                     \
+    do {                                     \
+        if ((rc = (expr)) != OHPIC_OK) {     \
+            std::string e_ = ctx->err;       \
+            release(ctx);                    \
+            fail(nullptr, rc, "%s", e_.c_str()); \
+            return rc;                       \
+        }                                    \
+    } while (0)
+#define CREATE_HIP(expr)                                                                    \
+    do {                                                                                    \
+        hipError_t e_ = (expr);                                                             \
+        if (e_ != hipSuccess) {                                                             \
+            release(ctx);                                                                   \
+            return fail(nullptr, OHPIC_EHIP, "%s failed: %s", #expr, hipGetErrorString(e_)); \
+        }                                                                                   \
+    } while (0)
+
+    CREATE_HIP(hipSetDevice(ctx->device));
+    CREATE_HIP(hipStreamCreateWithFlags(&ctx->st, hipStreamNonBlocking));
+    {
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, ctx->device) == hipSuccess)
+            ctx->num_cus = prop.multiProcessorCount;
+    }
+    ctx->n[0] = c.nx;
+    ctx->n[1] = c.ny;
+    ctx->n[2] = c.nz;
+    ctx->ng = (int64_t)c.nx * c.ny * c.nz;
+    pick_tile(c, ctx->T);
+    if (!tile_supported(ctx->T[0], ctx->T[1], ctx->T[2])) {
+        release(ctx);
+        return fail(nullptr, OHPIC_EINVAL, "tile %dx%dx%d not compiled in", c.tile[0], c.tile[1],
+                    c.tile[2]);
+    }
+    for (int a = 0; a < 3; ++a) ctx->nt[a] = (ctx->n[a] + ctx->T[a] - 1) / ctx->T[a];
+    ctx->ntiles = ctx->nt[0] * ctx->nt[1] * ctx->nt[2];
+    ctx->charge_bits = derive_charge_bits(c, ctx->ng);
+
+    InstDev &h = ctx->h;
+    for (int a = 0; a < 3; ++a) {
+        h.n[a] = ctx->n[a];
+        h.T[a] = ctx->T[a];
+        h.nt[a] = ctx->nt[a];
+        h.b0[a] = c.b0[a];
+    }
+    h.ntiles = ctx->ntiles;
+    h.nspecies = c.nspecies;
+    h.weight_order = c.weight_order;
+    h.track_ids = c.track_ids ? 1 : 0;
+    h.precision = c.precision;
+    h.dt = c.dt;
+    h.fix_scale = std::ldexp(1.0, ctx->charge_bits);
+    h.fix_inv = std::ldexp(1.0, -ctx->charge_bits);
+    const int nt = ctx->ntiles;
+    for (int s = 0; s < c.nspecies; ++s) {
+        SpeciesHost &S = ctx->sp[s];
+        SpecDev &D = h.sp[s];
+        S.capacity = c.capacity[s];
+        S.alloc = species_alloc(c, s, nt);
+        for (int b = 0; b < 2; ++b) {
+            for (int k = 0; k < 6; ++k) {
+                double *p = nullptr;
+                CREATE_CHK(dalloc(ctx, &p, S.alloc));
+                D.buf[b][k] = p;
+            }
+            D.buf[b][6] = nullptr;
+            D.pid[b] = nullptr;
+            if (c.track_ids) CREATE_CHK(dalloc(ctx, &D.pid[b], S.alloc));
+            CREATE_CHK(dalloc(ctx, &D.count[b], nt));
+            CREATE_HIP(launch_fill_i32(D.count[b], nt, 0, ctx->st));
+        }
+        CREATE_CHK(dalloc(ctx, &D.bin_base, nt));
+        CREATE_CHK(dalloc(ctx, &D.bin_cap, nt));
+        std::vector<int32_t> zero(nt, 0);
+        CREATE_CHK(set_bins(ctx, s, zero));
+        D.qm = c.qm[s];
+        D.q = c.q_macro[s];
+        D.mass = c.q_macro[s] / c.qm[s];
+        D.qfix = std::ldexp(c.q_macro[s], ctx->charge_bits);
+    }
+    for (int b = 0; b < 2; ++b) {
+        CREATE_CHK(dalloc(ctx, &h.rho_acc[b], ctx->ng));
+        CREATE_HIP(launch_fill_u64(h.rho_acc[b], ctx->ng, 0ull, ctx->st));
+    }
+    CREATE_CHK(dalloc(ctx, &h.rho, ctx->ng));
+    CREATE_CHK(dalloc(ctx, &h.phi, ctx->ng));
+    for (int a = 0; a < 3; ++a) CREATE_CHK(dalloc(ctx, &h.e[a], ctx->ng));
+    for (double *p : {h.rho, h.phi, h.e[0], h.e[1], h.e[2]})
+        CREATE_HIP(launch_fill_f64(p, ctx->ng, 0.0, ctx->st));
+    CREATE_CHK(dalloc(ctx, &h.diag, kDiagSlots));
+    CREATE_HIP(launch_fill_f64(h.diag, kDiagSlots, 0.0, ctx->st));
+    CREATE_CHK(dalloc(ctx, &h.flags, kFlagSlots));
+    CREATE_HIP(launch_fill_u64(h.flags, kFlagSlots, 0ull, ctx->st));
+    h.ovf.cap = overflow_cap(c);
+    for (int k = 0; k < 6; ++k) {
+        double *p = nullptr;
+        CREATE_CHK(dalloc(ctx, &p, h.ovf.cap));
+        h.ovf.comp[k] = p;
+    }
+    h.ovf.comp[6] = nullptr;
+    h.ovf.pid = nullptr;
+    if (c.track_ids) CREATE_CHK(dalloc(ctx, &h.ovf.pid, h.ovf.cap));
+    CREATE_CHK(dalloc(ctx, &h.ovf.dest, h.ovf.cap));
+    CREATE_CHK(dalloc(ctx, &h.ovf.spec, h.ovf.cap));
+    CREATE_CHK(dalloc(ctx, &ctx->d_tmp64, nt));
+    CREATE_CHK(dalloc(ctx, &ctx->d_tmp64b, nt));
+    CREATE_CHK(dalloc(ctx, &ctx->d_tmp32a, nt));
+    CREATE_CHK(dalloc(ctx, &ctx->d_tmp32b, nt));
+    CREATE_CHK(dalloc(ctx, &ctx->d_cursor, 1));
+    CREATE_CHK(dalloc(ctx, &ctx->d_inst, 1));
+    // FFT plans and the Green's-function denominators per axis
+    for (int a = 0; a < 3; ++a) {
+        CREATE_CHK(make_axis(ctx, ctx->n[a], ctx->ax[a]));
+        const int n = ctx->n[a];
+        std::vector<double> lam(n);
+        const double ref_pi = 3.14159265358979;  // science_constants.f90:4
+        for (int k = 0; k < n; ++k) {
+            // calc_wave_number, periodic branch (poisson_solver3d.f90:111-117)
+            const double wn = (k <= n / 2) ? 2.0 * std::sin(ref_pi * k / (double)n)
+                                           : 2.0 * std::sin(ref_pi * (n - k) / (double)n);
+            lam[k] = wn * wn;
+        }
+        CREATE_CHK(dalloc(ctx, &ctx->d_lam[a], n));
+        CREATE_HIP(hipMemcpy(ctx->d_lam[a], lam.data(), n * sizeof(double), hipMemcpyHostToDevice));
+    }
+    CREATE_HIP(hipHostMalloc((void **)&ctx->h_flags, kFlagSlots * sizeof(unsigned long long),
+                             hipHostMallocDefault));
+    CREATE_HIP(hipHostMalloc((void **)&ctx->h_diag, kDiagSlots * sizeof(double),
+                             hipHostMallocDefault));
+    CREATE_CHK(sync_upload_inst(ctx));
+    CREATE_CHK(build_work_items(ctx));
+    CREATE_HIP(hipStreamSynchronize(ctx->st));
+#undef CREATE_CHK
+#undef CREATE_HIP
+    ctx->par = 0;
+    ctx->rho_valid = false;
+    ctx->binned = true;
+    *out = ctx;
+    return OHPIC_OK;
+}
+
+int ohpic_destroy(ohpic_ctx *ctx)
+{
+    release(ctx);
+    return OHPIC_OK;
+}
+
+int ohpic_get_config(const ohpic_ctx *ctx, ohpic_config *out)
+{
+    if (!ctx || !out) return fail(nullptr, OHPIC_EINVAL, "NULL argument");
+    *out = ctx->cfg;
+    out->tile[0] = ctx->T[0];
+    out->tile[1] = ctx->T[1];
+    out->tile[2] = ctx->T[2];
+    out->charge_bits = ctx->charge_bits;
+    return OHPIC_OK;
+}
+
+int ohpic_load_particles(ohpic_ctx *ctx, int ispec, int64_t n, const double *x, const double *y,
+                         const double *z, const double *vx, const double *vy, const double *vz,
+                         const int64_t *pid)
+{
+    if (!ctx) return fail(nullptr, OHPIC_EINVAL, "ctx is NULL");
+    if (ispec < 0 || ispec >= ctx->cfg.nspecies)
+        return fail(ctx, OHPIC_EINVAL, "species %d out of range", ispec);
+    if (n < 0 || n > ctx->sp[ispec].capacity)
+        return fail(ctx, OHPIC_ENOMEM, "species %d: %lld particles exceed capacity %lld", ispec,
+                    (long long)n, (long long)ctx->sp[ispec].capacity);
+    if (n > 0 && (!x || !y || !z || !vx || !vy || !vz))
+        return fail(ctx, OHPIC_EINVAL, "NULL particle array");
+    if (!ctx->binned) return fail(ctx, OHPIC_ESTATE, "particles pushed but not re-binned");
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    const int stg = ctx->par ^ 1;
+    SpecDev &D = ctx->h.sp[ispec];
+    const double *src[6] = {x, y, z, vx, vy, vz};
+    for (int k = 0; k < 6 && n > 0; ++k)
+        HIPCHK(ctx, hipMemcpyAsync(D.buf[stg][k], src[k], n * sizeof(double),
+                                   hipMemcpyHostToDevice, ctx->st));
+    if (ctx->cfg.track_ids && n > 0) {
+        if (pid) {
+            HIPCHK(ctx, hipMemcpyAsync(D.pid[stg], pid, n * sizeof(int64_t),
+                                       hipMemcpyHostToDevice, ctx->st));
+        } else {
+            std::vector<int64_t> ids(n);
+            std::iota(ids.begin(), ids.end(), (int64_t)0);
+            HIPCHK(ctx, hipMemcpyAsync(D.pid[stg], ids.data(), n * sizeof(int64_t),
+                                       hipMemcpyHostToDevice, ctx->st));
+            HIPCHK(ctx, hipStreamSynchronize(ctx->st));
+        }
+    }
+    HIPCHK(ctx, hipStreamSynchronize(ctx->st));
+    int rc = bin_from_staging(ctx, ispec, n);
+    if (rc) return rc;
+    ctx->rho_valid = false;
+    return build_work_items(ctx);
+}
+
+int ohpic_particle_count(ohpic_ctx *ctx, int ispec, int64_t *n)
+{
+    if (!ctx || !n) return fail(ctx, OHPIC_EINVAL, "NULL argument");
+    if (ispec < 0 || ispec >= ctx->cfg.nspecies)
+        return fail(ctx, OHPIC_EINVAL, "species %d out of range", ispec);
+    *n = ctx->sp[ispec].n;
+    return OHPIC_OK;
+}
+
+int ohpic_store_particles(ohpic_ctx *ctx, int ispec, int64_t cap, int64_t *n, double *x,
+                          double *y, double *z, double *vx, double *vy, double *vz,
+                          int64_t *pid)
+{
+    if (!ctx || !n) return fail(ctx, OHPIC_EINVAL, "NULL argument");
+    if (ispec < 0 || ispec >= ctx->cfg.nspecies)
+        return fail(ctx, OHPIC_EINVAL, "species %d out of range", ispec);
+    if (!ctx->binned) return fail(ctx, OHPIC_ESTATE, "particles pushed but not re-binned");
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    int64_t cnt = 0;
+    int rc = bins_to_staging(ctx, ispec, 0, &cnt);
+    if (rc) return rc;
+    *n = cnt;
+    if (cnt > cap) return fail(ctx, OHPIC_EINVAL, "buffer holds %lld, need %lld", (long long)cap,
+                               (long long)cnt);
+    const int stg = ctx->par ^ 1;
+    SpecDev &D = ctx->h.sp[ispec];
+    double *dst[6] = {x, y, z, vx, vy, vz};
+    for (int k = 0; k < 6 && cnt > 0; ++k)
+        if (dst[k])
+            HIPCHK(ctx, hipMemcpyAsync(dst[k], D.buf[stg][k], cnt * sizeof(double),
+                                       hipMemcpyDeviceToHost, ctx->st));
+    if (pid && cnt > 0) {
+        if (!ctx->cfg.track_ids) return fail(ctx, OHPIC_ESTATE, "ids not tracked");
+        HIPCHK(ctx, hipMemcpyAsync(pid, D.pid[stg], cnt * sizeof(int64_t), hipMemcpyDeviceToHost,
+                                   ctx->st));
+    }
+    HIPCHK(ctx, hipStreamSynchronize(ctx->st));
+    return OHPIC_OK;
+}
+
+int ohpic_load_oh_particles(ohpic_ctx *ctx, int64_t n, const ohpic_oh_particle *rec)
+{
+    if (!ctx) return fail(nullptr, OHPIC_EINVAL, "ctx is NULL");
+    if (n > 0 && !rec) return fail(ctx, OHPIC_EINVAL, "records are NULL");
+    const int ns = ctx->cfg.nspecies;
+    std::vector<int64_t> cnt(ns, 0);
+    for (int64_t i = 0; i < n; ++i) {
+        const int s = rec[i].spec - 1;
+        if (s < 0 || s >= ns)
+            return fail(ctx, OHPIC_EINVAL, "record %lld: spec %d out of range", (long long)i,
+                        rec[i].spec);
+        cnt[s]++;
+    }
+    for (int s = 0; s < ns; ++s) {
+        std::vector<double> a[6];
+        for (auto &v : a) v.reserve(cnt[s]);
+        std::vector<int64_t> ids;
+        ids.reserve(cnt[s]);
+        for (int64_t i = 0; i < n; ++i) {
+            if (rec[i].spec - 1 != s) continue;
+            a[0].push_back(rec[i].x);
+            a[1].push_back(rec[i].y);
+            a[2].push_back(rec[i].z);
+            a[3].push_back(rec[i].vx);
+            a[4].push_back(rec[i].vy);
+            a[5].push_back(rec[i].vz);
+            ids.push_back(rec[i].pid);
+        }
+        int rc = ohpic_load_particles(ctx, s, cnt[s], a[0].data(), a[1].data(), a[2].data(),
+                                      a[3].data(), a[4].data(), a[5].data(), ids.data());
+        if (rc) return rc;
+    }
+    return OHPIC_OK;
+}
+
+int ohpic_store_oh_particles(ohpic_ctx *ctx, int64_t cap, int64_t *n, ohpic_oh_particle *rec)
+{
+    if (!ctx || !n) return fail(ctx, OHPIC_EINVAL, "NULL argument");
+    int64_t tot = 0;
+    for (int s = 0; s < ctx->cfg.nspecies; ++s) tot += ctx->sp[s].n;
+    *n = tot;
+    if (tot > cap) return fail(ctx, OHPIC_EINVAL, "buffer holds %lld, need %lld", (long long)cap,
+                               (long long)tot);
+    int64_t o = 0;
+    for (int s = 0; s < ctx->cfg.nspecies; ++s) {
+        const int64_t m = ctx->sp[s].n;
+        std::vector<double> a[6];
+        for (auto &v : a) v.resize(m);
+        std::vector<int64_t> ids(ctx->cfg.track_ids ? m : 0);
+        int64_t got = 0;
+        int rc = ohpic_store_particles(ctx, s, m, &got, a[0].data(), a[1].data(), a[2].data(),
+                                       a[3].data(), a[4].data(), a[5].data(),
+                                       ctx->cfg.track_ids ? ids.data() : nullptr);
+        if (rc) return rc;
+        for (int64_t i = 0; i < got; ++i, ++o) {
+            rec[o].x = a[0][i];
+            rec[o].y = a[1][i];
+            rec[o].z = a[2][i];
+            rec[o].vx = a[3][i];
+            rec[o].vy = a[4][i];
+            rec[o].vz = a[5][i];
+            rec[o].pid = ctx->cfg.track_ids ? ids[i] : 0;
+            rec[o].nid = 0;
+            rec[o].spec = s + 1;
+        }
+    }
+    return OHPIC_OK;
+}
+
+int ohpic_seed_uniform_maxwell(ohpic_ctx *ctx, uint64_t initstate, uint64_t initseq,
+                               const int64_t *np, const double *vth)
+{
+    if (!ctx || !np || !vth) return fail(ctx, OHPIC_EINVAL, "NULL argument");
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    // pcg_setseq_64_srandom_r (pcg_variants.h:790-798) on the host
+    Pcg32Dev r0;
+    const uint64_t mult = 6364136223846793005ULL;
+    r0.state = 0u;
+    r0.inc = (initseq << 1u) | 1u;
+    r0.state = r0.state * mult + r0.inc;
+    r0.state += initstate;
+    r0.state = r0.state * mult + r0.inc;
+    uint64_t offset = 0;
+    for (int s = 0; s < ctx->cfg.nspecies; ++s) {
+        if (np[s] < 0 || np[s] > ctx->sp[s].capacity)
+            return fail(ctx, OHPIC_ENOMEM, "species %d: %lld particles exceed capacity", s,
+                        (long long)np[s]);
+        const int stg = ctx->par ^ 1;
+        SpecDev &D = ctx->h.sp[s];
+        // one random_fix draw, then 7 draws per particle
+        HIPCHK(ctx, launch_load_uniform(r0, offset + 1, np[s], ctx->n[0], ctx->n[1], ctx->n[2],
+                                        vth[s], (double *)D.buf[stg][0], (double *)D.buf[stg][1],
+                                        (double *)D.buf[stg][2], (double *)D.buf[stg][3],
+                                        (double *)D.buf[stg][4], (double *)D.buf[stg][5],
+                                        ctx->cfg.track_ids ? D.pid[stg] : nullptr, 0, ctx->st));
+        offset += 1 + 7 * (uint64_t)np[s];
+        int rc = bin_from_staging(ctx, s, np[s]);
+        if (rc) return rc;
+    }
+    ctx->rho_valid = false;
+    return build_work_items(ctx);
+}
+
+int ohpic_seed_hump_maxwell(ohpic_ctx *ctx, uint64_t initstate, uint64_t initseq,
+                            const int64_t *np, const double *vth, double amp, double sigma,
+                            uint64_t budget)
+{
+    if (!ctx || !np || !vth) return fail(ctx, OHPIC_EINVAL, "NULL argument");
+    if (budget < 8) return fail(ctx, OHPIC_EINVAL, "budget must be >= 8 draws");
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    Pcg32Dev r0;
+    const uint64_t mult = 6364136223846793005ULL;
+    r0.state = 0u;
+    r0.inc = (initseq << 1u) | 1u;
+    r0.state = r0.state * mult + r0.inc;
+    r0.state += initstate;
+    r0.state = r0.state * mult + r0.inc;
+    for (int s = 0; s < ctx->cfg.nspecies; ++s) {
+        if (np[s] < 0 || np[s] > ctx->sp[s].capacity)
+            return fail(ctx, OHPIC_ENOMEM, "species %d: %lld particles exceed capacity", s,
+                        (long long)np[s]);
+        const int stg = ctx->par ^ 1;
+        SpecDev &D = ctx->h.sp[s];
+        HIPCHK(ctx, launch_fill_u64(ctx->d_cursor, 1, 0ull, ctx->st));
+        HIPCHK(ctx, launch_load_hump(r0, (uint64_t)s << 40, budget, np[s], ctx->n[0], ctx->n[1],
+                                     ctx->n[2], amp, sigma, vth[s], (double *)D.buf[stg][0],
+                                     (double *)D.buf[stg][1], (double *)D.buf[stg][2],
+                                     (double *)D.buf[stg][3], (double *)D.buf[stg][4],
+                                     (double *)D.buf[stg][5],
+                                     ctx->cfg.track_ids ? D.pid[stg] : nullptr, 0,
+                                     ctx->d_cursor, ctx->st));
+        unsigned long long failed = 0;
+        HIPCHK(ctx, hipMemcpyAsync(&failed, ctx->d_cursor, sizeof failed, hipMemcpyDeviceToHost,
+                                   ctx->st));
+        HIPCHK(ctx, hipStreamSynchronize(ctx->st));
+        if (failed)
+            return fail(ctx, OHPIC_EINVAL, "rejection sampling exhausted %llu draw budgets",
+                        failed);
+        int rc = bin_from_staging(ctx, s, np[s]);
+        if (rc) return rc;
+    }
+    ctx->rho_valid = false;
+    return build_work_items(ctx);
+}
+
+int ohpic_deposit(ohpic_ctx *ctx)
+{
+    if (!ctx) return fail(nullptr, OHPIC_EINVAL, "ctx is NULL");
+    if (!ctx->binned)
+        return fail(ctx, OHPIC_ESTATE, "deposit needs binned particles: call boundary first");
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    int rc = do_deposit(ctx);
+    if (rc) return rc;
+    HIPCHK(ctx, launch_rho_from_acc(ctx->h.rho_acc[ctx->par], ctx->h.rho, ctx->ng, ctx->h.fix_inv,
+                                    ctx->st));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->st));
+    timers_collect(ctx);
+    return OHPIC_OK;
+}
+
+int ohpic_field_solve(ohpic_ctx *ctx)
+{
+    if (!ctx) return fail(nullptr, OHPIC_EINVAL, "ctx is NULL");
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    int rc = do_solve(ctx, false);
+    if (rc) return rc;
+    HIPCHK(ctx, hipStreamSynchronize(ctx->st));
+    timers_collect(ctx);
+    return OHPIC_OK;
+}
+
+int ohpic_gather_push(ohpic_ctx *ctx)
+{
+    if (!ctx) return fail(nullptr, OHPIC_EINVAL, "ctx is NULL");
+    if (!ctx->binned)
+        return fail(ctx, OHPIC_ESTATE, "gather_push needs binned particles: call boundary");
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    HIPCHK(ctx, launch_fill_f64(ctx->h.diag, 2 * kMaxSpecies, 0.0, ctx->st));
+    HIPCHK(ctx, launch_particles(kModeGatherPush, ctx->cfg.precision, ctx->cfg.weight_order,
+                                 ctx->T, ctx->d_inst, ctx->d_items, ctx->nitems, ctx->par,
+                                 ctx->st));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->st));
+    ctx->binned = false;
+    ctx->rho_valid = false;
+    return OHPIC_OK;
+}
+
+int ohpic_boundary(ohpic_ctx *ctx)
+{
+    if (!ctx) return fail(nullptr, OHPIC_EINVAL, "ctx is NULL");
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    int rc = zero_next_counts(ctx);
+    if (rc) return rc;
+    if ((rc = clear_flags(ctx))) return rc;
+    HIPCHK(ctx, launch_particles(kModeBoundary, ctx->cfg.precision, ctx->cfg.weight_order, ctx->T,
+                                 ctx->d_inst, ctx->d_items, ctx->nitems, ctx->par, ctx->st));
+    ctx->par ^= 1;
+    ctx->binned = true;
+    ctx->rho_valid = false;
+    return check_after_rebin(ctx);
+}
+
+int ohpic_step(ohpic_ctx *ctx, int nsteps)
+{
+    if (!ctx) return fail(nullptr, OHPIC_EINVAL, "ctx is NULL");
+    if (nsteps < 0) return fail(ctx, OHPIC_EINVAL, "nsteps < 0");
+    if (!ctx->binned)
+        return fail(ctx, OHPIC_ESTATE, "particles pushed but not re-binned: call boundary");
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    for (int i = 0; i < nsteps; ++i) {
+        int rc = one_step(ctx);
+        if (rc) return rc;
+    }
+    return OHPIC_OK;
+}
+
+int ohpic_synchronize(ohpic_ctx *ctx)
+{
+    if (!ctx) return fail(nullptr, OHPIC_EINVAL, "ctx is NULL");
+    HIPCHK(ctx, hipStreamSynchronize(ctx->st));
+    timers_collect(ctx);
+    return OHPIC_OK;
+}
+
+static int field_ptr(ohpic_ctx *ctx, int which, double **p)
+{
+    switch (which) {
+    case OHPIC_FIELD_RHO: *p = ctx->h.rho; return OHPIC_OK;
+    case OHPIC_FIELD_PHI: *p = ctx->h.phi; return OHPIC_OK;
+    case OHPIC_FIELD_EX: *p = ctx->h.e[0]; return OHPIC_OK;
+    case OHPIC_FIELD_EY: *p = ctx->h.e[1]; return OHPIC_OK;
+    case OHPIC_FIELD_EZ: *p = ctx->h.e[2]; return OHPIC_OK;
+    default: return fail(ctx, OHPIC_EINVAL, "unknown field %d", which);
+    }
+}
+
+int ohpic_get_field(ohpic_ctx *ctx, int which, double *values, int nelem, const int32_t lb[3],
+                    const int32_t ub[3], int comp)
+{
+    if (!ctx || !values || !lb || !ub) return fail(ctx, OHPIC_EINVAL, "NULL argument");
+    const bool eb = which == OHPIC_FIELD_EB;
+    if (nelem < (eb ? 6 : 1) || (!eb && (comp < 0 || comp >= nelem)))
+        return fail(ctx, OHPIC_EINVAL, "bad nelem/comp");
+    for (int a = 0; a < 3; ++a)
+        if (ub[a] < lb[a]) return fail(ctx, OHPIC_EINVAL, "ub < lb");
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    std::vector<double> g[3];
+    const int ncomp = eb ? 3 : 1;
+    for (int c = 0; c < ncomp; ++c) {
+        double *p = nullptr;
+        int rc = eb ? field_ptr(ctx, OHPIC_FIELD_EX + c, &p) : field_ptr(ctx, which, &p);
+        if (rc) return rc;
+        if ((rc = grid_copy_out(ctx, p, g[c]))) return rc;
+    }
+    const int nx = ctx->n[0], ny = ctx->n[1], nz = ctx->n[2];
+    const int64_t sx = ub[0] - lb[0] + 1, sy = ub[1] - lb[1] + 1;
+    for (int iz = lb[2]; iz <= ub[2]; ++iz) {
+        const int gz = ((iz % nz) + nz) % nz;
+        for (int iy = lb[1]; iy <= ub[1]; ++iy) {
+            const int gy = ((iy % ny) + ny) % ny;
+            for (int ix = lb[0]; ix <= ub[0]; ++ix) {
+                const int gx = ((ix % nx) + nx) % nx;
+                const int64_t gi = gx + (int64_t)nx * (gy + (int64_t)ny * gz);
+                const int64_t hi = (int64_t)nelem * ((ix - lb[0]) + sx * ((iy - lb[1]) +
+                                                                          sy * (int64_t)(iz - lb[2])));
+                if (eb) {
+                    for (int c = 0; c < 3; ++c) values[hi + c] = g[c][gi];
+                    for (int c = 0; c < 3; ++c) values[hi + 3 + c] = ctx->cfg.b0[c];
+                } else {
+                    values[hi + comp] = g[0][gi];
+                }
+            }
+        }
+    }
+    return OHPIC_OK;
+}
+
+int ohpic_put_field(ohpic_ctx *ctx, int which, const double *values, int nelem,
+                    const int32_t lb[3], const int32_t ub[3], int comp)
+{
+    if (!ctx || !values || !lb || !ub) return fail(ctx, OHPIC_EINVAL, "NULL argument");
+    if (which == OHPIC_FIELD_EB) return fail(ctx, OHPIC_EINVAL, "put EB: put EX/EY/EZ instead");
+    if (nelem < 1 || comp < 0 || comp >= nelem) return fail(ctx, OHPIC_EINVAL, "bad nelem/comp");
+    const int nx = ctx->n[0], ny = ctx->n[1], nz = ctx->n[2];
+    for (int a = 0; a < 3; ++a)
+        if (lb[a] > 0 || ub[a] < ctx->n[a] - 1)
+            return fail(ctx, OHPIC_EINVAL, "put: bounds must cover [0, n)");
+    double *p = nullptr;
+    int rc = field_ptr(ctx, which, &p);
+    if (rc) return rc;
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    std::vector<double> g(ctx->ng);
+    const int64_t sx = ub[0] - lb[0] + 1, sy = ub[1] - lb[1] + 1;
+    for (int iz = 0; iz < nz; ++iz)
+        for (int iy = 0; iy < ny; ++iy)
+            for (int ix = 0; ix < nx; ++ix) {
+                const int64_t hi = (int64_t)nelem * ((ix - lb[0]) + sx * ((iy - lb[1]) +
+                                                                          sy * (int64_t)(iz - lb[2])));
+                g[ix + (int64_t)nx * (iy + (int64_t)ny * iz)] = values[hi + comp];
+            }
+    HIPCHK(ctx, hipMemcpyAsync(p, g.data(), ctx->ng * sizeof(double), hipMemcpyHostToDevice,
+                               ctx->st));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->st));
+    return OHPIC_OK;
+}
+
+int ohpic_fft3d_r2r(int device, int nx, int ny, int nz, int backward, const double *in,
+                    double *out)
+{
+    if (!in || !out) return fail(nullptr, OHPIC_EINVAL, "NULL argument");
+    if (nx < 1 || ny < 1 || nz < 1) return fail(nullptr, OHPIC_EINVAL, "sizes must be >= 1");
+    ohpic_ctx tmp;  // holds allocations and the error string only
+    tmp.device = device;
+    if (hipSetDevice(device) != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(nullptr, OHPIC_ENODEV, "device %d unavailable", device);
+    }
+    const int64_t ng = (int64_t)nx * ny * nz;
+    int rc = OHPIC_OK;
+    FftAxisDev ax[3];
+    double *buf = nullptr;
+    const int n[3] = {nx, ny, nz};
+    hipStream_t st = nullptr;
+    auto cleanup = [&]() {
+        for (void *p : tmp.allocs) hipFree(p);
+        tmp.allocs.clear();
+        if (st) hipStreamDestroy(st);
+    };
+    if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) {
+        cleanup();
+        return fail(nullptr, OHPIC_EHIP, "stream creation failed");
+    }
+    for (int a = 0; a < 3 && !rc; ++a) rc = make_axis(&tmp, n[a], ax[a]);
+    if (!rc) rc = dalloc(&tmp, &buf, ng);
+    if (rc) {
+        std::string e = tmp.err;
+        cleanup();
+        return fail(nullptr, rc, "%s", e.c_str());
+    }
+    hipError_t e = hipMemcpyAsync(buf, in, ng * sizeof(double), hipMemcpyHostToDevice, st);
+    const double *nl = nullptr;
+    if (!backward) {
+        for (int a = 0; a < 3 && e == hipSuccess; ++a)
+            e = fft_axis_pass(kFftR2HC, a, buf, false, 1.0, buf, 1.0, nx, ny, nz, ax[a], nl, nl,
+                              nl, st);
+    } else {
+        for (int a = 2; a >= 0 && e == hipSuccess; --a)
+            e = fft_axis_pass(kFftHC2R, a, buf, false, 1.0, buf, a == 0 ? 1.0 / (double)ng : 1.0,
+                              nx, ny, nz, ax[a], nl, nl, nl, st);
+    }
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(out, buf, ng * sizeof(double), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    cleanup();
+    if (e != hipSuccess) return fail(nullptr, OHPIC_EHIP, "fft3d: %s", hipGetErrorString(e));
+    return OHPIC_OK;
+}
+
+int ohpic_diagnostics(ohpic_ctx *ctx, ohpic_diag *out)
+{
+    if (!ctx || !out) return fail(ctx, OHPIC_EINVAL, "NULL argument");
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    HIPCHK(ctx, hipMemcpyAsync(ctx->h_diag, ctx->h.diag, kDiagSlots * sizeof(double),
+                               hipMemcpyDeviceToHost, ctx->st));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->st));
+    std::memset(out, 0, sizeof *out);
+    out->step = ctx->step;
+    double ke = 0.0;
+    for (int s = 0; s < ctx->cfg.nspecies; ++s) {
+        out->ke_old[s] = ctx->h_diag[kDiagKeOld + s];
+        out->ke_new[s] = ctx->h_diag[kDiagKeNew + s];
+        ke += 0.5 * (out->ke_old[s] + out->ke_new[s]);
+    }
+    out->field_energy = 0.5 * ctx->h_diag[kDiagWe];
+    out->field_energy_e2 = 0.5 * ctx->h_diag[kDiagWe2];
+    out->net_charge = ctx->h_diag[kDiagQnet];
+    out->total_energy = ke + out->field_energy;
+    out->repacks = ctx->repacks;
+    out->fast_particles = ctx->fast;
+    return OHPIC_OK;
+}
+
+int ohpic_cell_counts(ohpic_ctx *ctx, int ispec, int32_t *counts)
+{
+    if (!ctx || !counts) return fail(ctx, OHPIC_EINVAL, "NULL argument");
+    if (ispec >= ctx->cfg.nspecies) return fail(ctx, OHPIC_EINVAL, "species out of range");
+    if (!ctx->binned) return fail(ctx, OHPIC_ESTATE, "particles pushed but not re-binned");
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    if (!ctx->d_cellcounts) {
+        int rc = dalloc(ctx, &ctx->d_cellcounts, ctx->ng);
+        if (rc) return rc;
+    }
+    HIPCHK(ctx, launch_fill_i32(ctx->d_cellcounts, ctx->ng, 0, ctx->st));
+    for (int s = 0; s < ctx->cfg.nspecies; ++s) {
+        if (ispec >= 0 && s != ispec) continue;
+        HIPCHK(ctx, launch_cell_counts(ctx->d_inst, 0, s, ctx->par, ctx->ntiles, ctx->d_cellcounts,
+                                       ctx->st));
+    }
+    HIPCHK(ctx, hipMemcpyAsync(counts, ctx->d_cellcounts, ctx->ng * sizeof(int32_t),
+                               hipMemcpyDeviceToHost, ctx->st));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->st));
+    return OHPIC_OK;
+}
+
+int ohpic_enable_timing(ohpic_ctx *ctx, int enable)
+{
+    if (!ctx) return fail(nullptr, OHPIC_EINVAL, "ctx is NULL");
+    timers_collect(ctx);
+    ctx->timing = enable != 0;
+    for (auto &t : ctx->timers) {
+        t.total_ms = 0.0;
+        t.launches = 0;
+    }
+    return OHPIC_OK;
+}
+
+int ohpic_kernel_times(ohpic_ctx *ctx, int n, char names[][32], double *total_ms,
+                       int64_t *launches, int *count)
+{
+    if (!ctx || !count) return fail(ctx, OHPIC_EINVAL, "NULL argument");
+    HIPCHK(ctx, hipStreamSynchronize(ctx->st));
+    timers_collect(ctx);
+    int k = 0;
+    for (; k < kNumTimers && k < n; ++k) {
+        if (names) {
+            std::strncpy(names[k], kTimerNames[k], 31);
+            names[k][31] = 0;
+        }
+        if (total_ms) total_ms[k] = ctx->timers[k].total_ms;
+        if (launches) launches[k] = ctx->timers[k].launches;
+    }
+    *count = k;
+    return OHPIC_OK;
+}
+
+int ohpic_memory_footprint(ohpic_ctx *ctx, int64_t *bytes)
+{
+    if (!ctx || !bytes) return fail(ctx, OHPIC_EINVAL, "NULL argument");
+    *bytes = ctx->bytes;
+    return OHPIC_OK;
+}
+
+}  // extern "C"

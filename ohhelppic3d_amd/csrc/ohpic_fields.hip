@@ -1,0 +1,280 @@
+// ohpic_fields.hip -- grid kernels (fills, charge conversion, node-centred E
+// with energy diagnostics) and the seeded particle loaders for gfx950.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "ohpic_internal.hpp"
+#include "ohpic_kernels.hpp"
+
+namespace ohpic {
+
+namespace {
+
+constexpr double kRefPi = 3.14159265358979;  // science_constants.f90:4
+
+template <typename T>
+__global__ void k_fill(T *__restrict__ p, int64_t n, T v)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+__global__ void k_rho_from_acc(const unsigned long long *__restrict__ acc, double *__restrict__ rho,
+                               int64_t n, double scale)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        rho[i] = (double)(long long)acc[i] * scale;
+}
+
+__device__ __forceinline__ double wave_sum(double v)
+{
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    return v;
+}
+
+// E_a(i) = 0.5*((phi(i-e_a) - phi(i)) + (phi(i) - phi(i+e_a))): the edge
+// difference of poisson_field_solver.f90:139-144 averaged onto nodes as at
+// :165-170, periodic.  Also accumulates sum rho*phi, sum |E|^2 and sum rho.
+__global__ void __launch_bounds__(256)
+    k_efield(const double *__restrict__ phi, const double *__restrict__ rho,
+             double *__restrict__ ex, double *__restrict__ ey, double *__restrict__ ez, int nx,
+             int ny, int nz, double *__restrict__ diag)
+{
+    __shared__ double red[3][4];
+    const int64_t ng = (int64_t)nx * ny * nz;
+    double we = 0.0, we2 = 0.0, qn = 0.0;
+    for (int64_t o = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; o < ng;
+         o += (int64_t)gridDim.x * blockDim.x) {
+        const int ix = (int)(o % nx);
+        const int64_t r = o / nx;
+        const int iy = (int)(r % ny);
+        const int iz = (int)(r / ny);
+        const int64_t sx = 1, sy = nx, sz = (int64_t)nx * ny;
+        const int64_t oxp = ix + 1 < nx ? o + sx : o - (int64_t)(nx - 1) * sx;
+        const int64_t oxm = ix > 0 ? o - sx : o + (int64_t)(nx - 1) * sx;
+        const int64_t oyp = iy + 1 < ny ? o + sy : o - (int64_t)(ny - 1) * sy;
+        const int64_t oym = iy > 0 ? o - sy : o + (int64_t)(ny - 1) * sy;
+        const int64_t ozp = iz + 1 < nz ? o + sz : o - (int64_t)(nz - 1) * sz;
+        const int64_t ozm = iz > 0 ? o - sz : o + (int64_t)(nz - 1) * sz;
+        const double p0 = phi[o];
+        const double vx = 0.5 * ((phi[oxm] - p0) + (p0 - phi[oxp]));
+        const double vy = 0.5 * ((phi[oym] - p0) + (p0 - phi[oyp]));
+        const double vz = 0.5 * ((phi[ozm] - p0) + (p0 - phi[ozp]));
+        ex[o] = vx;
+        ey[o] = vy;
+        ez[o] = vz;
+        const double q = rho[o];
+        we += q * p0;
+        we2 += vx * vx + vy * vy + vz * vz;
+        qn += q;
+    }
+    we = wave_sum(we);
+    we2 = wave_sum(we2);
+    qn = wave_sum(qn);
+    const int w = threadIdx.x >> 6;
+    if (__lane_id() == 0) {
+        red[0][w] = we;
+        red[1][w] = we2;
+        red[2][w] = qn;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double a = 0, b = 0, c = 0;
+        for (int i = 0; i < 4; ++i) {
+            a += red[0][i];
+            b += red[1][i];
+            c += red[2][i];
+        }
+        atomicAdd(&diag[kDiagWe], a);
+        atomicAdd(&diag[kDiagWe2], b);
+        atomicAdd(&diag[kDiagQnet], c);
+    }
+}
+
+// ---------------------------------------------------------------- PCG32
+// lib/pcg-c: pcg_setseq_64 step (pcg_variants.h:588-591), XSH-RR output
+// (:156-159), advance (pcg-advance-64.c:44-59), double = ldexp(u32, -32)
+// (pcg32_double.c:60-63).
+constexpr uint64_t kPcgMult = 6364136223846793005ULL;
+
+__device__ __forceinline__ uint32_t pcg_next(Pcg32Dev &r)
+{
+    const uint64_t old = r.state;
+    r.state = old * kPcgMult + r.inc;
+    const uint32_t xs = (uint32_t)(((old >> 18u) ^ old) >> 27u);
+    const uint32_t rot = (uint32_t)(old >> 59u);
+    return (xs >> rot) | (xs << ((-rot) & 31u));
+}
+
+__device__ __forceinline__ double pcg_double(Pcg32Dev &r)
+{
+    return (double)pcg_next(r) * 2.3283064365386962890625e-10;  // 2^-32
+}
+
+__device__ __forceinline__ void pcg_advance(Pcg32Dev &r, uint64_t delta)
+{
+    uint64_t cm = kPcgMult, cp = r.inc, am = 1u, ap = 0u;
+    while (delta > 0) {
+        if (delta & 1u) {
+            am *= cm;
+            ap = ap * cm + cp;
+        }
+        cp = (cm + 1u) * cp;
+        cm *= cm;
+        delta >>= 1u;
+    }
+    r.state = am * r.state + ap;
+}
+
+__device__ __forceinline__ void maxwell3(Pcg32Dev &r, double vth, double &v1, double &v2,
+                                         double &v3)
+{
+    // four uniforms in the order of velocity_maxwell_distribution3d.f90:46-56;
+    // std = vth and log(1-u)
+    const double r1 = pcg_double(r), r2 = pcg_double(r), r3 = pcg_double(r),
+                 r4 = pcg_double(r);
+    const double a = vth * sqrt(-2.0 * log(1.0 - r1));
+    const double b = vth * sqrt(-2.0 * log(1.0 - r3));
+    v1 = a * cos(2.0 * kRefPi * r2);
+    v2 = a * sin(2.0 * kRefPi * r2);
+    v3 = b * sin(2.0 * kRefPi * r4);
+}
+
+__global__ void k_load_uniform(Pcg32Dev rng0, uint64_t offset, int64_t np, int nx, int ny,
+                               int nz, double vth, double *__restrict__ x, double *__restrict__ y,
+                               double *__restrict__ z, double *__restrict__ vx,
+                               double *__restrict__ vy, double *__restrict__ vz,
+                               int64_t *__restrict__ pid, int64_t pid0)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= np) return;
+    Pcg32Dev r = rng0;
+    pcg_advance(r, offset + 7u * (uint64_t)i);
+    // position_random_uniform_distribution.f90:41 with lower = 0, upper = n
+    x[i] = pcg_double(r) * (double)nx + 0.0;
+    y[i] = pcg_double(r) * (double)ny + 0.0;
+    z[i] = pcg_double(r) * (double)nz + 0.0;
+    maxwell3(r, vth, vx[i], vy[i], vz[i]);
+    if (pid) pid[i] = pid0 + i;
+}
+
+__global__ void k_load_hump(Pcg32Dev rng0, uint64_t offset, uint64_t budget, int64_t np, int nx,
+                            int ny, int nz, double amp, double sigma, double vth,
+                            double *__restrict__ x, double *__restrict__ y,
+                            double *__restrict__ z, double *__restrict__ vx,
+                            double *__restrict__ vy, double *__restrict__ vz,
+                            int64_t *__restrict__ pid, int64_t pid0,
+                            unsigned long long *__restrict__ fail)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= np) return;
+    Pcg32Dev r = rng0;
+    pcg_advance(r, offset + (uint64_t)i * budget);
+    const double cx = 0.5 * nx, cy = 0.5 * ny, cz = 0.5 * nz;
+    const double inv2s2 = 1.0 / (2.0 * sigma * sigma);
+    const uint64_t maxc = (budget - 4u) / 4u;
+    bool ok = false;
+    for (uint64_t c = 0; c < maxc; ++c) {
+        const double px = pcg_double(r) * (double)nx;
+        const double py = pcg_double(r) * (double)ny;
+        const double pz = pcg_double(r) * (double)nz;
+        const double u = pcg_double(r);
+        const double dx = px - cx, dy = py - cy, dz = pz - cz;
+        const double r2 = dx * dx + dy * dy + dz * dz;
+        const double f = (1.0 + amp * exp(-r2 * inv2s2)) / (1.0 + amp);
+        if (u < f) {
+            x[i] = px;
+            y[i] = py;
+            z[i] = pz;
+            ok = true;
+            break;
+        }
+    }
+    if (!ok) {
+        atomicAdd(fail, 1ull);
+        x[i] = 0.5 * nx;
+        y[i] = 0.5 * ny;
+        z[i] = 0.5 * nz;
+    }
+    maxwell3(r, vth, vx[i], vy[i], vz[i]);
+    if (pid) pid[i] = pid0 + i;
+}
+
+}  // namespace
+
+static unsigned grid_for(int64_t n)
+{
+    int64_t b = (n + 255) / 256;
+    if (b > 65536) b = 65536;
+    if (b < 1) b = 1;
+    return (unsigned)b;
+}
+
+hipError_t launch_fill_u64(unsigned long long *p, int64_t n, unsigned long long v, hipStream_t st)
+{
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fill<unsigned long long>, dim3(grid_for(n)), dim3(256), 0, st, p, n, v);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_i32(int32_t *p, int64_t n, int32_t v, hipStream_t st)
+{
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fill<int32_t>, dim3(grid_for(n)), dim3(256), 0, st, p, n, v);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_f64(double *p, int64_t n, double v, hipStream_t st)
+{
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fill<double>, dim3(grid_for(n)), dim3(256), 0, st, p, n, v);
+    return hipGetLastError();
+}
+
+hipError_t launch_rho_from_acc(const unsigned long long *acc, double *rho, int64_t n,
+                               double scale, hipStream_t st)
+{
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_rho_from_acc, dim3(grid_for(n)), dim3(256), 0, st, acc, rho, n, scale);
+    return hipGetLastError();
+}
+
+hipError_t launch_efield(const double *phi, const double *rho, double *ex, double *ey,
+                         double *ez, int nx, int ny, int nz, double *diag, hipStream_t st)
+{
+    const int64_t n = (int64_t)nx * ny * nz;
+    hipLaunchKernelGGL(k_efield, dim3(grid_for(n)), dim3(256), 0, st, phi, rho, ex, ey, ez, nx,
+                       ny, nz, diag);
+    return hipGetLastError();
+}
+
+hipError_t launch_load_uniform(Pcg32Dev rng0, uint64_t offset, int64_t np, int nx, int ny,
+                               int nz, double vth, double *x, double *y, double *z, double *vx,
+                               double *vy, double *vz, int64_t *pid, int64_t pid0,
+                               hipStream_t st)
+{
+    if (np <= 0) return hipSuccess;
+    const int64_t blocks = (np + 255) / 256;
+    hipLaunchKernelGGL(k_load_uniform, dim3((unsigned)blocks), dim3(256), 0, st, rng0, offset,
+                       np, nx, ny, nz, vth, x, y, z, vx, vy, vz, pid, pid0);
+    return hipGetLastError();
+}
+
+hipError_t launch_load_hump(Pcg32Dev rng0, uint64_t offset, uint64_t budget, int64_t np, int nx,
+                            int ny, int nz, double amp, double sigma, double vth, double *x,
+                            double *y, double *z, double *vx, double *vy, double *vz,
+                            int64_t *pid, int64_t pid0, unsigned long long *fail,
+                            hipStream_t st)
+{
+    if (np <= 0) return hipSuccess;
+    const int64_t blocks = (np + 255) / 256;
+    hipLaunchKernelGGL(k_load_hump, dim3((unsigned)blocks), dim3(256), 0, st, rng0, offset,
+                       budget, np, nx, ny, nz, amp, sigma, vth, x, y, z, vx, vy, vz, pid, pid0,
+                       fail);
+    return hipGetLastError();
+}
+
+}  // namespace ohpic

@@ -1,0 +1,60 @@
+// ohpic_kernels.hpp -- host-callable launchers of the gfx950 kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "ohpic_internal.hpp"
+
+namespace ohpic {
+
+enum ParticleMode : int {
+    kModeFused = 0,       // gather + push + wrap + deposit(new x) + re-bin
+    kModeDeposit = 1,     // deposit current positions
+    kModeGatherPush = 2,  // gather + push in place (no wrap, no re-bin)
+    kModeBoundary = 3     // wrap + re-bin
+};
+
+bool tile_supported(int tx, int ty, int tz);
+
+hipError_t launch_particles(int mode, int precision, int weight_order, const int T[3],
+                            const InstDev *insts, const WorkItem *items, int nitems, int par,
+                            hipStream_t st);
+hipError_t launch_bin_count(const InstDev *insts, int inst, int s, int nbuf, int64_t n,
+                            const int T[3], hipStream_t st);
+hipError_t launch_bin_scatter(const InstDev *insts, int inst, int s, int nbuf, int64_t n,
+                              const int T[3], hipStream_t st);
+hipError_t launch_bins_to_list(const InstDev *insts, int inst, int s, int src, int dst,
+                               int ntiles, const int64_t *obase, const int32_t *ocap,
+                               const int32_t *ocount, const int64_t *list_off, hipStream_t st);
+hipError_t launch_ovf_to_list(const InstDev *insts, int inst, int s, int dst, int64_t nov,
+                              int64_t off, unsigned long long *cursor, hipStream_t st);
+hipError_t launch_cell_counts(const InstDev *insts, int inst, int s, int par, int ntiles,
+                              int32_t *counts, hipStream_t st);
+
+// fields (ohpic_fields.hip)
+hipError_t launch_fill_u64(unsigned long long *p, int64_t n, unsigned long long v,
+                           hipStream_t st);
+hipError_t launch_fill_i32(int32_t *p, int64_t n, int32_t v, hipStream_t st);
+hipError_t launch_fill_f64(double *p, int64_t n, double v, hipStream_t st);
+hipError_t launch_rho_from_acc(const unsigned long long *acc, double *rho, int64_t n,
+                               double scale, hipStream_t st);
+hipError_t launch_efield(const double *phi, const double *rho, double *ex, double *ey,
+                         double *ez, int nx, int ny, int nz, double *diag, hipStream_t st);
+
+// seeded loaders (ohpic_fields.hip)
+struct Pcg32Dev {
+    uint64_t state, inc;
+};
+hipError_t launch_load_uniform(Pcg32Dev rng0, uint64_t offset, int64_t np, int nx, int ny,
+                               int nz, double vth, double *x, double *y, double *z, double *vx,
+                               double *vy, double *vz, int64_t *pid, int64_t pid0,
+                               hipStream_t st);
+hipError_t launch_load_hump(Pcg32Dev rng0, uint64_t offset, uint64_t budget, int64_t np, int nx,
+                            int ny, int nz, double amp, double sigma, double vth, double *x,
+                            double *y, double *z, double *vx, double *vy, double *vz,
+                            int64_t *pid, int64_t pid0, unsigned long long *fail,
+                            hipStream_t st);
+
+}  // namespace ohpic

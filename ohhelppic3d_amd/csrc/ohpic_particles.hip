@@ -1,0 +1,553 @@
+// ohpic_particles.hip -- particle kernels for gfx950 (FP64 storage).
+//
+// One workgroup processes one work item = (instance, tile, chunk): it stages
+// the node-centred E of the tile plus a one-node halo in LDS, walks its slice
+// of every species' bin, and for each particle
+//   gathers E with CIC weights   (linear_interpolator.f90:22-69),
+//   runs the Boris rotation + drift (particle_mover_boris.f90:21-64),
+//   applies the periodic wrap twice (periodic_particle_boundary.f90:42-53,
+//                                    ohhelppic3d.f90:353-355),
+//   deposits q*W(x_new) into an int64 fixed-point LDS copy of the tile's
+//   nodes plus halo (linear_scatter.f90:22-55), and
+//   appends the particle to the bin of its new tile in the other buffer
+//   (the replacement of OhHelp's correct_particle/transbound migration).
+// The LDS charge block is flushed with one global 64-bit atomic per node.
+// Integer accumulation makes rho independent of particle order and timing.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "ohpic_internal.hpp"
+#include "ohpic_kernels.hpp"
+
+namespace ohpic {
+
+namespace {
+
+template <int WO>
+__device__ __forceinline__ void cic_w(double r, double &w0, double &w1)
+{
+    // WO 0: lower node gets r (the reference's order); WO 1: lower gets 1-r
+    if (WO == 0) {
+        w0 = r;
+        w1 = 1.0 - r;
+    } else {
+        w0 = 1.0 - r;
+        w1 = r;
+    }
+}
+
+__device__ __forceinline__ double apply_periodic(double p, int n)
+{
+    const double dn = (double)n;
+    if (p < 0) p = dn + p;
+    if (p >= dn) p = p - dn;
+    return p;
+}
+
+__device__ __forceinline__ int lane_id() { return __lane_id(); }
+
+__device__ __forceinline__ double block_sum(double v, double *red)
+{
+    // wave64 reduction then across the 4 waves
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    const int w = threadIdx.x >> 6;
+    if (lane_id() == 0) red[w] = v;
+    __syncthreads();
+    double s = 0.0;
+    if (threadIdx.x == 0)
+        for (int i = 0; i < kBlock / 64; ++i) s += red[i];
+    __syncthreads();
+    return s;
+}
+
+struct P64 {
+    double x, y, z, vx, vy, vz;
+    int64_t id;
+};
+
+__device__ __forceinline__ void write_p64(const InstDev *__restrict__ I, const SpecDev &S, int s,
+                                          int nbuf, int t, int slot, const P64 &p, bool ids)
+{
+    if (slot < S.bin_cap[t]) {
+        const int64_t o = S.bin_base[t] + slot;
+        ((double *)S.buf[nbuf][0])[o] = p.x;
+        ((double *)S.buf[nbuf][1])[o] = p.y;
+        ((double *)S.buf[nbuf][2])[o] = p.z;
+        ((double *)S.buf[nbuf][3])[o] = p.vx;
+        ((double *)S.buf[nbuf][4])[o] = p.vy;
+        ((double *)S.buf[nbuf][5])[o] = p.vz;
+        if (ids) S.pid[nbuf][o] = p.id;
+    } else {
+        const unsigned long long o = atomicAdd(&I->flags[kFlagOverflow], 1ull);
+        if ((int64_t)o < I->ovf.cap) {
+            ((double *)I->ovf.comp[0])[o] = p.x;
+            ((double *)I->ovf.comp[1])[o] = p.y;
+            ((double *)I->ovf.comp[2])[o] = p.z;
+            ((double *)I->ovf.comp[3])[o] = p.vx;
+            ((double *)I->ovf.comp[4])[o] = p.vy;
+            ((double *)I->ovf.comp[5])[o] = p.vz;
+            if (ids) I->ovf.pid[o] = p.id;
+            I->ovf.dest[o] = t;
+            I->ovf.spec[o] = s;
+        } else {
+            atomicAdd(&I->flags[kFlagLost], 1ull);
+        }
+    }
+}
+
+template <int TX, int TY, int TZ, int WO, int MODE>
+__global__ void __launch_bounds__(kBlock)
+    k_particles_fp64(const InstDev *__restrict__ insts, const WorkItem *__restrict__ items,
+                     int par)
+{
+    constexpr bool PUSH = (MODE == kModeFused || MODE == kModeGatherPush);
+    constexpr bool REBIN = (MODE == kModeFused || MODE == kModeBoundary);
+    constexpr bool DEP = (MODE == kModeFused || MODE == kModeDeposit);
+    constexpr int EX = TX + 1, EY = TY + 1, EZ = TZ + 1, NE = EX * EY * EZ;
+    constexpr int RX = TX + 3, RY = TY + 3, RZ = TZ + 3, NR = RX * RY * RZ;
+    __shared__ double sE[PUSH ? 3 * NE : 1];
+    __shared__ unsigned long long sR[DEP ? NR : 1];
+    __shared__ double sRed[kBlock / 64];
+
+    const WorkItem it = items[blockIdx.x];
+    const InstDev *__restrict__ I = insts + it.inst;
+    const int nx = I->n[0], ny = I->n[1], nz = I->n[2];
+    const int ntx = I->nt[0], nty = I->nt[1];
+    const int tile = it.tile;
+    const int tx = tile % ntx, ty = (tile / ntx) % nty, tz = tile / (ntx * nty);
+    const int lox = tx * TX, loy = ty * TY, loz = tz * TZ;
+    const int tid = threadIdx.x;
+    const bool ids = I->track_ids != 0;
+
+    if (PUSH) {
+        const double *__restrict__ ex = I->e[0];
+        const double *__restrict__ ey = I->e[1];
+        const double *__restrict__ ez = I->e[2];
+        for (int l = tid; l < NE; l += kBlock) {
+            const int lx = l % EX, ly = (l / EX) % EY, lz = l / (EX * EY);
+            const int gx = (lox + lx) % nx, gy = (loy + ly) % ny, gz = (loz + lz) % nz;
+            const int64_t g = gx + (int64_t)nx * (gy + (int64_t)ny * gz);
+            sE[l] = ex[g];
+            sE[NE + l] = ey[g];
+            sE[2 * NE + l] = ez[g];
+        }
+    }
+    if (DEP)
+        for (int l = tid; l < NR; l += kBlock) sR[l] = 0ull;
+    __syncthreads();
+
+    const double dt = I->dt;
+    const double dt2 = dt * 0.5;
+    const double bx = I->b0[0], by = I->b0[1], bz = I->b0[2];
+    const int nbuf = par ^ 1;
+    const unsigned long long lanemask_lt = (1ull << lane_id()) - 1ull;
+    unsigned long long *__restrict__ racc = I->rho_acc[MODE == kModeFused ? nbuf : par];
+
+    for (int s = 0; s < I->nspecies; ++s) {
+        const SpecDev &S = I->sp[s];
+        const int cap = S.bin_cap[tile];
+        const int cnt = min(S.count[par][tile], cap);
+        const int64_t base = S.bin_base[tile];
+        int per = (cnt + it.nchunks - 1) / it.nchunks;
+        per = (per + 63) & ~63;
+        const int start = it.chunk * per;
+        const int end = min(cnt, start + per);
+        const double *__restrict__ cx = (const double *)S.buf[par][0];
+        const double *__restrict__ cy = (const double *)S.buf[par][1];
+        const double *__restrict__ cz = (const double *)S.buf[par][2];
+        const double *__restrict__ cvx = (const double *)S.buf[par][3];
+        const double *__restrict__ cvy = (const double *)S.buf[par][4];
+        const double *__restrict__ cvz = (const double *)S.buf[par][5];
+        const double qm = S.qm;
+        const double qfix = S.qfix;
+        // Boris constants for the uniform B (t = qm B dt/2, s = 2t/(1+|t|^2))
+        const double t0 = qm * bx * dt2, t1 = qm * by * dt2, t2 = qm * bz * dt2;
+        const double tt = t0 * t0 + t1 * t1 + t2 * t2;
+        const double s0 = 2 * t0 / (1 + tt), s1 = 2 * t1 / (1 + tt), s2 = 2 * t2 / (1 + tt);
+        double keo = 0.0, ken = 0.0;
+
+        for (int p0 = start; p0 < end; p0 += kBlock) {
+            const int p = p0 + tid;
+            const bool active = p < end;
+            P64 q{0, 0, 0, 0, 0, 0, 0};
+            if (active) {
+                const int64_t o = base + p;
+                q.x = cx[o];
+                q.y = cy[o];
+                q.z = cz[o];
+                q.vx = cvx[o];
+                q.vy = cvy[o];
+                q.vz = cvz[o];
+                if (ids) q.id = S.pid[par][o];
+            }
+            if (PUSH && active) {
+                const int ix = (int)q.x, iy = (int)q.y, iz = (int)q.z;
+                double wx0, wx1, wy0, wy1, wz0, wz1;
+                cic_w<WO>(q.x - (double)ix, wx0, wx1);
+                cic_w<WO>(q.y - (double)iy, wy0, wy1);
+                cic_w<WO>(q.z - (double)iz, wz0, wz1);
+                // binned particles lie in the tile; the clamp only guards LDS
+                const int lx = min(max(ix - lox, 0), TX - 1);
+                const int ly = min(max(iy - loy, 0), TY - 1);
+                const int lz = min(max(iz - loz, 0), TZ - 1);
+                const int l0 = lx + EX * (ly + EY * lz);
+                double e0 = 0.0, e1 = 0.0, e2 = 0.0;
+                const double wx[2] = {wx0, wx1}, wy[2] = {wy0, wy1}, wz[2] = {wz0, wz1};
+#pragma unroll
+                for (int i = 0; i < 2; ++i)
+#pragma unroll
+                    for (int j = 0; j < 2; ++j)
+#pragma unroll
+                        for (int k = 0; k < 2; ++k) {
+                            const int l = l0 + i + EX * (j + EY * k);
+                            const double w = wx[i] * wy[j] * wz[k];
+                            e0 += sE[l] * w;
+                            e1 += sE[NE + l] * w;
+                            e2 += sE[2 * NE + l] * w;
+                        }
+                keo += q.vx * q.vx + q.vy * q.vy + q.vz * q.vz;
+                // Boris (particle_mover_boris.f90:42-62)
+                const double um0 = q.vx + qm * e0 * dt2;
+                const double um1 = q.vy + qm * e1 * dt2;
+                const double um2 = q.vz + qm * e2 * dt2;
+                const double ua0 = um0 + (um1 * t2 - um2 * t1);
+                const double ua1 = um1 + (um2 * t0 - um0 * t2);
+                const double ua2 = um2 + (um0 * t1 - um1 * t0);
+                const double up0 = um0 + (ua1 * s2 - ua2 * s1);
+                const double up1 = um1 + (ua2 * s0 - ua0 * s2);
+                const double up2 = um2 + (ua0 * s1 - ua1 * s0);
+                q.vx = up0 + qm * e0 * dt2;
+                q.vy = up1 + qm * e1 * dt2;
+                q.vz = up2 + qm * e2 * dt2;
+                ken += q.vx * q.vx + q.vy * q.vy + q.vz * q.vz;
+                q.x = q.x + q.vx * dt;
+                q.y = q.y + q.vy * dt;
+                q.z = q.z + q.vz * dt;
+            }
+            if (MODE == kModeGatherPush) {
+                if (active) {
+                    const int64_t o = base + p;
+                    double *wx_ = (double *)S.buf[par][0];
+                    double *wy_ = (double *)S.buf[par][1];
+                    double *wz_ = (double *)S.buf[par][2];
+                    double *wvx = (double *)S.buf[par][3];
+                    double *wvy = (double *)S.buf[par][4];
+                    double *wvz = (double *)S.buf[par][5];
+                    wx_[o] = q.x;
+                    wy_[o] = q.y;
+                    wz_[o] = q.z;
+                    wvx[o] = q.vx;
+                    wvy[o] = q.vy;
+                    wvz[o] = q.vz;
+                }
+                continue;
+            }
+            int ntile = tile;
+            if (REBIN && active) {
+                q.x = apply_periodic(apply_periodic(q.x, nx), nx);
+                q.y = apply_periodic(apply_periodic(q.y, ny), ny);
+                q.z = apply_periodic(apply_periodic(q.z, nz), nz);
+                if (!(q.x >= 0.0 && q.x < nx && q.y >= 0.0 && q.y < ny && q.z >= 0.0 &&
+                      q.z < nz)) {
+                    atomicAdd(&I->flags[kFlagRange], 1ull);
+                    q.x = fmin(fmax(q.x - floor(q.x / nx) * nx, 0.0), nx - 1e-9);
+                    q.y = fmin(fmax(q.y - floor(q.y / ny) * ny, 0.0), ny - 1e-9);
+                    q.z = fmin(fmax(q.z - floor(q.z / nz) * nz, 0.0), nz - 1e-9);
+                }
+                const int cxi = (int)q.x, cyi = (int)q.y, czi = (int)q.z;
+                ntile = cxi / TX + ntx * (cyi / TY + nty * (czi / TZ));
+            }
+            if (DEP && active) {
+                const int ix = (int)q.x, iy = (int)q.y, iz = (int)q.z;
+                double wx0, wx1, wy0, wy1, wz0, wz1;
+                cic_w<WO>(q.x - (double)ix, wx0, wx1);
+                cic_w<WO>(q.y - (double)iy, wy0, wy1);
+                cic_w<WO>(q.z - (double)iz, wz0, wz1);
+                const double wx[2] = {wx0, wx1}, wy[2] = {wy0, wy1}, wz[2] = {wz0, wz1};
+                const int dx = wrap_index(ix - lox + 1, nx);
+                const int dy = wrap_index(iy - loy + 1, ny);
+                const int dz = wrap_index(iz - loz + 1, nz);
+                if (dx <= TX + 1 && dy <= TY + 1 && dz <= TZ + 1) {
+                    const int l0 = dx + RX * (dy + RY * dz);
+#pragma unroll
+                    for (int i = 0; i < 2; ++i)
+#pragma unroll
+                        for (int j = 0; j < 2; ++j)
+#pragma unroll
+                            for (int k = 0; k < 2; ++k) {
+                                const long long v = fix_round(wx[i] * wy[j] * wz[k], qfix);
+                                atomicAdd(&sR[l0 + i + RX * (j + RY * k)],
+                                          (unsigned long long)v);
+                            }
+                } else {
+                    atomicAdd(&I->flags[kFlagFast], 1ull);
+#pragma unroll
+                    for (int i = 0; i < 2; ++i)
+#pragma unroll
+                        for (int j = 0; j < 2; ++j)
+#pragma unroll
+                            for (int k = 0; k < 2; ++k) {
+                                const int gx = (ix + i) % nx, gy = (iy + j) % ny,
+                                          gz = (iz + k) % nz;
+                                const long long v = fix_round(wx[i] * wy[j] * wz[k], qfix);
+                                atomicAdd(&racc[gx + (int64_t)nx * (gy + (int64_t)ny * gz)],
+                                          (unsigned long long)v);
+                            }
+                }
+            }
+            if (REBIN) {
+                const bool stay = active && ntile == tile;
+                const unsigned long long m = __ballot(stay);
+                if (m) {
+                    const int leader = __ffsll((unsigned long long)m) - 1;
+                    int b = 0;
+                    if (lane_id() == leader)
+                        b = atomicAdd(&S.count[nbuf][tile], (int)__popcll(m));
+                    b = __shfl(b, leader, 64);
+                    if (stay) {
+                        const int slot = b + (int)__popcll(m & lanemask_lt);
+                        write_p64(I, S, s, nbuf, tile, slot, q, ids);
+                    }
+                }
+                if (active && ntile != tile) {
+                    const int slot = atomicAdd(&S.count[nbuf][ntile], 1);
+                    write_p64(I, S, s, nbuf, ntile, slot, q, ids);
+                }
+            }
+        }
+        if (PUSH) {
+            const double so = block_sum(keo, sRed);
+            const double sn = block_sum(ken, sRed);
+            if (tid == 0) {
+                atomicAdd(&I->diag[kDiagKeOld + s], 0.5 * S.mass * so);
+                atomicAdd(&I->diag[kDiagKeNew + s], 0.5 * S.mass * sn);
+            }
+        }
+    }
+
+    if (DEP) {
+        __syncthreads();
+        for (int l = tid; l < NR; l += kBlock) {
+            const unsigned long long v = sR[l];
+            if (v == 0ull) continue;
+            const int lx = l % RX, ly = (l / RX) % RY, lz = l / (RX * RY);
+            const int gx = ((lox - 1 + lx) % nx + nx) % nx;
+            const int gy = ((loy - 1 + ly) % ny + ny) % ny;
+            const int gz = ((loz - 1 + lz) % nz + nz) % nz;
+            atomicAdd(&racc[gx + (int64_t)nx * (gy + (int64_t)ny * gz)], v);
+        }
+    }
+}
+
+// ---------------------------------------------------------------- binning
+// Count particles of a contiguous staging list into bins (tile of the cell).
+__global__ void k_bin_count(const InstDev *__restrict__ insts, int inst, int s, int nbuf,
+                            int64_t n, int tx_, int ty_, int tz_)
+{
+    const InstDev *I = insts + inst;
+    const SpecDev &S = I->sp[s];
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double x = ((const double *)S.buf[nbuf][0])[i];
+    const double y = ((const double *)S.buf[nbuf][1])[i];
+    const double z = ((const double *)S.buf[nbuf][2])[i];
+    if (!(x >= 0.0 && x < I->n[0] && y >= 0.0 && y < I->n[1] && z >= 0.0 && z < I->n[2])) {
+        atomicAdd(&I->flags[kFlagRange], 1ull);
+        return;
+    }
+    const int t = ((int)x) / tx_ + I->nt[0] * (((int)y) / ty_ + I->nt[1] * (((int)z) / tz_));
+    atomicAdd(&S.count[nbuf ^ 1][t], 1);
+}
+
+// Scatter a contiguous staging list (buffer nbuf, entries [0, n)) into the
+// bins of buffer nbuf^1; count[nbuf^1] must be zero on entry and ends as
+// the bin occupancy.
+__global__ void k_bin_scatter(const InstDev *__restrict__ insts, int inst, int s, int nbuf,
+                              int64_t n, int tx_, int ty_, int tz_)
+{
+    const InstDev *I = insts + inst;
+    const SpecDev &S = I->sp[s];
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    P64 q;
+    q.x = ((const double *)S.buf[nbuf][0])[i];
+    q.y = ((const double *)S.buf[nbuf][1])[i];
+    q.z = ((const double *)S.buf[nbuf][2])[i];
+    q.vx = ((const double *)S.buf[nbuf][3])[i];
+    q.vy = ((const double *)S.buf[nbuf][4])[i];
+    q.vz = ((const double *)S.buf[nbuf][5])[i];
+    const bool ids = I->track_ids != 0;
+    q.id = ids ? S.pid[nbuf][i] : 0;
+    if (!(q.x >= 0.0 && q.x < I->n[0] && q.y >= 0.0 && q.y < I->n[1] && q.z >= 0.0 &&
+          q.z < I->n[2]))
+        return;
+    const int t =
+        ((int)q.x) / tx_ + I->nt[0] * (((int)q.y) / ty_ + I->nt[1] * (((int)q.z) / tz_));
+    const int slot = atomicAdd(&S.count[nbuf ^ 1][t], 1);
+    write_p64(I, S, s, nbuf ^ 1, t, slot, q, ids);
+}
+
+// Compact the bins of buffer `src` (old bases/caps given explicitly) into a
+// contiguous list [0, n) of buffer `dst` -- used to store particles and as
+// the first half of a repack.  One workgroup per bin.
+__global__ void k_bins_to_list(const InstDev *__restrict__ insts, int inst, int s, int src,
+                               int dst, const int64_t *__restrict__ obase,
+                               const int32_t *__restrict__ ocap,
+                               const int32_t *__restrict__ ocount,
+                               const int64_t *__restrict__ list_off)
+{
+    const InstDev *I = insts + inst;
+    const SpecDev &S = I->sp[s];
+    const int t = blockIdx.x;
+    const int cnt = min(ocount[t], ocap[t]);
+    const int64_t ib = obase[t];
+    const int64_t ob = list_off[t];
+    const bool ids = I->track_ids != 0;
+    for (int i = threadIdx.x; i < cnt; i += blockDim.x) {
+#pragma unroll
+        for (int c = 0; c < 6; ++c)
+            ((double *)S.buf[dst][c])[ob + i] = ((const double *)S.buf[src][c])[ib + i];
+        if (ids) S.pid[dst][ob + i] = S.pid[src][ib + i];
+    }
+}
+
+// Append overflow entries of species s to the contiguous list of buffer dst
+// at offset `off`.
+__global__ void k_ovf_to_list(const InstDev *__restrict__ insts, int inst, int s, int dst,
+                              int64_t nov, int64_t off, unsigned long long *cursor)
+{
+    const InstDev *I = insts + inst;
+    const SpecDev &S = I->sp[s];
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nov || I->ovf.spec[i] != s) return;
+    const int64_t o = off + (int64_t)atomicAdd(cursor, 1ull);
+#pragma unroll
+    for (int c = 0; c < 6; ++c) ((double *)S.buf[dst][c])[o] = ((const double *)I->ovf.comp[c])[i];
+    if (I->track_ids) S.pid[dst][o] = I->ovf.pid[i];
+}
+
+__global__ void k_cell_counts(const InstDev *__restrict__ insts, int inst, int s, int par,
+                              int32_t *__restrict__ counts)
+{
+    const InstDev *I = insts + inst;
+    const SpecDev &S = I->sp[s];
+    const int t = blockIdx.x;
+    const int cnt = min(S.count[par][t], S.bin_cap[t]);
+    const int64_t b = S.bin_base[t];
+    const int nx = I->n[0], ny = I->n[1];
+    for (int i = threadIdx.x; i < cnt; i += blockDim.x) {
+        const double x = ((const double *)S.buf[par][0])[b + i];
+        const double y = ((const double *)S.buf[par][1])[b + i];
+        const double z = ((const double *)S.buf[par][2])[b + i];
+        const int64_t c = (int)x + (int64_t)nx * ((int)y + (int64_t)ny * (int)z);
+        atomicAdd(&counts[c], 1);
+    }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------- launchers
+template <int TX, int TY, int TZ, int WO>
+static hipError_t launch_tile(int mode, const InstDev *insts, const WorkItem *items,
+                              int nitems, int par, hipStream_t st)
+{
+    dim3 g(nitems), b(kBlock);
+    switch (mode) {
+    case kModeFused:
+        hipLaunchKernelGGL((k_particles_fp64<TX, TY, TZ, WO, kModeFused>), g, b, 0, st, insts,
+                           items, par);
+        break;
+    case kModeDeposit:
+        hipLaunchKernelGGL((k_particles_fp64<TX, TY, TZ, WO, kModeDeposit>), g, b, 0, st,
+                           insts, items, par);
+        break;
+    case kModeGatherPush:
+        hipLaunchKernelGGL((k_particles_fp64<TX, TY, TZ, WO, kModeGatherPush>), g, b, 0, st,
+                           insts, items, par);
+        break;
+    case kModeBoundary:
+        hipLaunchKernelGGL((k_particles_fp64<TX, TY, TZ, WO, kModeBoundary>), g, b, 0, st,
+                           insts, items, par);
+        break;
+    default:
+        return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+bool tile_supported(int tx, int ty, int tz)
+{
+    return (tx == ty && ty == tz && (tx == 4 || tx == 8)) || (tx == 8 && ty == 8 && tz == 4) ||
+           (tx == 16 && ty == 8 && tz == 4);
+}
+
+hipError_t launch_particles(int mode, int precision, int weight_order, const int T[3],
+                            const InstDev *insts, const WorkItem *items, int nitems, int par,
+                            hipStream_t st)
+{
+    if (nitems <= 0) return hipSuccess;
+    if (precision != 0) return hipErrorInvalidValue;
+#define OHPIC_TILE(A, B, C)                                                     \
+    if (T[0] == A && T[1] == B && T[2] == C)                                    \
+        return weight_order == 0                                                \
+                   ? launch_tile<A, B, C, 0>(mode, insts, items, nitems, par, st) \
+                   : launch_tile<A, B, C, 1>(mode, insts, items, nitems, par, st);
+    OHPIC_TILE(4, 4, 4)
+    OHPIC_TILE(8, 8, 8)
+    OHPIC_TILE(8, 8, 4)
+    OHPIC_TILE(16, 8, 4)
+#undef OHPIC_TILE
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_bin_count(const InstDev *insts, int inst, int s, int nbuf, int64_t n,
+                            const int T[3], hipStream_t st)
+{
+    if (n <= 0) return hipSuccess;
+    const int64_t blocks = (n + 255) / 256;
+    hipLaunchKernelGGL(k_bin_count, dim3((unsigned)blocks), dim3(256), 0, st, insts, inst, s,
+                       nbuf, n, T[0], T[1], T[2]);
+    return hipGetLastError();
+}
+
+hipError_t launch_bin_scatter(const InstDev *insts, int inst, int s, int nbuf, int64_t n,
+                              const int T[3], hipStream_t st)
+{
+    if (n <= 0) return hipSuccess;
+    const int64_t blocks = (n + 255) / 256;
+    hipLaunchKernelGGL(k_bin_scatter, dim3((unsigned)blocks), dim3(256), 0, st, insts, inst, s,
+                       nbuf, n, T[0], T[1], T[2]);
+    return hipGetLastError();
+}
+
+hipError_t launch_bins_to_list(const InstDev *insts, int inst, int s, int src, int dst,
+                               int ntiles, const int64_t *obase, const int32_t *ocap,
+                               const int32_t *ocount, const int64_t *list_off, hipStream_t st)
+{
+    if (ntiles <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_bins_to_list, dim3(ntiles), dim3(256), 0, st, insts, inst, s, src, dst,
+                       obase, ocap, ocount, list_off);
+    return hipGetLastError();
+}
+
+hipError_t launch_ovf_to_list(const InstDev *insts, int inst, int s, int dst, int64_t nov,
+                              int64_t off, unsigned long long *cursor, hipStream_t st)
+{
+    if (nov <= 0) return hipSuccess;
+    const int64_t blocks = (nov + 255) / 256;
+    hipLaunchKernelGGL(k_ovf_to_list, dim3((unsigned)blocks), dim3(256), 0, st, insts, inst, s,
+                       dst, nov, off, cursor);
+    return hipGetLastError();
+}
+
+hipError_t launch_cell_counts(const InstDev *insts, int inst, int s, int par, int ntiles,
+                              int32_t *counts, hipStream_t st)
+{
+    if (ntiles <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_cell_counts, dim3(ntiles), dim3(256), 0, st, insts, inst, s, par,
+                       counts);
+    return hipGetLastError();
+}
+
+}  // namespace ohpic
