@@ -221,23 +221,25 @@ def test_energy_history_matches_oracle(ohp, orc):
 # ------------------------------------------------------------------ boundaries
 def test_wrap_edge_cases_bit_exact(ohp, orc):
     # q = 0 and B = 0 -> E = 0 exactly, v unchanged, x' = x + v dt exact (dt = 1/8)
-    n = (8, 8, 8)
+    n = (16, 8, 8)
     dt = 0.125
-    x = np.array([0.0, 7.5, 0.25, 0.5, 0.5, 3.0, 2.0])
-    v = np.array([-2.0 ** -57, 4.0, -4.0, -68.0, -100.0, 0.0, 8.0])
+    x = np.array([0.0, 15.5, 0.25, 0.5, 0.5, 3.0, 2.0, 0.5])
+    v = np.array([-2.0 ** -57, 4.0, -4.0, -132.0, -200.0, 0.0, 8.0, 48.0])
     y = np.full_like(x, 1.5)
     z = np.full_like(x, 6.25)
     zero = np.zeros_like(x)
     sp = [dict(qm=-1.0, q=0.0, x=x, y=y, z=z, vx=v, vy=zero, vz=zero)]
     osim = orc.OracleSim(n, sp, dt, (0, 0, 0), 0)
     osim.step(1)
-    with sim_for(sp, n, dt=dt, b0=(0, 0, 0)) as sim:
+    with sim_for(sp, n, dt=dt, b0=(0, 0, 0), tile=(4, 4, 4)) as sim:
         sim.step(1)
         g = by_pid(sim.store_particles(0, True))
         d = sim.diagnostics()
     assert np.array_equal(g["x"], osim.sp[0]["x"])
-    assert list(g["x"]) == [0.0, 0.0, 7.75, 0.0, 4.0, 3.0, 3.0]
-    assert d["fast_particles"] >= 2
+    # -tiny -> n -> 0; n -> 0; -0.25 -> 15.75; -16 -> 0; -24.5 -> -8.5 -> 7.5 (two wraps)
+    assert list(g["x"]) == [0.0, 0.0, 15.75, 0.0, 7.5, 3.0, 3.0, 6.5]
+    # two particles end outside the deposit halo of their old tile
+    assert d["fast_particles"] == 2
 
 
 def test_repack_after_bin_overflow(ohp, orc):
