@@ -145,6 +145,24 @@ def run_cpu_baseline(seconds: float):
                       f"{len(times)} steps, median step {med * 1e3:.1f} ms, 1 thread"}
 
 
+def run_copy_calibration(gib: float = 2.0, reps: int = 5):
+    """A plain device copy of known size: its FETCH_SIZE / WRITE_SIZE counters
+    calibrate the per-dispatch HBM byte counts of the PIC kernels."""
+    import torch
+
+    n = int(gib * (1 << 30) // 8)
+    a = torch.ones(n, dtype=torch.float64, device="cuda:0")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(reps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t) / reps
+    return {"copy_bytes_read": 8 * n, "copy_bytes_written": 8 * n, "reps": reps,
+            "copy_GBps": 16 * n / dt / 1e9}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -156,7 +174,11 @@ def main():
     ap.add_argument("--chunks", type=int, default=0)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true")
     args = ap.parse_args()
+    if args.copy_calibration:
+        print(json.dumps(run_copy_calibration()), flush=True)
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
               "(one MI355X holds the whole 256^3 domain)", file=sys.stderr)
