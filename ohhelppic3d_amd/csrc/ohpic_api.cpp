@@ -72,6 +72,7 @@ struct ohpic_ctx {
     int64_t *d_tmp64b = nullptr; // [ntiles]
     unsigned long long *d_cursor = nullptr;
     int32_t *d_cellcounts = nullptr;
+    double *d_epartial = nullptr;  // 3 * kEfieldMaxBlocks
     FftAxisDev ax[3]{};
     double *d_lam[3]{};
     unsigned long long *h_flags = nullptr;  // pinned
@@ -138,14 +139,6 @@ int64_t species_alloc(const ohpic_config &c, int s, int ntiles)
     return (int64_t)std::ceil((double)c.capacity[s] * (1.0 + slack)) + (int64_t)ntiles * 80 + 64;
 }
 
-int64_t overflow_cap(const ohpic_config &c)
-{
-    const double f = c.overflow_frac > 0 ? c.overflow_frac : 0.05;
-    int64_t tot = 0;
-    for (int s = 0; s < c.nspecies; ++s) tot += c.capacity[s];
-    return (int64_t)std::ceil(f * (double)tot) + 4096;
-}
-
 void pick_tile(const ohpic_config &c, int T[3])
 {
     for (int a = 0; a < 3; ++a) T[a] = c.tile[a];
@@ -184,9 +177,8 @@ int64_t estimate_bytes(const ohpic_config &c)
     int64_t b = 0;
     for (int s = 0; s < c.nspecies; ++s) {
         b += 2 * species_alloc(c, s, (int)nt) * per_particle;
-        b += nt * (8 + 4 + 4 + 4);
+        b += nt * (8 + 4 + 8 + 8);
     }
-    b += overflow_cap(c) * (per_particle + 8);
     b += ng * (8 * 2 + 8 * 5) + ng * 4;  // acc x2, rho, phi, E x3, cell counts
     b += nt * (16 + 8 + 8 + 4 + 4) * 2;
     return b;
@@ -204,25 +196,25 @@ void timer_begin(ohpic_ctx *ctx, int k)
     if (!ctx->timing) return;
     TimerSlot &t = ctx->timers[k];
     if (!t.a) {
-        hipEventCreate(&t.a);
-        hipEventCreate(&t.b);
+        (void)hipEventCreate(&t.a);
+        (void)hipEventCreate(&t.b);
     }
     if (t.pending) {
-        hipEventSynchronize(t.b);
+        (void)hipEventSynchronize(t.b);
         float ms = 0;
-        hipEventElapsedTime(&ms, t.a, t.b);
+        (void)hipEventElapsedTime(&ms, t.a, t.b);
         t.total_ms += ms;
         t.launches += 1;
         t.pending = false;
     }
-    hipEventRecord(t.a, ctx->st);
+    (void)hipEventRecord(t.a, ctx->st);
 }
 
 void timer_end(ohpic_ctx *ctx, int k)
 {
     if (!ctx->timing) return;
     TimerSlot &t = ctx->timers[k];
-    hipEventRecord(t.b, ctx->st);
+    (void)hipEventRecord(t.b, ctx->st);
     t.pending = true;
 }
 
@@ -231,9 +223,9 @@ void timers_collect(ohpic_ctx *ctx)
     for (int k = 0; k < kNumTimers; ++k) {
         TimerSlot &t = ctx->timers[k];
         if (!t.pending) continue;
-        hipEventSynchronize(t.b);
+        (void)hipEventSynchronize(t.b);
         float ms = 0;
-        hipEventElapsedTime(&ms, t.a, t.b);
+        (void)hipEventElapsedTime(&ms, t.a, t.b);
         t.total_ms += ms;
         t.launches += 1;
         t.pending = false;
@@ -265,22 +257,32 @@ int build_work_items(ohpic_ctx *ctx)
     std::vector<WorkItem> items;
     items.reserve((size_t)nt * nch);
     for (int c = 0; c < nch; ++c)
-        for (int i = 0; i < nt; ++i) items.push_back(WorkItem{0, order[i], c, nch});
+        for (int i = 0; i < nt; ++i)
+            items.push_back(WorkItem{0, order[i], (int16_t)c, (int16_t)nch, 0});
     // heavy tiles first within each chunk round: interleave so the first
     // dispatched items are the heaviest chunks
     std::stable_sort(items.begin(), items.end(), [&](const WorkItem &a, const WorkItem &b) {
         return load[a.tile] > load[b.tile];
     });
+    for (size_t i = 0; i < items.size(); ++i) items[i].slot = (int32_t)i;
     if ((int)items.size() > ctx->nitems_alloc) {
-        if (ctx->d_items) {
-            hipFree(ctx->d_items);
-            ctx->allocs.erase(std::find(ctx->allocs.begin(), ctx->allocs.end(),
-                                        (void *)ctx->d_items));
-            ctx->d_items = nullptr;
+        for (void *old : {(void *)ctx->d_items, (void *)ctx->h.ke_partial}) {
+            if (!old) continue;
+            HIPCHK(ctx, hipStreamSynchronize(ctx->st));
+            (void)hipFree(old);
+            ctx->allocs.erase(std::find(ctx->allocs.begin(), ctx->allocs.end(), old));
         }
+        ctx->d_items = nullptr;
+        ctx->h.ke_partial = nullptr;
         int rc = dalloc(ctx, &ctx->d_items, (int64_t)items.size());
         if (rc) return rc;
+        rc = dalloc(ctx, &ctx->h.ke_partial, (int64_t)items.size() * kKeSlots);
+        if (rc) return rc;
+        HIPCHK(ctx, launch_fill_f64(ctx->h.ke_partial, (int64_t)items.size() * kKeSlots, 0.0,
+                                    ctx->st));
         ctx->nitems_alloc = (int)items.size();
+        rc = sync_upload_inst(ctx);
+        if (rc) return rc;
     }
     ctx->nitems = (int)items.size();
     HIPCHK(ctx, hipMemcpyAsync(ctx->d_items, items.data(), items.size() * sizeof(WorkItem),
@@ -290,7 +292,7 @@ int build_work_items(ohpic_ctx *ctx)
 }
 
 // Assign capacities/bases from per-bin counts and upload them.
-int set_bins(ohpic_ctx *ctx, int s, const std::vector<int32_t> &counts)
+int set_bins(ohpic_ctx *ctx, int s, const std::vector<int64_t> &counts)
 {
     SpeciesHost &S = ctx->sp[s];
     const double slack = ctx->cfg.slack > 0 ? ctx->cfg.slack : 0.15;
@@ -331,106 +333,139 @@ int clear_flags(ohpic_ctx *ctx)
     return OHPIC_OK;
 }
 
-// Bin a contiguous list of n particles held in buffer stg = par^1 into the
-// bins of buffer par.
-int bin_from_staging(ohpic_ctx *ctx, int s, int64_t n)
+int download_words(ohpic_ctx *ctx, const unsigned long long *dev, std::vector<int64_t> &F,
+                   std::vector<int64_t> &B)
 {
     const int nt = ctx->ntiles;
-    const int stg = ctx->par ^ 1;
-    int32_t *cnt = ctx->h.sp[s].count[ctx->par];
-    HIPCHK(ctx, launch_fill_i32(cnt, nt, 0, ctx->st));
+    std::vector<unsigned long long> w(nt);
+    HIPCHK(ctx, hipMemcpyAsync(w.data(), dev, nt * sizeof(unsigned long long),
+                               hipMemcpyDeviceToHost, ctx->st));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->st));
+    F.resize(nt);
+    B.resize(nt);
+    for (int t = 0; t < nt; ++t) {
+        F[t] = w[t] & 0xffffffffull;
+        B[t] = (int64_t)(w[t] >> 32);
+    }
+    return OHPIC_OK;
+}
+
+// Bin a contiguous list of n particles held in buffer par^1 into the bins of
+// buffer par (wrap != 0: wrap positions periodically first).
+int bin_from_staging(ohpic_ctx *ctx, int s, int64_t n, int wrap)
+{
+    const int nt = ctx->ntiles;
+    const int lbuf = ctx->par ^ 1;
+    unsigned long long *cnt = ctx->h.sp[s].count[ctx->par];
+    HIPCHK(ctx, launch_fill_u64(cnt, nt, 0ull, ctx->st));
     int rc = clear_flags(ctx);
     if (rc) return rc;
-    HIPCHK(ctx, launch_bin_count(ctx->d_inst, 0, s, stg, n, ctx->T, ctx->st));
-    std::vector<int32_t> counts(nt);
-    HIPCHK(ctx, hipMemcpyAsync(counts.data(), cnt, nt * sizeof(int32_t), hipMemcpyDeviceToHost,
-                               ctx->st));
-    rc = read_flags(ctx);
-    if (rc) return rc;
+    HIPCHK(ctx, launch_bin_count(ctx->d_inst, 0, s, lbuf, n, ctx->T, wrap, ctx->st));
+    std::vector<int64_t> F, B;
+    if ((rc = download_words(ctx, cnt, F, B))) return rc;
+    if ((rc = read_flags(ctx))) return rc;
     if (ctx->h_flags[kFlagRange])
         return fail(ctx, OHPIC_EINVAL, "species %d: %llu particle positions outside [0, n)", s,
                     (unsigned long long)ctx->h_flags[kFlagRange]);
-    rc = set_bins(ctx, s, counts);
+    rc = set_bins(ctx, s, F);
     if (rc) return rc;
-    HIPCHK(ctx, launch_fill_i32(cnt, nt, 0, ctx->st));
-    HIPCHK(ctx, launch_bin_scatter(ctx->d_inst, 0, s, stg, n, ctx->T, ctx->st));
+    HIPCHK(ctx, launch_fill_u64(cnt, nt, 0ull, ctx->st));
+    HIPCHK(ctx, launch_bin_scatter(ctx->d_inst, 0, s, lbuf, n, ctx->T, wrap, ctx->st));
     rc = read_flags(ctx);
     if (rc) return rc;
-    if (ctx->h_flags[kFlagOverflow] || ctx->h_flags[kFlagLost])
+    if (ctx->h_flags[kFlagOverflow])
         return fail(ctx, OHPIC_ENOMEM, "binning overflowed (internal error)");
     ctx->sp[s].n = n;
     return OHPIC_OK;
 }
 
-// Compact the bins of buffer par (counts in count[par], possibly above cap)
-// plus species-s overflow entries into a contiguous list in buffer par^1.
-// Returns the number of particles listed.
-int bins_to_staging(ohpic_ctx *ctx, int s, int64_t nov, int64_t *nlisted)
+// Compact both runs of every bin of buffer par into a contiguous list in
+// buffer par^1; returns the number of particles listed.
+int bins_to_staging(ohpic_ctx *ctx, int s, int64_t *nlisted)
 {
     const int nt = ctx->ntiles;
-    std::vector<int32_t> counts(nt);
-    HIPCHK(ctx, hipMemcpyAsync(counts.data(), ctx->h.sp[s].count[ctx->par], nt * sizeof(int32_t),
-                               hipMemcpyDeviceToHost, ctx->st));
-    HIPCHK(ctx, hipStreamSynchronize(ctx->st));
+    std::vector<int64_t> F, B;
+    int rc = download_words(ctx, ctx->h.sp[s].count[ctx->par], F, B);
+    if (rc) return rc;
     std::vector<int64_t> off(nt);
     int64_t tot = 0;
     for (int t = 0; t < nt; ++t) {
         off[t] = tot;
-        tot += std::min(counts[t], ctx->sp[s].cap[t]);
+        const int64_t cap = ctx->sp[s].cap[t];
+        const int64_t f = std::min(F[t], cap);
+        tot += f + std::min(B[t], cap - f);
     }
     HIPCHK(ctx, hipMemcpyAsync(ctx->d_tmp64, off.data(), nt * sizeof(int64_t),
                                hipMemcpyHostToDevice, ctx->st));
-    HIPCHK(ctx, launch_bins_to_list(ctx->d_inst, 0, s, ctx->par, ctx->par ^ 1, nt,
-                                    ctx->h.sp[s].bin_base, ctx->h.sp[s].bin_cap,
-                                    ctx->h.sp[s].count[ctx->par], ctx->d_tmp64, ctx->st));
-    if (nov > 0) {
-        HIPCHK(ctx, launch_fill_u64(ctx->d_cursor, 1, 0ull, ctx->st));
-        HIPCHK(ctx, launch_ovf_to_list(ctx->d_inst, 0, s, ctx->par ^ 1, nov, tot, ctx->d_cursor,
-                                       ctx->st));
-        unsigned long long added = 0;
-        HIPCHK(ctx, hipMemcpyAsync(&added, ctx->d_cursor, sizeof added, hipMemcpyDeviceToHost,
-                                   ctx->st));
-        HIPCHK(ctx, hipStreamSynchronize(ctx->st));
-        tot += (int64_t)added;
-    }
+    HIPCHK(ctx, launch_bins_to_list(ctx->d_inst, 0, s, ctx->par, ctx->par ^ 1, nt, ctx->d_tmp64,
+                                    ctx->st));
     HIPCHK(ctx, hipStreamSynchronize(ctx->st));
     *nlisted = tot;
     return OHPIC_OK;
 }
 
-int repack(ohpic_ctx *ctx, int64_t nov)
+int zero_next_counts(ohpic_ctx *ctx)
+{
+    for (int s = 0; s < ctx->cfg.nspecies; ++s)
+        HIPCHK(ctx, launch_fill_u64(ctx->h.sp[s].count[ctx->par ^ 1], ctx->ntiles, 0ull, ctx->st));
+    return OHPIC_OK;
+}
+
+// Retry of a step whose exchange overflowed a bin: run the kernel again in
+// list mode (every particle to a contiguous list of buffer par^1), then bin
+// each species' list into buffer par with capacities sized from it.
+int rerun_as_list(ohpic_ctx *ctx, int list_mode)
 {
     timer_begin(ctx, 3);
+    int rc = zero_next_counts(ctx);
+    if (rc) return rc;
+    if (list_mode == kModeFusedList)
+        HIPCHK(ctx, launch_fill_u64(ctx->h.rho_acc[ctx->par ^ 1], ctx->ng, 0ull, ctx->st));
+    if ((rc = clear_flags(ctx))) return rc;
+    HIPCHK(ctx, launch_particles(list_mode, ctx->cfg.precision, ctx->cfg.weight_order, ctx->T,
+                                 ctx->d_inst, ctx->d_items, ctx->nitems, ctx->par, ctx->st));
+    if (list_mode == kModeFusedList)
+        HIPCHK(ctx, launch_reduce_partials(ctx->h.ke_partial, ctx->nitems, kKeSlots, kKeSlots,
+                                           ctx->h.diag + kDiagKeOld, 0, ctx->st));
+    if ((rc = read_flags(ctx))) return rc;
+    if (ctx->h_flags[kFlagRange])
+        return fail(ctx, OHPIC_ESTATE, "%llu particles moved more than two box lengths",
+                    (unsigned long long)ctx->h_flags[kFlagRange]);
+    std::vector<int64_t> nlist(ctx->cfg.nspecies);
     for (int s = 0; s < ctx->cfg.nspecies; ++s) {
-        int64_t n = 0;
-        int rc = bins_to_staging(ctx, s, nov, &n);
-        if (rc) return rc;
-        rc = bin_from_staging(ctx, s, n);
-        if (rc) return rc;
+        unsigned long long w = 0;
+        HIPCHK(ctx, hipMemcpyAsync(&w, ctx->h.sp[s].count[ctx->par ^ 1], sizeof w,
+                                   hipMemcpyDeviceToHost, ctx->st));
+        HIPCHK(ctx, hipStreamSynchronize(ctx->st));
+        nlist[s] = (int64_t)w;
+    }
+    for (int s = 0; s < ctx->cfg.nspecies; ++s)
+        if ((rc = bin_from_staging(ctx, s, nlist[s], 0))) return rc;
+    if (list_mode == kModeFusedList) {
+        // the step's charge is in accumulator par^1 while the particles are
+        // back in buffer par: swap the accumulators
+        std::swap(ctx->h.rho_acc[0], ctx->h.rho_acc[1]);
+        if ((rc = sync_upload_inst(ctx))) return rc;
     }
     timer_end(ctx, 3);
-    int rc = clear_flags(ctx);
-    if (rc) return rc;
+    if ((rc = clear_flags(ctx))) return rc;
     ctx->repacks += 1;
     return build_work_items(ctx);
 }
 
-int check_after_rebin(ohpic_ctx *ctx)
+// After a kernel that re-bins into buffer par^1: 0 = fine, 1 = a bin
+// overflowed (the output is incomplete), < 0 = error.
+int check_rebin(ohpic_ctx *ctx)
 {
     int rc = read_flags(ctx);
     if (rc) return rc;
     timers_collect(ctx);
     const unsigned long long *f = ctx->h_flags;
-    ctx->fast += (int64_t)f[kFlagFast];
-    if (f[kFlagLost])
-        return fail(ctx, OHPIC_ELOST, "%llu particles lost: overflow list full (raise slack)",
-                    (unsigned long long)f[kFlagLost]);
     if (f[kFlagRange])
         return fail(ctx, OHPIC_ESTATE,
                     "%llu particles moved more than two box lengths in one step",
                     (unsigned long long)f[kFlagRange]);
-    if (f[kFlagOverflow]) return repack(ctx, (int64_t)f[kFlagOverflow]);
-    return OHPIC_OK;
+    return f[kFlagOverflow] ? 1 : 0;
 }
 
 int do_deposit(ohpic_ctx *ctx)
@@ -463,17 +498,9 @@ int do_solve(ohpic_ctx *ctx, bool from_acc)
                               L1, L2, ctx->st));
     HIPCHK(ctx, fft_axis_pass(kFftHC2R, 0, phi, false, 1.0, phi, 1.0 / (double)ctx->ng, nx, ny,
                               nz, ctx->ax[0], L0, L1, L2, ctx->st));
-    HIPCHK(ctx, launch_fill_f64(ctx->h.diag + kDiagWe, 3, 0.0, ctx->st));
     HIPCHK(ctx, launch_efield(phi, rho, ctx->h.e[0], ctx->h.e[1], ctx->h.e[2], nx, ny, nz,
-                              ctx->h.diag, ctx->st));
+                              ctx->d_epartial, ctx->h.diag, ctx->st));
     timer_end(ctx, 1);
-    return OHPIC_OK;
-}
-
-int zero_next_counts(ohpic_ctx *ctx)
-{
-    for (int s = 0; s < ctx->cfg.nspecies; ++s)
-        HIPCHK(ctx, launch_fill_i32(ctx->h.sp[s].count[ctx->par ^ 1], ctx->ntiles, 0, ctx->st));
     return OHPIC_OK;
 }
 
@@ -482,7 +509,6 @@ int one_step(ohpic_ctx *ctx)
     int rc;
     if (!ctx->rho_valid && (rc = do_deposit(ctx))) return rc;
     if ((rc = do_solve(ctx, true))) return rc;
-    HIPCHK(ctx, launch_fill_f64(ctx->h.diag, 2 * kMaxSpecies, 0.0, ctx->st));
     if ((rc = zero_next_counts(ctx))) return rc;
     HIPCHK(ctx, launch_fill_u64(ctx->h.rho_acc[ctx->par ^ 1], ctx->ng, 0ull, ctx->st));
     if ((rc = clear_flags(ctx))) return rc;
@@ -490,10 +516,20 @@ int one_step(ohpic_ctx *ctx)
     HIPCHK(ctx, launch_particles(kModeFused, ctx->cfg.precision, ctx->cfg.weight_order, ctx->T,
                                  ctx->d_inst, ctx->d_items, ctx->nitems, ctx->par, ctx->st));
     timer_end(ctx, 0);
-    ctx->par ^= 1;
+    HIPCHK(ctx, launch_reduce_partials(ctx->h.ke_partial, ctx->nitems, kKeSlots, kKeSlots,
+                                       ctx->h.diag + kDiagKeOld, 0, ctx->st));
+    rc = check_rebin(ctx);
+    if (rc < 0) return rc;
+    ctx->fast += (int64_t)ctx->h_flags[kFlagFast];
+    if (rc == 1) {
+        // a bin overflowed: redo the step into a list and bin it into buffer par
+        if ((rc = rerun_as_list(ctx, kModeFusedList))) return rc;
+    } else {
+        ctx->par ^= 1;
+    }
     ctx->step += 1;
     ctx->rho_valid = true;
-    return check_after_rebin(ctx);
+    return OHPIC_OK;
 }
 
 int grid_copy_out(ohpic_ctx *ctx, const double *dev, std::vector<double> &host)
@@ -524,17 +560,17 @@ int make_axis(ohpic_ctx *ctx, int n, FftAxisDev &ax)
 void release(ohpic_ctx *ctx)
 {
     if (!ctx) return;
-    hipSetDevice(ctx->device);
-    if (ctx->st) hipStreamSynchronize(ctx->st);
-    for (void *p : ctx->allocs) hipFree(p);
+    (void)hipSetDevice(ctx->device);
+    if (ctx->st) (void)hipStreamSynchronize(ctx->st);
+    for (void *p : ctx->allocs) (void)hipFree(p);
     ctx->allocs.clear();
-    if (ctx->h_flags) hipHostFree(ctx->h_flags);
-    if (ctx->h_diag) hipHostFree(ctx->h_diag);
+    if (ctx->h_flags) (void)hipHostFree(ctx->h_flags);
+    if (ctx->h_diag) (void)hipHostFree(ctx->h_diag);
     for (auto &t : ctx->timers) {
-        if (t.a) hipEventDestroy(t.a);
-        if (t.b) hipEventDestroy(t.b);
+        if (t.a) (void)hipEventDestroy(t.a);
+        if (t.b) (void)hipEventDestroy(t.b);
     }
-    if (ctx->st) hipStreamDestroy(ctx->st);
+    if (ctx->st) (void)hipStreamDestroy(ctx->st);
     delete ctx;
 }
 
@@ -716,11 +752,11 @@ int ohpic_create(const ohpic_config *cfg, ohpic_ctx **out)
             D.pid[b] = nullptr;
             if (c.track_ids) CREATE_CHK(dalloc(ctx, &D.pid[b], S.alloc));
             CREATE_CHK(dalloc(ctx, &D.count[b], nt));
-            CREATE_HIP(launch_fill_i32(D.count[b], nt, 0, ctx->st));
+            CREATE_HIP(launch_fill_u64(D.count[b], nt, 0ull, ctx->st));
         }
         CREATE_CHK(dalloc(ctx, &D.bin_base, nt));
         CREATE_CHK(dalloc(ctx, &D.bin_cap, nt));
-        std::vector<int32_t> zero(nt, 0);
+        std::vector<int64_t> zero(nt, 0);
         CREATE_CHK(set_bins(ctx, s, zero));
         D.qm = c.qm[s];
         D.q = c.q_macro[s];
@@ -740,22 +776,12 @@ int ohpic_create(const ohpic_config *cfg, ohpic_ctx **out)
     CREATE_HIP(launch_fill_f64(h.diag, kDiagSlots, 0.0, ctx->st));
     CREATE_CHK(dalloc(ctx, &h.flags, kFlagSlots));
     CREATE_HIP(launch_fill_u64(h.flags, kFlagSlots, 0ull, ctx->st));
-    h.ovf.cap = overflow_cap(c);
-    for (int k = 0; k < 6; ++k) {
-        double *p = nullptr;
-        CREATE_CHK(dalloc(ctx, &p, h.ovf.cap));
-        h.ovf.comp[k] = p;
-    }
-    h.ovf.comp[6] = nullptr;
-    h.ovf.pid = nullptr;
-    if (c.track_ids) CREATE_CHK(dalloc(ctx, &h.ovf.pid, h.ovf.cap));
-    CREATE_CHK(dalloc(ctx, &h.ovf.dest, h.ovf.cap));
-    CREATE_CHK(dalloc(ctx, &h.ovf.spec, h.ovf.cap));
     CREATE_CHK(dalloc(ctx, &ctx->d_tmp64, nt));
     CREATE_CHK(dalloc(ctx, &ctx->d_tmp64b, nt));
     CREATE_CHK(dalloc(ctx, &ctx->d_tmp32a, nt));
     CREATE_CHK(dalloc(ctx, &ctx->d_tmp32b, nt));
     CREATE_CHK(dalloc(ctx, &ctx->d_cursor, 1));
+    CREATE_CHK(dalloc(ctx, &ctx->d_epartial, 3 * kEfieldMaxBlocks));
     CREATE_CHK(dalloc(ctx, &ctx->d_inst, 1));
     // FFT plans and the Green's-function denominators per axis
     for (int a = 0; a < 3; ++a) {
@@ -838,7 +864,7 @@ int ohpic_load_particles(ohpic_ctx *ctx, int ispec, int64_t n, const double *x, 
         }
     }
     HIPCHK(ctx, hipStreamSynchronize(ctx->st));
-    int rc = bin_from_staging(ctx, ispec, n);
+    int rc = bin_from_staging(ctx, ispec, n, 0);
     if (rc) return rc;
     ctx->rho_valid = false;
     return build_work_items(ctx);
@@ -863,7 +889,7 @@ int ohpic_store_particles(ohpic_ctx *ctx, int ispec, int64_t cap, int64_t *n, do
     if (!ctx->binned) return fail(ctx, OHPIC_ESTATE, "particles pushed but not re-binned");
     HIPCHK(ctx, hipSetDevice(ctx->device));
     int64_t cnt = 0;
-    int rc = bins_to_staging(ctx, ispec, 0, &cnt);
+    int rc = bins_to_staging(ctx, ispec, &cnt);
     if (rc) return rc;
     *n = cnt;
     if (cnt > cap) return fail(ctx, OHPIC_EINVAL, "buffer holds %lld, need %lld", (long long)cap,
@@ -980,7 +1006,7 @@ int ohpic_seed_uniform_maxwell(ohpic_ctx *ctx, uint64_t initstate, uint64_t init
                                         (double *)D.buf[stg][4], (double *)D.buf[stg][5],
                                         ctx->cfg.track_ids ? D.pid[stg] : nullptr, 0, ctx->st));
         offset += 1 + 7 * (uint64_t)np[s];
-        int rc = bin_from_staging(ctx, s, np[s]);
+        int rc = bin_from_staging(ctx, s, np[s], 0);
         if (rc) return rc;
     }
     ctx->rho_valid = false;
@@ -1022,7 +1048,7 @@ int ohpic_seed_hump_maxwell(ohpic_ctx *ctx, uint64_t initstate, uint64_t initseq
         if (failed)
             return fail(ctx, OHPIC_EINVAL, "rejection sampling exhausted %llu draw budgets",
                         failed);
-        int rc = bin_from_staging(ctx, s, np[s]);
+        int rc = bin_from_staging(ctx, s, np[s], 0);
         if (rc) return rc;
     }
     ctx->rho_valid = false;
@@ -1061,10 +1087,11 @@ int ohpic_gather_push(ohpic_ctx *ctx)
     if (!ctx->binned)
         return fail(ctx, OHPIC_ESTATE, "gather_push needs binned particles: call boundary");
     HIPCHK(ctx, hipSetDevice(ctx->device));
-    HIPCHK(ctx, launch_fill_f64(ctx->h.diag, 2 * kMaxSpecies, 0.0, ctx->st));
     HIPCHK(ctx, launch_particles(kModeGatherPush, ctx->cfg.precision, ctx->cfg.weight_order,
                                  ctx->T, ctx->d_inst, ctx->d_items, ctx->nitems, ctx->par,
                                  ctx->st));
+    HIPCHK(ctx, launch_reduce_partials(ctx->h.ke_partial, ctx->nitems, kKeSlots, kKeSlots,
+                                       ctx->h.diag + kDiagKeOld, 0, ctx->st));
     HIPCHK(ctx, hipStreamSynchronize(ctx->st));
     ctx->binned = false;
     ctx->rho_valid = false;
@@ -1080,10 +1107,17 @@ int ohpic_boundary(ohpic_ctx *ctx)
     if ((rc = clear_flags(ctx))) return rc;
     HIPCHK(ctx, launch_particles(kModeBoundary, ctx->cfg.precision, ctx->cfg.weight_order, ctx->T,
                                  ctx->d_inst, ctx->d_items, ctx->nitems, ctx->par, ctx->st));
-    ctx->par ^= 1;
+    rc = check_rebin(ctx);
+    if (rc < 0) return rc;
+    if (rc == 1) {
+        // a bin overflowed: redo the wrap into a list and bin it into buffer par
+        if ((rc = rerun_as_list(ctx, kModeBoundaryList))) return rc;
+    } else {
+        ctx->par ^= 1;
+    }
     ctx->binned = true;
     ctx->rho_valid = false;
-    return check_after_rebin(ctx);
+    return OHPIC_OK;
 }
 
 int ohpic_step(ohpic_ctx *ctx, int nsteps)
@@ -1208,9 +1242,9 @@ int ohpic_fft3d_r2r(int device, int nx, int ny, int nz, int backward, const doub
     const int n[3] = {nx, ny, nz};
     hipStream_t st = nullptr;
     auto cleanup = [&]() {
-        for (void *p : tmp.allocs) hipFree(p);
+        for (void *p : tmp.allocs) (void)hipFree(p);
         tmp.allocs.clear();
-        if (st) hipStreamDestroy(st);
+        if (st) (void)hipStreamDestroy(st);
     };
     if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) {
         cleanup();

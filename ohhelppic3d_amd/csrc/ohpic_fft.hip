@@ -25,7 +25,6 @@ namespace ohpic {
 
 namespace {
 
-constexpr int kFftBlock = 256;
 
 struct cplx {
     double re, im;
@@ -36,120 +35,132 @@ __device__ __forceinline__ cplx cmul(cplx a, cplx b)
     return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
 }
 
-// One Stockham radix-p stage over `nlines` complex lines of length n held in
-// LDS.  Input layout per line: y_prev[(row + q*r)*Ls + f]; output
-// y[row*L + f + Ls*u] = sum_q (w_L^{f q} y_prev[...]) w_p^{u q}.
+// One Stockham radix-P stage, in place over `nlines` complex lines of length
+// n held in LDS.  Task (line, row, f): inputs y[(row + q*r)*Ls + f] times
+// w_L^{f q}, outputs y[row*L + f + Ls*u] = sum_q (...) w_p^{u q}.  Every
+// thread reads the inputs of all its tasks into registers, the block
+// synchronises, then the outputs are written back over the same buffer.
 template <int SIGN>
-__device__ void fft_stage(const cplx *__restrict__ src, cplx *__restrict__ dst, int n,
-                          int nlines, int p, int Ls, const double2 *__restrict__ tw)
+__device__ __forceinline__ cplx twiddle(const double2 *__restrict__ tw, int k)
 {
-    const int L = Ls * p;
-    const int r = n / L;
-    const int tasks_per_line = n / p;
-    const int total = tasks_per_line * nlines;
-    for (int t = threadIdx.x; t < total; t += blockDim.x) {
-        const int line = t / tasks_per_line;
-        const int i = t - line * tasks_per_line;
-        const int row = i / Ls;
-        const int f = i - row * Ls;
-        const cplx *s = src + line * n;
-        cplx *d = dst + line * n;
-        if (p == 4) {
-            cplx a[4];
+    const double2 w = tw[k];
+    return cplx{w.x, SIGN < 0 ? w.y : -w.y};
+}
+
+template <int SIGN, int P>
+__device__ __forceinline__ void butterfly(cplx *a, const double2 *__restrict__ tw, int step)
+{
+    if (P == 2) {
+        const cplx t = a[1];
+        a[1] = {a[0].re - t.re, a[0].im - t.im};
+        a[0] = {a[0].re + t.re, a[0].im + t.im};
+    } else if (P == 4) {
+        const cplx t0 = {a[0].re + a[2].re, a[0].im + a[2].im};
+        const cplx t1 = {a[0].re - a[2].re, a[0].im - a[2].im};
+        const cplx t2 = {a[1].re + a[3].re, a[1].im + a[3].im};
+        const cplx t3 = {a[1].re - a[3].re, a[1].im - a[3].im};
+        const cplx wt3 = {-SIGN * t3.im, SIGN * t3.re};  // (SIGN i) * t3
+        a[0] = {t0.re + t2.re, t0.im + t2.im};
+        a[1] = {t1.re + wt3.re, t1.im + wt3.im};
+        a[2] = {t0.re - t2.re, t0.im - t2.im};
+        a[3] = {t1.re - wt3.re, t1.im - wt3.im};
+    } else if (P == 3) {
+        const double sn = SIGN * 0.86602540378443864676372317075294;  // sin(2 pi/3)
+        const cplx ts = {a[1].re + a[2].re, a[1].im + a[2].im};
+        const cplx td = {a[1].re - a[2].re, a[1].im - a[2].im};
+        const cplx m = {a[0].re - 0.5 * ts.re, a[0].im - 0.5 * ts.im};
+        const cplx j = {-sn * td.im, sn * td.re};
+        a[0] = {a[0].re + ts.re, a[0].im + ts.im};
+        a[1] = {m.re + j.re, m.im + j.im};
+        a[2] = {m.re - j.re, m.im - j.im};
+    } else {
+        cplx o[P];
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                cplx v = s[(row + q * r) * Ls + f];
-                if (q > 0 && f > 0) {
-                    double2 w = tw[f * q * r];
-                    v = cmul(v, cplx{w.x, SIGN < 0 ? w.y : -w.y});
-                }
-                a[q] = v;
-            }
-            cplx t0 = {a[0].re + a[2].re, a[0].im + a[2].im};
-            cplx t1 = {a[0].re - a[2].re, a[0].im - a[2].im};
-            cplx t2 = {a[1].re + a[3].re, a[1].im + a[3].im};
-            cplx t3 = {a[1].re - a[3].re, a[1].im - a[3].im};
-            // w4 * t3 with w4 = SIGN * i
-            cplx wt3 = {-SIGN * t3.im, SIGN * t3.re};
-            const int ob = row * L + f;
-            d[ob] = {t0.re + t2.re, t0.im + t2.im};
-            d[ob + Ls] = {t1.re + wt3.re, t1.im + wt3.im};
-            d[ob + 2 * Ls] = {t0.re - t2.re, t0.im - t2.im};
-            d[ob + 3 * Ls] = {t1.re - wt3.re, t1.im - wt3.im};
-        } else if (p == 2) {
-            cplx a0 = s[row * Ls + f];
-            cplx a1 = s[(row + r) * Ls + f];
-            if (f > 0) {
-                double2 w = tw[f * r];
-                a1 = cmul(a1, cplx{w.x, SIGN < 0 ? w.y : -w.y});
-            }
-            const int ob = row * L + f;
-            d[ob] = {a0.re + a1.re, a0.im + a1.im};
-            d[ob + Ls] = {a0.re - a1.re, a0.im - a1.im};
-        } else if (p == 3) {
-            cplx a[3];
+        for (int u = 0; u < P; ++u) {
+            cplx acc = a[0];
 #pragma unroll
-            for (int q = 0; q < 3; ++q) {
-                cplx v = s[(row + q * r) * Ls + f];
-                if (q > 0 && f > 0) {
-                    double2 w = tw[f * q * r];
-                    v = cmul(v, cplx{w.x, SIGN < 0 ? w.y : -w.y});
-                }
-                a[q] = v;
+            for (int q = 1; q < P; ++q) {
+                const cplx v = cmul(a[q], twiddle<SIGN>(tw, ((u * q) % P) * step));
+                acc.re += v.re;
+                acc.im += v.im;
             }
-            const double c = -0.5;
-            const double sn = SIGN * 0.86602540378443864676372317075294;  // sin(2 pi/3)
-            cplx tsum = {a[1].re + a[2].re, a[1].im + a[2].im};
-            cplx tdif = {a[1].re - a[2].re, a[1].im - a[2].im};
-            cplx m = {a[0].re + c * tsum.re, a[0].im + c * tsum.im};
-            cplx j = {-sn * tdif.im, sn * tdif.re};  // i * sn * tdif
-            const int ob = row * L + f;
-            d[ob] = {a[0].re + tsum.re, a[0].im + tsum.im};
-            d[ob + Ls] = {m.re + j.re, m.im + j.im};
-            d[ob + 2 * Ls] = {m.re - j.re, m.im - j.im};
-        } else {
-            // generic small prime (p <= kMaxRadix)
-            cplx a[kMaxRadix];
-            for (int q = 0; q < p; ++q) {
-                cplx v = s[(row + q * r) * Ls + f];
-                if (q > 0 && f > 0) {
-                    double2 w = tw[f * q * r];
-                    v = cmul(v, cplx{w.x, SIGN < 0 ? w.y : -w.y});
-                }
-                a[q] = v;
-            }
-            const int step = n / p;
-            const int ob = row * L + f;
-            for (int u = 0; u < p; ++u) {
-                cplx acc = a[0];
-                for (int q = 1; q < p; ++q) {
-                    double2 w = tw[((u * q) % p) * step];
-                    cplx v = cmul(a[q], cplx{w.x, SIGN < 0 ? w.y : -w.y});
-                    acc.re += v.re;
-                    acc.im += v.im;
-                }
-                d[ob + Ls * u] = acc;
-            }
+            o[u] = acc;
         }
+#pragma unroll
+        for (int u = 0; u < P; ++u) a[u] = o[u];
     }
 }
 
-// Runs all stages; returns the buffer holding the result.
-template <int SIGN>
-__device__ cplx *fft_lds(cplx *a, cplx *b, int n, int nlines, const FftAxisDev &ax)
+template <int SIGN, int P>
+__device__ void fft_stage(cplx *__restrict__ buf, int n, int nlines, int Ls,
+                          const double2 *__restrict__ tw)
 {
-    cplx *src = a, *dst = b;
+    constexpr int KMAX = (kFftLdsCplx / kFftBlock + P - 1) / P;  // tasks per thread
+    const int L = Ls * P;
+    const int r = n / L;
+    const int tpl = n / P;
+    const int total = tpl * nlines;
+    cplx a[KMAX][P];
+#pragma unroll
+    for (int k = 0; k < KMAX; ++k) {
+        const int t = threadIdx.x + k * kFftBlock;
+        if (t < total) {
+            const int line = t / tpl;
+            const int i = t - line * tpl;
+            const int row = i / Ls;
+            const int f = i - row * Ls;
+            const cplx *s = buf + line * n;
+#pragma unroll
+            for (int q = 0; q < P; ++q) {
+                cplx v = s[(row + q * r) * Ls + f];
+                if (q > 0 && f > 0) v = cmul(v, twiddle<SIGN>(tw, f * q * r));
+                a[k][q] = v;
+            }
+            butterfly<SIGN, P>(a[k], tw, n / P);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < KMAX; ++k) {
+        const int t = threadIdx.x + k * kFftBlock;
+        if (t < total) {
+            const int line = t / tpl;
+            const int i = t - line * tpl;
+            const int row = i / Ls;
+            const int f = i - row * Ls;
+            cplx *d = buf + line * n + row * L + f;
+#pragma unroll
+            for (int u = 0; u < P; ++u) d[Ls * u] = a[k][u];
+        }
+    }
+    __syncthreads();
+}
+
+// Runs all stages in place.  RS selects the compiled radix set (0: 4 and 2;
+// 1: 4, 2 and 3; 2: every prime up to kMaxRadix) so power-of-two lengths do
+// not pay the register cost of the large generic stages.
+template <int SIGN, int RS>
+__device__ void fft_lds(cplx *buf, int n, int nlines, const FftAxisDev &ax)
+{
     int Ls = 1;
     for (int s = 0; s < ax.nfac; ++s) {
         const int p = ax.fac[s];
-        fft_stage<SIGN>(src, dst, n, nlines, p, Ls, ax.tw);
-        __syncthreads();
+        if (p == 4) {
+            fft_stage<SIGN, 4>(buf, n, nlines, Ls, ax.tw);
+        } else if (p == 2) {
+            fft_stage<SIGN, 2>(buf, n, nlines, Ls, ax.tw);
+        } else if (RS >= 1 && p == 3) {
+            fft_stage<SIGN, 3>(buf, n, nlines, Ls, ax.tw);
+        } else if (RS >= 2) {
+            switch (p) {
+            case 5: fft_stage<SIGN, 5>(buf, n, nlines, Ls, ax.tw); break;
+            case 7: fft_stage<SIGN, 7>(buf, n, nlines, Ls, ax.tw); break;
+            case 11: fft_stage<SIGN, 11>(buf, n, nlines, Ls, ax.tw); break;
+            default: fft_stage<SIGN, 13>(buf, n, nlines, Ls, ax.tw); break;
+            }
+        }
         Ls *= p;
-        cplx *t = src;
-        src = dst;
-        dst = t;
     }
-    return src;
 }
 
 // Line addressing for an axis pass.  Returns the base element of line g of
@@ -238,91 +249,111 @@ __device__ __forceinline__ double hc_value(const cplx *X, int n, int c, int whic
     }
 }
 
-// Build the packed complex spectrum Z = A + i B from halfcomplex pairs H
-// (H[c][m] = (hc_a[m], hc_b[m])).
-__device__ void build_packed(const cplx *H, cplx *Z, int n, int nlines)
+// Packed complex spectrum Z[k] = A[k] + i B[k] from halfcomplex pairs
+// H[c][m] = (hc_a[m], hc_b[m]).
+__device__ __forceinline__ cplx packed_from_hc(const cplx *H, int n, int c, int k)
 {
-    const int total = n * nlines;
-    for (int t = threadIdx.x; t < total; t += blockDim.x) {
-        const int c = t / n;
-        const int k = t - c * n;
-        const cplx hk = H[c * n + k];
-        const cplx hnk = H[c * n + (k == 0 ? 0 : n - k)];
-        double ra, ia, rb, ib;
-        if (k == 0 || 2 * k == n) {
-            ra = hk.re;
-            ia = 0.0;
-            rb = hk.im;
-            ib = 0.0;
-        } else if (2 * k < n) {
-            ra = hk.re;
-            ia = hnk.re;
-            rb = hk.im;
-            ib = hnk.im;
-        } else {
-            ra = hnk.re;
-            ia = -hk.re;
-            rb = hnk.im;
-            ib = -hk.im;
-        }
-        Z[t] = {ra - ib, ia + rb};
+    const cplx hk = H[c * n + k];
+    const cplx hnk = H[c * n + (k == 0 ? 0 : n - k)];
+    double ra, ia, rb, ib;
+    if (k == 0 || 2 * k == n) {
+        ra = hk.re;
+        ia = 0.0;
+        rb = hk.im;
+        ib = 0.0;
+    } else if (2 * k < n) {
+        ra = hk.re;
+        ia = hnk.re;
+        rb = hk.im;
+        ib = hnk.im;
+    } else {
+        ra = hnk.re;
+        ia = -hk.re;
+        rb = hnk.im;
+        ib = -hk.im;
     }
+    return cplx{ra - ib, ia + rb};
 }
 
-template <int KIND, int AXIS, typename SrcT>
+constexpr int kElemMax = kFftLdsCplx / kFftBlock;  // LDS elements per thread
+
+// In place: buf <- packed spectrum built from the halfcomplex pairs in buf.
+__device__ void build_packed_inplace(cplx *buf, int n, int nlines)
+{
+    const int total = n * nlines;
+    cplx z[kElemMax];
+#pragma unroll
+    for (int k = 0; k < kElemMax; ++k) {
+        const int t = threadIdx.x + k * kFftBlock;
+        if (t < total) {
+            const int c = t / n;
+            z[k] = packed_from_hc(buf, n, c, t - c * n);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < kElemMax; ++k) {
+        const int t = threadIdx.x + k * kFftBlock;
+        if (t < total) buf[t] = z[k];
+    }
+    __syncthreads();
+}
+
+template <int KIND, int AXIS, int RS, typename SrcT>
 __global__ void __launch_bounds__(kFftBlock)
     k_r2r_axis(const SrcT *__restrict__ src, double *__restrict__ dst, double src_scale,
                double out_scale, int nx, int ny, int nz, FftAxisDev ax, int G,
                const double *__restrict__ lamx, const double *__restrict__ lamy,
                const double *__restrict__ lamz)
 {
-    extern __shared__ cplx lds[];
+    __shared__ cplx buf[kFftLdsCplx];
     const int n = ax.n;
     const int nlines = (G + 1) / 2;
-    cplx *A = lds;
-    cplx *B = lds + nlines * n;
     const int grp = blockIdx.x;
     const int64_t estride = AXIS == 0 ? 1 : (AXIS == 1 ? (int64_t)nx : (int64_t)nx * ny);
 
-    load_lines<AXIS, SrcT>(src, src_scale, A, grp, G, n, estride, nx, ny, nz);
+    load_lines<AXIS, SrcT>(src, src_scale, buf, grp, G, n, estride, nx, ny, nz);
     __syncthreads();
 
-    cplx *R;
     if (KIND == kFftR2HC || KIND == kFftPoisson) {
-        cplx *X = fft_lds<-1>(A, B, n, nlines, ax);
-        if (KIND == kFftR2HC) {
-            R = X;
-        } else {
-            // halfcomplex pairs, divided by the Green's-function denominator
-            cplx *Y = (X == A) ? B : A;
+        fft_lds<-1, RS>(buf, n, nlines, ax);
+        if (KIND == kFftPoisson) {
+            // halfcomplex pairs divided by the Green's-function denominator
             const int total = n * nlines;
-            for (int t = threadIdx.x; t < total; t += blockDim.x) {
-                const int c = t / n;
-                const int m = t - c * n;
-                double va = hc_value(X, n, c, 0, m);
-                double vb = hc_value(X, n, c, 1, m);
-                LineGeom la, lb;
-                const bool oka = line_of<AXIS>(grp, 2 * c, G, nx, ny, nz, la);
-                const bool okb = line_of<AXIS>(grp, 2 * c + 1, G, nx, ny, nz, lb);
-                if (oka) {
-                    const double lam = lamx[la.ix] + lamy[la.iy] + lamz[m];
-                    va = (la.ix == 0 && la.iy == 0 && m == 0) ? 0.0 : va / lam;
+            cplx h[kElemMax];
+#pragma unroll
+            for (int k = 0; k < kElemMax; ++k) {
+                const int t = threadIdx.x + k * kFftBlock;
+                if (t < total) {
+                    const int c = t / n;
+                    const int m = t - c * n;
+                    double va = hc_value(buf, n, c, 0, m);
+                    double vb = hc_value(buf, n, c, 1, m);
+                    LineGeom la, lb;
+                    if (line_of<AXIS>(grp, 2 * c, G, nx, ny, nz, la)) {
+                        const double lam = lamx[la.ix] + lamy[la.iy] + lamz[m];
+                        va = (la.ix == 0 && la.iy == 0 && m == 0) ? 0.0 : va / lam;
+                    }
+                    if (line_of<AXIS>(grp, 2 * c + 1, G, nx, ny, nz, lb)) {
+                        const double lam = lamx[lb.ix] + lamy[lb.iy] + lamz[m];
+                        vb = (lb.ix == 0 && lb.iy == 0 && m == 0) ? 0.0 : vb / lam;
+                    }
+                    h[k] = {va, vb};
                 }
-                if (okb) {
-                    const double lam = lamx[lb.ix] + lamy[lb.iy] + lamz[m];
-                    vb = (lb.ix == 0 && lb.iy == 0 && m == 0) ? 0.0 : vb / lam;
-                }
-                Y[t] = {va, vb};
             }
             __syncthreads();
-            build_packed(Y, X, n, nlines);
+#pragma unroll
+            for (int k = 0; k < kElemMax; ++k) {
+                const int t = threadIdx.x + k * kFftBlock;
+                if (t < total) buf[t] = h[k];
+            }
             __syncthreads();
-            R = fft_lds<+1>(X, Y, n, nlines, ax);
+            build_packed_inplace(buf, n, nlines);
+            fft_lds<+1, RS>(buf, n, nlines, ax);
         }
     } else {
-        build_packed(A, B, n, nlines);
-        __syncthreads();
-        R = fft_lds<+1>(B, A, n, nlines, ax);
+        build_packed_inplace(buf, n, nlines);
+        fft_lds<+1, RS>(buf, n, nlines, ax);
     }
 
     // write back
@@ -334,9 +365,9 @@ __global__ void __launch_bounds__(kFftBlock)
         if (!line_of<AXIS>(grp, g, G, nx, ny, nz, lg)) continue;
         double v;
         if (KIND == kFftR2HC) {
-            v = hc_value(R, n, g >> 1, g & 1, j);
+            v = hc_value(buf, n, g >> 1, g & 1, j);
         } else {
-            const cplx z = R[(g >> 1) * n + j];
+            const cplx z = buf[(g >> 1) * n + j];
             v = (g & 1) ? z.im : z.re;
         }
         dst[lg.base + (int64_t)j * estride] = v * out_scale;
@@ -347,9 +378,9 @@ __global__ void __launch_bounds__(kFftBlock)
 
 int fft_group_lines(int n)
 {
-    // real lines per workgroup: keep 2 LDS buffers of G/2 complex lines of n
-    // under 64 KiB, at least 2 lines, at most 64
-    int g = 4096 / (n > 0 ? n : 1);
+    // real lines per workgroup: G/2 complex lines of n fill at most
+    // kFftLdsCplx complex values (32 KiB); at least 2 lines, at most 64
+    int g = 2 * (kFftLdsCplx / (n > 0 ? n : 1));
     if (g > 64) g = 64;
     if (g < 2) g = 2;
     return g & ~1;
@@ -357,7 +388,7 @@ int fft_group_lines(int n)
 
 size_t fft_lds_bytes(int n, int G)
 {
-    return (size_t)2 * ((G + 1) / 2) * n * sizeof(cplx);
+    return (size_t)((G + 1) / 2) * n * sizeof(cplx);
 }
 
 int64_t fft_groups(int axis, int G, int nx, int ny, int nz)
@@ -365,6 +396,16 @@ int64_t fft_groups(int axis, int G, int nx, int ny, int nz)
     if (axis == 0) return ((int64_t)ny * nz + G - 1) / G;
     if (axis == 1) return (int64_t)((nx + G - 1) / G) * nz;
     return ((int64_t)nx * ny + G - 1) / G;
+}
+
+int fft_radix_set(const FftAxisDev &ax)
+{
+    int rs = 0;
+    for (int i = 0; i < ax.nfac; ++i) {
+        if (ax.fac[i] == 3 && rs < 1) rs = 1;
+        if (ax.fac[i] > 4) rs = 2;
+    }
+    return rs;
 }
 
 template <int KIND, int AXIS, typename SrcT>
@@ -375,9 +416,21 @@ static hipError_t launch_one(const SrcT *src, double *dst, double src_scale, dou
     const int G = fft_group_lines(ax.n);
     const int64_t groups = fft_groups(AXIS, G, nx, ny, nz);
     if (groups <= 0) return hipSuccess;
-    const size_t lds = fft_lds_bytes(ax.n, G);
-    hipLaunchKernelGGL((k_r2r_axis<KIND, AXIS, SrcT>), dim3((unsigned)groups), dim3(kFftBlock),
-                       lds, st, src, dst, src_scale, out_scale, nx, ny, nz, ax, G, lx, ly, lz);
+    const dim3 g((unsigned)groups), b(kFftBlock);
+    switch (fft_radix_set(ax)) {
+    case 0:
+        hipLaunchKernelGGL((k_r2r_axis<KIND, AXIS, 0, SrcT>), g, b, 0, st, src, dst, src_scale,
+                           out_scale, nx, ny, nz, ax, G, lx, ly, lz);
+        break;
+    case 1:
+        hipLaunchKernelGGL((k_r2r_axis<KIND, AXIS, 1, SrcT>), g, b, 0, st, src, dst, src_scale,
+                           out_scale, nx, ny, nz, ax, G, lx, ly, lz);
+        break;
+    default:
+        hipLaunchKernelGGL((k_r2r_axis<KIND, AXIS, 2, SrcT>), g, b, 0, st, src, dst, src_scale,
+                           out_scale, nx, ny, nz, ax, G, lx, ly, lz);
+        break;
+    }
     return hipGetLastError();
 }
 

@@ -9,8 +9,10 @@
 
 namespace ohpic {
 
-constexpr int kMaxRadix = 31;   // largest prime factor an axis length may have
+constexpr int kMaxRadix = 13;   // largest prime factor an axis length may have
 constexpr int kMaxFactors = 24;
+constexpr int kFftBlock = 256;     // threads per FFT workgroup
+constexpr int kFftLdsCplx = 2048;  // complex values per workgroup (32 KiB LDS)
 constexpr int kMaxFftLen = 2048;
 
 enum FftKind : int { kFftR2HC = 0, kFftHC2R = 1, kFftPoisson = 2 };

@@ -37,39 +37,42 @@ __device__ __forceinline__ double wave_sum(double v)
 
 // E_a(i) = 0.5*((phi(i-e_a) - phi(i)) + (phi(i) - phi(i+e_a))): the edge
 // difference of poisson_field_solver.f90:139-144 averaged onto nodes as at
-// :165-170, periodic.  Also accumulates sum rho*phi, sum |E|^2 and sum rho.
+// :165-170, periodic.  Each block walks whole x rows (32-bit row indexing)
+// and writes its partial sums of rho*phi, |E|^2 and rho to partial[3*block];
+// k_reduce_partials adds them in block order.
 __global__ void __launch_bounds__(256)
     k_efield(const double *__restrict__ phi, const double *__restrict__ rho,
              double *__restrict__ ex, double *__restrict__ ey, double *__restrict__ ez, int nx,
-             int ny, int nz, double *__restrict__ diag)
+             int ny, int nz, double *__restrict__ partial)
 {
     __shared__ double red[3][4];
-    const int64_t ng = (int64_t)nx * ny * nz;
+    const int rows = ny * nz;
+    const int64_t sy = nx, sz = (int64_t)nx * ny;
     double we = 0.0, we2 = 0.0, qn = 0.0;
-    for (int64_t o = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; o < ng;
-         o += (int64_t)gridDim.x * blockDim.x) {
-        const int ix = (int)(o % nx);
-        const int64_t r = o / nx;
-        const int iy = (int)(r % ny);
-        const int iz = (int)(r / ny);
-        const int64_t sx = 1, sy = nx, sz = (int64_t)nx * ny;
-        const int64_t oxp = ix + 1 < nx ? o + sx : o - (int64_t)(nx - 1) * sx;
-        const int64_t oxm = ix > 0 ? o - sx : o + (int64_t)(nx - 1) * sx;
-        const int64_t oyp = iy + 1 < ny ? o + sy : o - (int64_t)(ny - 1) * sy;
-        const int64_t oym = iy > 0 ? o - sy : o + (int64_t)(ny - 1) * sy;
-        const int64_t ozp = iz + 1 < nz ? o + sz : o - (int64_t)(nz - 1) * sz;
-        const int64_t ozm = iz > 0 ? o - sz : o + (int64_t)(nz - 1) * sz;
-        const double p0 = phi[o];
-        const double vx = 0.5 * ((phi[oxm] - p0) + (p0 - phi[oxp]));
-        const double vy = 0.5 * ((phi[oym] - p0) + (p0 - phi[oyp]));
-        const double vz = 0.5 * ((phi[ozm] - p0) + (p0 - phi[ozp]));
-        ex[o] = vx;
-        ey[o] = vy;
-        ez[o] = vz;
-        const double q = rho[o];
-        we += q * p0;
-        we2 += vx * vx + vy * vy + vz * vz;
-        qn += q;
+    for (int r = blockIdx.x; r < rows; r += gridDim.x) {
+        const int iy = r % ny;
+        const int iz = r / ny;
+        const int64_t row = (int64_t)r * nx;
+        const int64_t dyp = iy + 1 < ny ? sy : -(int64_t)(ny - 1) * sy;
+        const int64_t dym = iy > 0 ? -sy : (int64_t)(ny - 1) * sy;
+        const int64_t dzp = iz + 1 < nz ? sz : -(int64_t)(nz - 1) * sz;
+        const int64_t dzm = iz > 0 ? -sz : (int64_t)(nz - 1) * sz;
+        for (int ix = threadIdx.x; ix < nx; ix += blockDim.x) {
+            const int64_t o = row + ix;
+            const int64_t oxp = ix + 1 < nx ? o + 1 : row;
+            const int64_t oxm = ix > 0 ? o - 1 : row + nx - 1;
+            const double p0 = phi[o];
+            const double vx = 0.5 * ((phi[oxm] - p0) + (p0 - phi[oxp]));
+            const double vy = 0.5 * ((phi[o + dym] - p0) + (p0 - phi[o + dyp]));
+            const double vz = 0.5 * ((phi[o + dzm] - p0) + (p0 - phi[o + dzp]));
+            ex[o] = vx;
+            ey[o] = vy;
+            ez[o] = vz;
+            const double q = rho[o];
+            we += q * p0;
+            we2 += vx * vx + vy * vy + vz * vz;
+            qn += q;
+        }
     }
     we = wave_sum(we);
     we2 = wave_sum(we2);
@@ -83,14 +86,35 @@ __global__ void __launch_bounds__(256)
     __syncthreads();
     if (threadIdx.x == 0) {
         double a = 0, b = 0, c = 0;
-        for (int i = 0; i < 4; ++i) {
+        for (int i = 0; i < (int)(blockDim.x >> 6); ++i) {
             a += red[0][i];
             b += red[1][i];
             c += red[2][i];
         }
-        atomicAdd(&diag[kDiagWe], a);
-        atomicAdd(&diag[kDiagWe2], b);
-        atomicAdd(&diag[kDiagQnet], c);
+        partial[3 * blockIdx.x + 0] = a;
+        partial[3 * blockIdx.x + 1] = b;
+        partial[3 * blockIdx.x + 2] = c;
+    }
+}
+
+// out[k] (+)= sum over i of partial[i*stride + k], k < width, summed in index
+// order by one block (deterministic).
+__global__ void __launch_bounds__(256)
+    k_reduce_partials(const double *__restrict__ partial, int count, int stride, int width,
+                      double *__restrict__ out, int accumulate)
+{
+    __shared__ double red[256];
+    for (int k = 0; k < width; ++k) {
+        double s = 0.0;
+        for (int i = threadIdx.x; i < count; i += blockDim.x) s += partial[(int64_t)i * stride + k];
+        red[threadIdx.x] = s;
+        __syncthreads();
+        for (int h = blockDim.x / 2; h > 0; h >>= 1) {
+            if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) out[k] = (accumulate ? out[k] : 0.0) + red[0];
+        __syncthreads();
     }
 }
 
@@ -242,12 +266,30 @@ hipError_t launch_rho_from_acc(const unsigned long long *acc, double *rho, int64
     return hipGetLastError();
 }
 
-hipError_t launch_efield(const double *phi, const double *rho, double *ex, double *ey,
-                         double *ez, int nx, int ny, int nz, double *diag, hipStream_t st)
+int efield_blocks(int ny, int nz)
 {
-    const int64_t n = (int64_t)nx * ny * nz;
-    hipLaunchKernelGGL(k_efield, dim3(grid_for(n)), dim3(256), 0, st, phi, rho, ex, ey, ez, nx,
-                       ny, nz, diag);
+    const int rows = ny * nz;
+    return rows < kEfieldMaxBlocks ? rows : kEfieldMaxBlocks;
+}
+
+hipError_t launch_efield(const double *phi, const double *rho, double *ex, double *ey,
+                         double *ez, int nx, int ny, int nz, double *partial, double *diag,
+                         hipStream_t st)
+{
+    const int blocks = efield_blocks(ny, nz);
+    hipLaunchKernelGGL(k_efield, dim3(blocks), dim3(256), 0, st, phi, rho, ex, ey, ez, nx, ny,
+                       nz, partial);
+    hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(256), 0, st, partial, blocks, 3, 3,
+                       diag + kDiagWe, 0);
+    return hipGetLastError();
+}
+
+hipError_t launch_reduce_partials(const double *partial, int count, int stride, int width,
+                                  double *out, int accumulate, hipStream_t st)
+{
+    if (count <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(256), 0, st, partial, count, stride,
+                       width, out, accumulate);
     return hipGetLastError();
 }
 

@@ -12,8 +12,8 @@
 //   charge    : two int64 fixed-point grids (current / next) of n nodes;
 //               integer sums make the deposit independent of summation order.
 //   fields    : rho, phi, Ex, Ey, Ez as double, n nodes each, x fastest.
-//   overflow  : a list of particles whose destination bin was full; a full
-//               re-bin (repack) drains it.
+//   a step whose exchange overflows a bin is discarded and re-run after a
+//   full re-bin (repack) of the untouched input buffer.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -37,11 +37,10 @@ enum DiagSlot : int {
 
 // flag slots (uint64) per instance
 enum FlagSlot : int {
-    kFlagOverflow = 0,  // particles appended to the overflow list (slot allocator)
+    kFlagOverflow = 0,  // particles that found their destination bin full
     kFlagFast = 1,      // particles whose new position left the deposit halo
-    kFlagLost = 2,      // overflow list full: particles dropped (fatal)
-    kFlagRange = 3,     // positions still outside [0, n) after two wraps (fatal)
-    kFlagSlots = 4
+    kFlagRange = 2,     // positions outside [0, n) (after the wraps when wrapping)
+    kFlagSlots = 3
 };
 
 struct SpecDev {
@@ -49,19 +48,11 @@ struct SpecDev {
     int64_t *pid[2];       // optional ids
     int64_t *bin_base;     // [ntiles]
     int32_t *bin_cap;      // [ntiles]
-    int32_t *count[2];     // [ntiles] particles per bin (may exceed cap: overflow)
+    unsigned long long *count[2];  // [ntiles] front | back << 32 (see above)
     double qm;             // charge-to-mass ratio
     double q;              // macro charge
     double mass;           // q / qm
     double qfix;           // q * 2^charge_bits
-};
-
-struct OvfDev {
-    void *comp[7];         // same component types as the particle buffers
-    int64_t *pid;
-    int32_t *dest;         // destination tile
-    int32_t *spec;         // species
-    int64_t cap;
 };
 
 struct InstDev {
@@ -83,16 +74,19 @@ struct InstDev {
     double *phi;
     double *e[3];
     double *diag;                 // kDiagSlots
+    double *ke_partial;           // [items][kKeSlots], reduced in item order
     unsigned long long *flags;    // kFlagSlots
-    OvfDev ovf;
 };
 
 struct WorkItem {
     int32_t inst;
     int32_t tile;
-    int32_t chunk;
-    int32_t nchunks;
+    int16_t chunk;     // this item's slice of every bin of the tile
+    int16_t nchunks;   // slices per bin
+    int32_t slot;      // row of the per-item partial-sum table
 };
+
+constexpr int kKeSlots = 2 * kMaxSpecies;  // per item: ke_old[8], ke_new[8]
 
 // ---------------------------------------------------------------- helpers
 __device__ __forceinline__ long long fix_round(double w, double qfix)
@@ -102,6 +96,19 @@ __device__ __forceinline__ long long fix_round(double w, double qfix)
     const double kMagic = 6755399441055744.0;  // 1.5 * 2^52
     double y = __fma_rn(w, qfix, kMagic);
     return __double_as_longlong(y) - 0x4338000000000000LL;
+}
+
+__host__ __device__ __forceinline__ int bin_front(unsigned long long w)
+{
+    return (int)(w & 0xffffffffull);
+}
+
+__host__ __device__ __forceinline__ int bin_back(unsigned long long w) { return (int)(w >> 32); }
+
+// slot of the v-th particle of a bin with front run F and back run B
+__device__ __forceinline__ int bin_slot(int v, int F, int B, int cap)
+{
+    return v < F ? v : cap - B + (v - F);
 }
 
 __device__ __forceinline__ int wrap_index(int i, int n)

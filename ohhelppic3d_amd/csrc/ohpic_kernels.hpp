@@ -13,7 +13,12 @@ enum ParticleMode : int {
     kModeFused = 0,       // gather + push + wrap + deposit(new x) + re-bin
     kModeDeposit = 1,     // deposit current positions
     kModeGatherPush = 2,  // gather + push in place (no wrap, no re-bin)
-    kModeBoundary = 3     // wrap + re-bin
+    kModeBoundary = 3,    // wrap + re-bin
+    // as kModeFused / kModeBoundary, but every particle goes to a contiguous
+    // list [0, count[par^1][0]) of buffer par^1 instead of a bin (the retry
+    // after a bin overflowed)
+    kModeFusedList = 4,
+    kModeBoundaryList = 5
 };
 
 bool tile_supported(int tx, int ty, int tz);
@@ -21,15 +26,15 @@ bool tile_supported(int tx, int ty, int tz);
 hipError_t launch_particles(int mode, int precision, int weight_order, const int T[3],
                             const InstDev *insts, const WorkItem *items, int nitems, int par,
                             hipStream_t st);
-hipError_t launch_bin_count(const InstDev *insts, int inst, int s, int nbuf, int64_t n,
-                            const int T[3], hipStream_t st);
-hipError_t launch_bin_scatter(const InstDev *insts, int inst, int s, int nbuf, int64_t n,
-                              const int T[3], hipStream_t st);
+// Bin a contiguous list [0, n) of buffer lbuf into the front runs of buffer
+// lbuf^1 (count first, then scatter once capacities are set); wrap != 0
+// applies the periodic wrap twice to each position first.
+hipError_t launch_bin_count(const InstDev *insts, int inst, int s, int lbuf, int64_t n,
+                            const int T[3], int wrap, hipStream_t st);
+hipError_t launch_bin_scatter(const InstDev *insts, int inst, int s, int lbuf, int64_t n,
+                              const int T[3], int wrap, hipStream_t st);
 hipError_t launch_bins_to_list(const InstDev *insts, int inst, int s, int src, int dst,
-                               int ntiles, const int64_t *obase, const int32_t *ocap,
-                               const int32_t *ocount, const int64_t *list_off, hipStream_t st);
-hipError_t launch_ovf_to_list(const InstDev *insts, int inst, int s, int dst, int64_t nov,
-                              int64_t off, unsigned long long *cursor, hipStream_t st);
+                               int ntiles, const int64_t *list_off, hipStream_t st);
 hipError_t launch_cell_counts(const InstDev *insts, int inst, int s, int par, int ntiles,
                               int32_t *counts, hipStream_t st);
 
@@ -40,8 +45,14 @@ hipError_t launch_fill_i32(int32_t *p, int64_t n, int32_t v, hipStream_t st);
 hipError_t launch_fill_f64(double *p, int64_t n, double v, hipStream_t st);
 hipError_t launch_rho_from_acc(const unsigned long long *acc, double *rho, int64_t n,
                                double scale, hipStream_t st);
+constexpr int kEfieldMaxBlocks = 4096;
+int efield_blocks(int ny, int nz);
+// partial: scratch of 3 * efield_blocks(ny, nz) doubles; writes diag[kDiagWe..kDiagQnet]
 hipError_t launch_efield(const double *phi, const double *rho, double *ex, double *ey,
-                         double *ez, int nx, int ny, int nz, double *diag, hipStream_t st);
+                         double *ez, int nx, int ny, int nz, double *partial, double *diag,
+                         hipStream_t st);
+hipError_t launch_reduce_partials(const double *partial, int count, int stride, int width,
+                                  double *out, int accumulate, hipStream_t st);
 
 // seeded loaders (ohpic_fields.hip)
 struct Pcg32Dev {

@@ -9,8 +9,11 @@
 //                                    ohhelppic3d.f90:353-355),
 //   deposits q*W(x_new) into an int64 fixed-point LDS copy of the tile's
 //   nodes plus halo (linear_scatter.f90:22-55), and
-//   appends the particle to the bin of its new tile in the other buffer
-//   (the replacement of OhHelp's correct_particle/transbound migration).
+//   re-bins it (the replacement of OhHelp's correct_particle/transbound):
+//   a particle that stays in its tile goes to the front run of the tile's
+//   bin in the other buffer (slot from an LDS counter), a particle that left
+//   goes through a per-wave LDS buffer to the back run of its new tile's bin
+//   (batched 64-bit atomics on the bin's front|back word).
 // The LDS charge block is flushed with one global 64-bit atomic per node.
 // Integer accumulation makes rho independent of particle order and timing.
 #include <hip/hip_runtime.h>
@@ -23,6 +26,14 @@
 namespace ohpic {
 
 namespace {
+
+// Minimum waves per SIMD the fused kernel is compiled for (register budget).
+#ifndef OHPIC_PUSH_MIN_WAVES
+#define OHPIC_PUSH_MIN_WAVES 1
+#endif
+
+constexpr int kWaves = kBlock / 64;
+constexpr int kLeaveCap = 32;  // per-wave buffer of particles that left the tile
 
 template <int WO>
 __device__ __forceinline__ void cic_w(double r, double &w0, double &w1)
@@ -49,14 +60,13 @@ __device__ __forceinline__ int lane_id() { return __lane_id(); }
 
 __device__ __forceinline__ double block_sum(double v, double *red)
 {
-    // wave64 reduction then across the 4 waves
     for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
     const int w = threadIdx.x >> 6;
     if (lane_id() == 0) red[w] = v;
     __syncthreads();
     double s = 0.0;
     if (threadIdx.x == 0)
-        for (int i = 0; i < kBlock / 64; ++i) s += red[i];
+        for (int i = 0; i < kWaves; ++i) s += red[i];
     __syncthreads();
     return s;
 }
@@ -66,49 +76,92 @@ struct P64 {
     int64_t id;
 };
 
-__device__ __forceinline__ void write_p64(const InstDev *__restrict__ I, const SpecDev &S, int s,
-                                          int nbuf, int t, int slot, const P64 &p, bool ids)
+__device__ __forceinline__ void store_p64(const SpecDev &S, int nbuf, int64_t o, const P64 &p,
+                                          bool ids)
 {
-    if (slot < S.bin_cap[t]) {
-        const int64_t o = S.bin_base[t] + slot;
-        ((double *)S.buf[nbuf][0])[o] = p.x;
-        ((double *)S.buf[nbuf][1])[o] = p.y;
-        ((double *)S.buf[nbuf][2])[o] = p.z;
-        ((double *)S.buf[nbuf][3])[o] = p.vx;
-        ((double *)S.buf[nbuf][4])[o] = p.vy;
-        ((double *)S.buf[nbuf][5])[o] = p.vz;
-        if (ids) S.pid[nbuf][o] = p.id;
-    } else {
-        const unsigned long long o = atomicAdd(&I->flags[kFlagOverflow], 1ull);
-        if ((int64_t)o < I->ovf.cap) {
-            ((double *)I->ovf.comp[0])[o] = p.x;
-            ((double *)I->ovf.comp[1])[o] = p.y;
-            ((double *)I->ovf.comp[2])[o] = p.z;
-            ((double *)I->ovf.comp[3])[o] = p.vx;
-            ((double *)I->ovf.comp[4])[o] = p.vy;
-            ((double *)I->ovf.comp[5])[o] = p.vz;
-            if (ids) I->ovf.pid[o] = p.id;
-            I->ovf.dest[o] = t;
-            I->ovf.spec[o] = s;
-        } else {
-            atomicAdd(&I->flags[kFlagLost], 1ull);
-        }
+    ((double *)S.buf[nbuf][0])[o] = p.x;
+    ((double *)S.buf[nbuf][1])[o] = p.y;
+    ((double *)S.buf[nbuf][2])[o] = p.z;
+    ((double *)S.buf[nbuf][3])[o] = p.vx;
+    ((double *)S.buf[nbuf][4])[o] = p.vy;
+    ((double *)S.buf[nbuf][5])[o] = p.vz;
+    if (ids) S.pid[nbuf][o] = p.id;
+}
+
+__device__ __forceinline__ P64 load_p64(const SpecDev &S, int b, int64_t o, bool ids)
+{
+    P64 q;
+    q.x = ((const double *)S.buf[b][0])[o];
+    q.y = ((const double *)S.buf[b][1])[o];
+    q.z = ((const double *)S.buf[b][2])[o];
+    q.vx = ((const double *)S.buf[b][3])[o];
+    q.vy = ((const double *)S.buf[b][4])[o];
+    q.vz = ((const double *)S.buf[b][5])[o];
+    q.id = ids ? S.pid[b][o] : 0;
+    return q;
+}
+
+// Append one particle to the back run of bin t (slot from the packed word).
+__device__ __forceinline__ void route_back(const SpecDev &S, int nbuf, int t, const P64 &q,
+                                           bool ids, unsigned long long *flags)
+{
+    const unsigned long long old = atomicAdd(&S.count[nbuf][t], 1ull << 32);
+    const int f = bin_front(old), b = bin_back(old);
+    const int cap = S.bin_cap[t];
+    if (b + f < cap)
+        store_p64(S, nbuf, S.bin_base[t] + (cap - 1 - b), q, ids);
+    else
+        atomicAdd(&flags[kFlagOverflow], 1ull);
+}
+
+struct LeaveBuf {
+    double c[6][kLeaveCap];
+    int64_t id[kLeaveCap];
+    int32_t dest[kLeaveCap];
+};
+
+// Route the first `cnt` buffered leavers of this wave (lane l takes entry l).
+__device__ __forceinline__ void flush_leavers(LeaveBuf &lb, int cnt, const SpecDev &S, int nbuf,
+                                              bool ids, unsigned long long *flags)
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const int l = lane_id();
+    if (l < cnt) {
+        P64 q;
+        q.x = lb.c[0][l];
+        q.y = lb.c[1][l];
+        q.z = lb.c[2][l];
+        q.vx = lb.c[3][l];
+        q.vy = lb.c[4][l];
+        q.vz = lb.c[5][l];
+        q.id = lb.id[l];
+        route_back(S, nbuf, lb.dest[l], q, ids, flags);
     }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
 template <int TX, int TY, int TZ, int WO, int MODE>
 __global__ void __launch_bounds__(kBlock)
+    __attribute__((amdgpu_waves_per_eu(OHPIC_PUSH_MIN_WAVES, 8)))
     k_particles_fp64(const InstDev *__restrict__ insts, const WorkItem *__restrict__ items,
                      int par)
 {
-    constexpr bool PUSH = (MODE == kModeFused || MODE == kModeGatherPush);
-    constexpr bool REBIN = (MODE == kModeFused || MODE == kModeBoundary);
-    constexpr bool DEP = (MODE == kModeFused || MODE == kModeDeposit);
+    constexpr bool LIST = (MODE == kModeFusedList || MODE == kModeBoundaryList);
+    constexpr bool PUSH = (MODE == kModeFused || MODE == kModeGatherPush || MODE == kModeFusedList);
+    constexpr bool REBIN = (MODE == kModeFused || MODE == kModeBoundary || LIST);
+    constexpr bool DEP = (MODE == kModeFused || MODE == kModeDeposit || MODE == kModeFusedList);
     constexpr int EX = TX + 1, EY = TY + 1, EZ = TZ + 1, NE = EX * EY * EZ;
     constexpr int RX = TX + 3, RY = TY + 3, RZ = TZ + 3, NR = RX * RY * RZ;
     __shared__ double sE[PUSH ? 3 * NE : 1];
     __shared__ unsigned long long sR[DEP ? NR : 1];
-    __shared__ double sRed[kBlock / 64];
+    __shared__ LeaveBuf sLeave[(REBIN && !LIST) ? kWaves : 1];
+    __shared__ int sLcnt[kWaves];
+    __shared__ double sRed[kWaves];
+    __shared__ int sFront, sFrontLimit;
 
     const WorkItem it = items[blockIdx.x];
     const InstDev *__restrict__ I = insts + it.inst;
@@ -118,7 +171,10 @@ __global__ void __launch_bounds__(kBlock)
     const int tx = tile % ntx, ty = (tile / ntx) % nty, tz = tile / (ntx * nty);
     const int lox = tx * TX, loy = ty * TY, loz = tz * TZ;
     const int tid = threadIdx.x;
+    const int wv = tid >> 6;
     const bool ids = I->track_ids != 0;
+    const bool excl = it.nchunks == 1;  // this workgroup alone fills the tile's front runs
+    unsigned long long *__restrict__ flags = I->flags;
 
     if (PUSH) {
         const double *__restrict__ ex = I->e[0];
@@ -135,6 +191,7 @@ __global__ void __launch_bounds__(kBlock)
     }
     if (DEP)
         for (int l = tid; l < NR; l += kBlock) sR[l] = 0ull;
+    if (tid < kWaves) sLcnt[tid] = 0;
     __syncthreads();
 
     const double dt = I->dt;
@@ -142,23 +199,35 @@ __global__ void __launch_bounds__(kBlock)
     const double bx = I->b0[0], by = I->b0[1], bz = I->b0[2];
     const int nbuf = par ^ 1;
     const unsigned long long lanemask_lt = (1ull << lane_id()) - 1ull;
-    unsigned long long *__restrict__ racc = I->rho_acc[MODE == kModeFused ? nbuf : par];
+    unsigned long long *__restrict__ racc =
+        I->rho_acc[(MODE == kModeFused || MODE == kModeFusedList) ? nbuf : par];
 
     for (int s = 0; s < I->nspecies; ++s) {
         const SpecDev &S = I->sp[s];
         const int cap = S.bin_cap[tile];
-        const int cnt = min(S.count[par][tile], cap);
+        const unsigned long long word = S.count[par][tile];
+        const int F = min(bin_front(word), cap);
+        const int B = min(bin_back(word), cap - F);
+        const int cnt = F + B;
         const int64_t base = S.bin_base[tile];
         int per = (cnt + it.nchunks - 1) / it.nchunks;
         per = (per + 63) & ~63;
         const int start = it.chunk * per;
         const int end = min(cnt, start + per);
-        const double *__restrict__ cx = (const double *)S.buf[par][0];
-        const double *__restrict__ cy = (const double *)S.buf[par][1];
-        const double *__restrict__ cz = (const double *)S.buf[par][2];
-        const double *__restrict__ cvx = (const double *)S.buf[par][3];
-        const double *__restrict__ cvy = (const double *)S.buf[par][4];
-        const double *__restrict__ cvz = (const double *)S.buf[par][5];
+        if (REBIN) {
+            if (tid == 0) {
+                int lim = cap;
+                if (!LIST && excl && cnt > 0) {
+                    // reserve the whole front run: arrivals from neighbours stay behind it
+                    const unsigned long long old =
+                        atomicAdd(&S.count[nbuf][tile], (unsigned long long)cnt);
+                    lim = cap - bin_back(old);
+                }
+                sFront = 0;
+                sFrontLimit = lim;
+            }
+            __syncthreads();
+        }
         const double qm = S.qm;
         const double qfix = S.qfix;
         // Boris constants for the uniform B (t = qm B dt/2, s = 2t/(1+|t|^2))
@@ -167,20 +236,15 @@ __global__ void __launch_bounds__(kBlock)
         const double s0 = 2 * t0 / (1 + tt), s1 = 2 * t1 / (1 + tt), s2 = 2 * t2 / (1 + tt);
         double keo = 0.0, ken = 0.0;
 
+        P64 nextq{0, 0, 0, 0, 0, 0, 0};
+        if (start + tid < end) nextq = load_p64(S, par, base + bin_slot(start + tid, F, B, cap), ids);
         for (int p0 = start; p0 < end; p0 += kBlock) {
             const int p = p0 + tid;
             const bool active = p < end;
-            P64 q{0, 0, 0, 0, 0, 0, 0};
-            if (active) {
-                const int64_t o = base + p;
-                q.x = cx[o];
-                q.y = cy[o];
-                q.z = cz[o];
-                q.vx = cvx[o];
-                q.vy = cvy[o];
-                q.vz = cvz[o];
-                if (ids) q.id = S.pid[par][o];
-            }
+            P64 q = nextq;
+            // prefetch the next iteration's particle while this one is processed
+            if (p + kBlock < end)
+                nextq = load_p64(S, par, base + bin_slot(p + kBlock, F, B, cap), ids);
             if (PUSH && active) {
                 const int ix = (int)q.x, iy = (int)q.y, iz = (int)q.z;
                 double wx0, wx1, wy0, wy1, wz0, wz1;
@@ -226,21 +290,7 @@ __global__ void __launch_bounds__(kBlock)
                 q.z = q.z + q.vz * dt;
             }
             if (MODE == kModeGatherPush) {
-                if (active) {
-                    const int64_t o = base + p;
-                    double *wx_ = (double *)S.buf[par][0];
-                    double *wy_ = (double *)S.buf[par][1];
-                    double *wz_ = (double *)S.buf[par][2];
-                    double *wvx = (double *)S.buf[par][3];
-                    double *wvy = (double *)S.buf[par][4];
-                    double *wvz = (double *)S.buf[par][5];
-                    wx_[o] = q.x;
-                    wy_[o] = q.y;
-                    wz_[o] = q.z;
-                    wvx[o] = q.vx;
-                    wvy[o] = q.vy;
-                    wvz[o] = q.vz;
-                }
+                if (active) store_p64(S, par, base + bin_slot(p, F, B, cap), q, ids);
                 continue;
             }
             int ntile = tile;
@@ -250,7 +300,7 @@ __global__ void __launch_bounds__(kBlock)
                 q.z = apply_periodic(apply_periodic(q.z, nz), nz);
                 if (!(q.x >= 0.0 && q.x < nx && q.y >= 0.0 && q.y < ny && q.z >= 0.0 &&
                       q.z < nz)) {
-                    atomicAdd(&I->flags[kFlagRange], 1ull);
+                    atomicAdd(&flags[kFlagRange], 1ull);
                     q.x = fmin(fmax(q.x - floor(q.x / nx) * nx, 0.0), nx - 1e-9);
                     q.y = fmin(fmax(q.y - floor(q.y / ny) * ny, 0.0), ny - 1e-9);
                     q.z = fmin(fmax(q.z - floor(q.z / nz) * nz, 0.0), nz - 1e-9);
@@ -281,7 +331,7 @@ __global__ void __launch_bounds__(kBlock)
                                           (unsigned long long)v);
                             }
                 } else {
-                    atomicAdd(&I->flags[kFlagFast], 1ull);
+                    atomicAdd(&flags[kFlagFast], 1ull);
 #pragma unroll
                     for (int i = 0; i < 2; ++i)
 #pragma unroll
@@ -296,32 +346,92 @@ __global__ void __launch_bounds__(kBlock)
                             }
                 }
             }
-            if (REBIN) {
+            if (LIST) {
+                // retry path: every particle to the species' contiguous list
+                const unsigned long long m = __ballot(active);
+                if (m) {
+                    const int leader = __ffsll((unsigned long long)m) - 1;
+                    unsigned long long b = 0;
+                    if (lane_id() == leader)
+                        b = atomicAdd(&S.count[nbuf][0], (unsigned long long)__popcll(m));
+                    b = __shfl(b, leader, 64);
+                    if (active) store_p64(S, nbuf, (int64_t)b + __popcll(m & lanemask_lt), q, ids);
+                }
+            } else if (REBIN) {
+                // particles that stayed: front run of this tile's bin
                 const bool stay = active && ntile == tile;
                 const unsigned long long m = __ballot(stay);
                 if (m) {
                     const int leader = __ffsll((unsigned long long)m) - 1;
-                    int b = 0;
-                    if (lane_id() == leader)
-                        b = atomicAdd(&S.count[nbuf][tile], (int)__popcll(m));
+                    const int nst = (int)__popcll(m);
+                    int b = 0, lim = 0;
+                    if (excl) {
+                        if (lane_id() == leader) b = atomicAdd(&sFront, nst);
+                        lim = sFrontLimit;
+                    } else if (lane_id() == leader) {
+                        const unsigned long long old =
+                            atomicAdd(&S.count[nbuf][tile], (unsigned long long)nst);
+                        b = bin_front(old);
+                        lim = cap - bin_back(old);
+                    }
                     b = __shfl(b, leader, 64);
+                    lim = __shfl(lim, leader, 64);
                     if (stay) {
                         const int slot = b + (int)__popcll(m & lanemask_lt);
-                        write_p64(I, S, s, nbuf, tile, slot, q, ids);
+                        if (slot < lim)
+                            store_p64(S, nbuf, base + slot, q, ids);
+                        else
+                            atomicAdd(&flags[kFlagOverflow], 1ull);
                     }
                 }
-                if (active && ntile != tile) {
-                    const int slot = atomicAdd(&S.count[nbuf][ntile], 1);
-                    write_p64(I, S, s, nbuf, ntile, slot, q, ids);
+                // particles that left: per-wave LDS buffer, routed in batches
+                const bool leave = active && ntile != tile;
+                const unsigned long long lm = __ballot(leave);
+                if (lm) {
+                    const int nl = (int)__popcll(lm);
+                    int c = sLcnt[wv];
+                    if (c + nl > kLeaveCap) {
+                        flush_leavers(sLeave[wv], c, S, nbuf, ids, flags);
+                        c = 0;
+                    }
+                    if (nl <= kLeaveCap) {
+                        if (leave) {
+                            const int slot = c + (int)__popcll(lm & lanemask_lt);
+                            LeaveBuf &lb = sLeave[wv];
+                            lb.c[0][slot] = q.x;
+                            lb.c[1][slot] = q.y;
+                            lb.c[2][slot] = q.z;
+                            lb.c[3][slot] = q.vx;
+                            lb.c[4][slot] = q.vy;
+                            lb.c[5][slot] = q.vz;
+                            lb.id[slot] = q.id;
+                            lb.dest[slot] = ntile;
+                        }
+                        c += nl;
+                    } else if (leave) {
+                        route_back(S, nbuf, ntile, q, ids, flags);
+                    }
+                    if (lane_id() == 0) sLcnt[wv] = c;
                 }
             }
+        }
+        if (REBIN && !LIST) {
+            flush_leavers(sLeave[wv], sLcnt[wv], S, nbuf, ids, flags);
+            __syncthreads();
+            if (tid < kWaves) sLcnt[tid] = 0;
+            if (tid == 0 && excl && cnt > sFront) {
+                // release the reserved but unused part of the front run
+                atomicAdd(&S.count[nbuf][tile],
+                          (unsigned long long)(-(long long)(cnt - sFront)));
+            }
+            __syncthreads();
         }
         if (PUSH) {
             const double so = block_sum(keo, sRed);
             const double sn = block_sum(ken, sRed);
             if (tid == 0) {
-                atomicAdd(&I->diag[kDiagKeOld + s], 0.5 * S.mass * so);
-                atomicAdd(&I->diag[kDiagKeNew + s], 0.5 * S.mass * sn);
+                I->ke_partial[(int64_t)it.slot * kKeSlots + s] = 0.5 * S.mass * so;
+                I->ke_partial[(int64_t)it.slot * kKeSlots + kMaxSpecies + s] = 0.5 * S.mass * sn;
             }
         }
     }
@@ -341,90 +451,85 @@ __global__ void __launch_bounds__(kBlock)
 }
 
 // ---------------------------------------------------------------- binning
-// Count particles of a contiguous staging list into bins (tile of the cell).
-__global__ void k_bin_count(const InstDev *__restrict__ insts, int inst, int s, int nbuf,
-                            int64_t n, int tx_, int ty_, int tz_)
+__device__ __forceinline__ bool in_box(const InstDev *I, double x, double y, double z)
+{
+    return x >= 0.0 && x < I->n[0] && y >= 0.0 && y < I->n[1] && z >= 0.0 && z < I->n[2];
+}
+
+// Count a contiguous list [0, n) of buffer lbuf into the bin words of buffer
+// lbuf^1 (front runs).  wrap != 0 applies the periodic wrap twice first.
+__global__ void k_bin_count(const InstDev *__restrict__ insts, int inst, int s, int lbuf,
+                            int64_t n, int tx_, int ty_, int tz_, int wrap)
 {
     const InstDev *I = insts + inst;
     const SpecDev &S = I->sp[s];
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const double x = ((const double *)S.buf[nbuf][0])[i];
-    const double y = ((const double *)S.buf[nbuf][1])[i];
-    const double z = ((const double *)S.buf[nbuf][2])[i];
-    if (!(x >= 0.0 && x < I->n[0] && y >= 0.0 && y < I->n[1] && z >= 0.0 && z < I->n[2])) {
+    double x = ((const double *)S.buf[lbuf][0])[i];
+    double y = ((const double *)S.buf[lbuf][1])[i];
+    double z = ((const double *)S.buf[lbuf][2])[i];
+    if (wrap) {
+        x = apply_periodic(apply_periodic(x, I->n[0]), I->n[0]);
+        y = apply_periodic(apply_periodic(y, I->n[1]), I->n[1]);
+        z = apply_periodic(apply_periodic(z, I->n[2]), I->n[2]);
+    }
+    if (!in_box(I, x, y, z)) {
         atomicAdd(&I->flags[kFlagRange], 1ull);
         return;
     }
     const int t = ((int)x) / tx_ + I->nt[0] * (((int)y) / ty_ + I->nt[1] * (((int)z) / tz_));
-    atomicAdd(&S.count[nbuf ^ 1][t], 1);
+    atomicAdd(&S.count[lbuf ^ 1][t], 1ull);
 }
 
-// Scatter a contiguous staging list (buffer nbuf, entries [0, n)) into the
-// bins of buffer nbuf^1; count[nbuf^1] must be zero on entry and ends as
-// the bin occupancy.
-__global__ void k_bin_scatter(const InstDev *__restrict__ insts, int inst, int s, int nbuf,
-                              int64_t n, int tx_, int ty_, int tz_)
+// Scatter the list into the front runs of buffer lbuf^1 (words zeroed on
+// entry; caps already sized from k_bin_count).
+__global__ void k_bin_scatter(const InstDev *__restrict__ insts, int inst, int s, int lbuf,
+                              int64_t n, int tx_, int ty_, int tz_, int wrap)
 {
     const InstDev *I = insts + inst;
     const SpecDev &S = I->sp[s];
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    P64 q;
-    q.x = ((const double *)S.buf[nbuf][0])[i];
-    q.y = ((const double *)S.buf[nbuf][1])[i];
-    q.z = ((const double *)S.buf[nbuf][2])[i];
-    q.vx = ((const double *)S.buf[nbuf][3])[i];
-    q.vy = ((const double *)S.buf[nbuf][4])[i];
-    q.vz = ((const double *)S.buf[nbuf][5])[i];
     const bool ids = I->track_ids != 0;
-    q.id = ids ? S.pid[nbuf][i] : 0;
-    if (!(q.x >= 0.0 && q.x < I->n[0] && q.y >= 0.0 && q.y < I->n[1] && q.z >= 0.0 &&
-          q.z < I->n[2]))
-        return;
+    P64 q = load_p64(S, lbuf, i, ids);
+    if (wrap) {
+        q.x = apply_periodic(apply_periodic(q.x, I->n[0]), I->n[0]);
+        q.y = apply_periodic(apply_periodic(q.y, I->n[1]), I->n[1]);
+        q.z = apply_periodic(apply_periodic(q.z, I->n[2]), I->n[2]);
+    }
+    if (!in_box(I, q.x, q.y, q.z)) return;
     const int t =
         ((int)q.x) / tx_ + I->nt[0] * (((int)q.y) / ty_ + I->nt[1] * (((int)q.z) / tz_));
-    const int slot = atomicAdd(&S.count[nbuf ^ 1][t], 1);
-    write_p64(I, S, s, nbuf ^ 1, t, slot, q, ids);
+    const unsigned long long old = atomicAdd(&S.count[lbuf ^ 1][t], 1ull);
+    const int slot = bin_front(old);
+    if (slot < S.bin_cap[t])
+        store_p64(S, lbuf ^ 1, S.bin_base[t] + slot, q, ids);
+    else
+        atomicAdd(&I->flags[kFlagOverflow], 1ull);
 }
 
-// Compact the bins of buffer `src` (old bases/caps given explicitly) into a
-// contiguous list [0, n) of buffer `dst` -- used to store particles and as
-// the first half of a repack.  One workgroup per bin.
+// Compact both runs of every bin of buffer src into a contiguous list of
+// buffer dst at list_off[t].  One workgroup per bin.
 __global__ void k_bins_to_list(const InstDev *__restrict__ insts, int inst, int s, int src,
-                               int dst, const int64_t *__restrict__ obase,
-                               const int32_t *__restrict__ ocap,
-                               const int32_t *__restrict__ ocount,
-                               const int64_t *__restrict__ list_off)
+                               int dst, const int64_t *__restrict__ list_off)
 {
     const InstDev *I = insts + inst;
     const SpecDev &S = I->sp[s];
     const int t = blockIdx.x;
-    const int cnt = min(ocount[t], ocap[t]);
-    const int64_t ib = obase[t];
+    const unsigned long long w = S.count[src][t];
+    const int cap = S.bin_cap[t];
+    const int F = min(bin_front(w), cap);
+    const int B = min(bin_back(w), cap - F);
+    const int64_t ib = S.bin_base[t];
     const int64_t ob = list_off[t];
     const bool ids = I->track_ids != 0;
-    for (int i = threadIdx.x; i < cnt; i += blockDim.x) {
+    for (int v = threadIdx.x; v < F + B; v += blockDim.x) {
+        const int64_t o = ib + bin_slot(v, F, B, cap);
 #pragma unroll
         for (int c = 0; c < 6; ++c)
-            ((double *)S.buf[dst][c])[ob + i] = ((const double *)S.buf[src][c])[ib + i];
-        if (ids) S.pid[dst][ob + i] = S.pid[src][ib + i];
+            ((double *)S.buf[dst][c])[ob + v] = ((const double *)S.buf[src][c])[o];
+        if (ids) S.pid[dst][ob + v] = S.pid[src][o];
     }
-}
-
-// Append overflow entries of species s to the contiguous list of buffer dst
-// at offset `off`.
-__global__ void k_ovf_to_list(const InstDev *__restrict__ insts, int inst, int s, int dst,
-                              int64_t nov, int64_t off, unsigned long long *cursor)
-{
-    const InstDev *I = insts + inst;
-    const SpecDev &S = I->sp[s];
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nov || I->ovf.spec[i] != s) return;
-    const int64_t o = off + (int64_t)atomicAdd(cursor, 1ull);
-#pragma unroll
-    for (int c = 0; c < 6; ++c) ((double *)S.buf[dst][c])[o] = ((const double *)I->ovf.comp[c])[i];
-    if (I->track_ids) S.pid[dst][o] = I->ovf.pid[i];
 }
 
 __global__ void k_cell_counts(const InstDev *__restrict__ insts, int inst, int s, int par,
@@ -433,13 +538,17 @@ __global__ void k_cell_counts(const InstDev *__restrict__ insts, int inst, int s
     const InstDev *I = insts + inst;
     const SpecDev &S = I->sp[s];
     const int t = blockIdx.x;
-    const int cnt = min(S.count[par][t], S.bin_cap[t]);
+    const unsigned long long w = S.count[par][t];
+    const int cap = S.bin_cap[t];
+    const int F = min(bin_front(w), cap);
+    const int B = min(bin_back(w), cap - F);
     const int64_t b = S.bin_base[t];
     const int nx = I->n[0], ny = I->n[1];
-    for (int i = threadIdx.x; i < cnt; i += blockDim.x) {
-        const double x = ((const double *)S.buf[par][0])[b + i];
-        const double y = ((const double *)S.buf[par][1])[b + i];
-        const double z = ((const double *)S.buf[par][2])[b + i];
+    for (int v = threadIdx.x; v < F + B; v += blockDim.x) {
+        const int64_t o = b + bin_slot(v, F, B, cap);
+        const double x = ((const double *)S.buf[par][0])[o];
+        const double y = ((const double *)S.buf[par][1])[o];
+        const double z = ((const double *)S.buf[par][2])[o];
         const int64_t c = (int)x + (int64_t)nx * ((int)y + (int64_t)ny * (int)z);
         atomicAdd(&counts[c], 1);
     }
@@ -468,6 +577,14 @@ static hipError_t launch_tile(int mode, const InstDev *insts, const WorkItem *it
         break;
     case kModeBoundary:
         hipLaunchKernelGGL((k_particles_fp64<TX, TY, TZ, WO, kModeBoundary>), g, b, 0, st,
+                           insts, items, par);
+        break;
+    case kModeFusedList:
+        hipLaunchKernelGGL((k_particles_fp64<TX, TY, TZ, WO, kModeFusedList>), g, b, 0, st,
+                           insts, items, par);
+        break;
+    case kModeBoundaryList:
+        hipLaunchKernelGGL((k_particles_fp64<TX, TY, TZ, WO, kModeBoundaryList>), g, b, 0, st,
                            insts, items, par);
         break;
     default:
@@ -501,43 +618,32 @@ hipError_t launch_particles(int mode, int precision, int weight_order, const int
     return hipErrorInvalidValue;
 }
 
-hipError_t launch_bin_count(const InstDev *insts, int inst, int s, int nbuf, int64_t n,
-                            const int T[3], hipStream_t st)
+hipError_t launch_bin_count(const InstDev *insts, int inst, int s, int lbuf, int64_t n,
+                            const int T[3], int wrap, hipStream_t st)
 {
     if (n <= 0) return hipSuccess;
     const int64_t blocks = (n + 255) / 256;
     hipLaunchKernelGGL(k_bin_count, dim3((unsigned)blocks), dim3(256), 0, st, insts, inst, s,
-                       nbuf, n, T[0], T[1], T[2]);
+                       lbuf, n, T[0], T[1], T[2], wrap);
     return hipGetLastError();
 }
 
-hipError_t launch_bin_scatter(const InstDev *insts, int inst, int s, int nbuf, int64_t n,
-                              const int T[3], hipStream_t st)
+hipError_t launch_bin_scatter(const InstDev *insts, int inst, int s, int lbuf, int64_t n,
+                              const int T[3], int wrap, hipStream_t st)
 {
     if (n <= 0) return hipSuccess;
     const int64_t blocks = (n + 255) / 256;
     hipLaunchKernelGGL(k_bin_scatter, dim3((unsigned)blocks), dim3(256), 0, st, insts, inst, s,
-                       nbuf, n, T[0], T[1], T[2]);
+                       lbuf, n, T[0], T[1], T[2], wrap);
     return hipGetLastError();
 }
 
 hipError_t launch_bins_to_list(const InstDev *insts, int inst, int s, int src, int dst,
-                               int ntiles, const int64_t *obase, const int32_t *ocap,
-                               const int32_t *ocount, const int64_t *list_off, hipStream_t st)
+                               int ntiles, const int64_t *list_off, hipStream_t st)
 {
     if (ntiles <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_bins_to_list, dim3(ntiles), dim3(256), 0, st, insts, inst, s, src, dst,
-                       obase, ocap, ocount, list_off);
-    return hipGetLastError();
-}
-
-hipError_t launch_ovf_to_list(const InstDev *insts, int inst, int s, int dst, int64_t nov,
-                              int64_t off, unsigned long long *cursor, hipStream_t st)
-{
-    if (nov <= 0) return hipSuccess;
-    const int64_t blocks = (nov + 255) / 256;
-    hipLaunchKernelGGL(k_ovf_to_list, dim3((unsigned)blocks), dim3(256), 0, st, insts, inst, s,
-                       dst, nov, off, cursor);
+                       list_off);
     return hipGetLastError();
 }
 
