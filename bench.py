@@ -31,30 +31,46 @@ def log(*a):
 
 
 WORKLOADS = {
-    # name: grid, ppc per species, precision
-    "uniform256_ppc64_fp64": dict(n=(256, 256, 256), ppc=64, precision=0),
-    "uniform128_ppc64_fp64": dict(n=(128, 128, 128), ppc=64, precision=0),
-    "uniform32_ppc16_fp64": dict(n=(32, 32, 32), ppc=16, precision=0),
+    # name: grid, mean ppc per species, precision, density hump amplitude
+    "uniform256_ppc64_fp64": dict(n=(256, 256, 256), ppc=64, precision=0, hump=0.0),
+    "uniform256_ppc64_mixed": dict(n=(256, 256, 256), ppc=64, precision=1, hump=0.0),
+    "hump256_ppc64_mixed": dict(n=(256, 256, 256), ppc=64, precision=1, hump=9.0),
+    "uniform128_ppc64_fp64": dict(n=(128, 128, 128), ppc=64, precision=0, hump=0.0),
+    "uniform128_ppc64_mixed": dict(n=(128, 128, 128), ppc=64, precision=1, hump=0.0),
+    "uniform32_ppc16_fp64": dict(n=(32, 32, 32), ppc=16, precision=0, hump=0.0),
 }
 
 
-def run_gpu(args):
+def make_sim(ohp, wl, args, track_ids=False):
+    n = wl["n"]
+    ppc = wl["ppc"]
+    npcl = ppc * n[0] * n[1] * n[2]
+    cfg = ohp.SimConfig(n=n, qm=[-1.0, 0.01], q_macro=[-1.0 / ppc, 1.0 / ppc],
+                        capacity=[npcl, npcl], dt=0.1, b0=(0.0, 0.0, 0.5),
+                        precision=wl["precision"], slack=args.slack, track_ids=track_ids,
+                        tile=tuple(args.tile), chunks=args.chunks)
+    sim = ohp.Simulation(cfg)
+    if wl["hump"] > 0:
+        # n(r) = n0 (1 + 9 exp(-|r-c|^2 / (2 16^2))), mean ppc over the box
+        sim.seed_hump_maxwell([npcl, npcl], [1.0, 0.1], wl["hump"], 16.0, seed=(42, 52),
+                              budget=2052)
+    else:
+        sim.seed_uniform_maxwell([npcl, npcl], [1.0, 0.1], seed=(42, 52))
+    return sim, cfg, npcl
+
+
+def run_gpu(args, workload=None):
     import ohhelppic3d_amd as ohp
 
-    wl = WORKLOADS[args.workload]
+    workload = workload or args.workload
+    wl = WORKLOADS[workload]
     n = wl["n"]
     ppc = wl["ppc"]
     ng = n[0] * n[1] * n[2]
-    npcl = ppc * ng
-    cfg = ohp.SimConfig(n=n, qm=[-1.0, 0.01], q_macro=[-1.0 / ppc, 1.0 / ppc],
-                        capacity=[npcl, npcl], dt=0.1, b0=(0.0, 0.0, 0.5),
-                        precision=wl["precision"], slack=args.slack, track_ids=False,
-                        tile=tuple(args.tile), chunks=args.chunks)
-    log(f"[bench] workload {args.workload}: grid {n}, {2 * npcl} particles, "
-        f"est. footprint {ohp.estimate_footprint(cfg) / 1e9:.1f} GB")
     t0 = time.time()
-    sim = ohp.Simulation(cfg)
-    sim.seed_uniform_maxwell([npcl, npcl], [1.0, 0.1], seed=(42, 52))
+    sim, cfg, npcl = make_sim(ohp, wl, args)
+    log(f"[bench] workload {workload}: grid {n}, {2 * npcl} particles, "
+        f"est. footprint {ohp.estimate_footprint(cfg) / 1e9:.1f} GB")
     log(f"[bench] created + loaded in {time.time() - t0:.1f} s, tile {sim.tile}, "
         f"device bytes {sim.memory_footprint() / 1e9:.1f} GB")
     for i in range(args.warmup):
@@ -85,7 +101,7 @@ def run_gpu(args):
     alg_bytes = 2 * sp_bytes * total_p + 24 * ng + 8 * ng
     achieved = alg_bytes / (push_avg * 1e-3) / 1e9 if push_avg > 0 else 0.0
     traffic = None
-    tfile = os.path.join(ROOT, "profiles", f"traffic_push_fused_{args.workload}.json")
+    tfile = os.path.join(ROOT, "profiles", f"traffic_push_fused_{workload}.json")
     if os.path.exists(tfile):
         with open(tfile) as f:
             traffic = json.load(f).get("hbm_bytes_per_launch")
@@ -103,7 +119,7 @@ def run_gpu(args):
         "vs_baseline": None,
         "dtype": "f64" if wl["precision"] == 0 else "f32+f64",
         "data": "synthetic (device-seeded PCG32 uniform Maxwellian)",
-        "config": {"workload": args.workload, "grid": list(n), "ppc_per_species": ppc,
+        "config": {"workload": workload, "grid": list(n), "ppc_per_species": ppc,
                    "species": 2, "particles": total_p, "dt": 0.1, "b0z": 0.5,
                    "tile": list(args.tile), "slack": args.slack},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -115,6 +131,43 @@ def run_gpu(args):
                    "repacks": diag["repacks"]},
     }
     return res
+
+
+def mixed_error_vs_fp64(args):
+    """FP64 and mixed runs from the same seeded state at 128^3 x 64 ppc:
+    relative L2 of rho, phi and E after the first solve (the loaded state)
+    and after the second (one push later), for both CIC weight orders."""
+    import ohhelppic3d_amd as ohp
+
+    out = {"setup": "128^3 x 64 ppc x 2 species, seed 42/52; step0 = fields of the "
+                    "loaded state, step1 = fields after one push"}
+    for wname, wo in (("reference_weights", ohp.WEIGHTS_REFERENCE),
+                      ("standard_weights", ohp.WEIGHTS_STANDARD)):
+        grids = {}
+        for prec in ("fp64", "mixed"):
+            wl = WORKLOADS[f"uniform128_ppc64_{prec}"]
+            n, ppc = wl["n"], wl["ppc"]
+            npcl = ppc * n[0] * n[1] * n[2]
+            cfg = ohp.SimConfig(n=n, qm=[-1.0, 0.01], q_macro=[-1.0 / ppc, 1.0 / ppc],
+                                capacity=[npcl, npcl], precision=wl["precision"],
+                                slack=args.slack, weight_order=wo, tile=tuple(args.tile))
+            sim = ohp.Simulation(cfg)
+            sim.seed_uniform_maxwell([npcl, npcl], [1.0, 0.1], seed=(42, 52))
+            res = []
+            for _ in range(2):
+                sim.step(1)
+                res.append([sim.grid(w) for w in (ohp.FIELD_RHO, ohp.FIELD_PHI, ohp.FIELD_EX,
+                                                  ohp.FIELD_EY, ohp.FIELD_EZ)])
+            sim.close()
+            grids[prec] = res
+        out[wname] = {}
+        for st in range(2):
+            e = {}
+            for k, a, b in zip(("rho", "phi", "Ex", "Ey", "Ez"), grids["mixed"][st],
+                               grids["fp64"][st]):
+                e[k] = float(np.linalg.norm(a - b) / np.linalg.norm(b))
+            out[wname][f"step{st}"] = e
+    return out
 
 
 def run_cpu_baseline(seconds: float):
@@ -174,6 +227,7 @@ def main():
     ap.add_argument("--chunks", type=int, default=0)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true")
     args = ap.parse_args()
     if args.copy_calibration:
@@ -184,6 +238,17 @@ def main():
               "(one MI355X holds the whole 256^3 domain)", file=sys.stderr)
         sys.exit(2)
     res = run_gpu(args)
+    if not args.no_mixed and args.workload == "uniform256_ppc64_fp64":
+        mixed = {}
+        for name in ("uniform256_ppc64_mixed", "hump256_ppc64_mixed"):
+            log(f"[bench] mixed-precision workload {name}")
+            r = run_gpu(args, name)
+            mixed[name] = {"value": r["value"], "ms_per_step": r["ms_per_step"],
+                           "vs_fp64": r["value"] / res["value"], "roofline": r["roofline"],
+                           "energy": r["energy"]}
+        log("[bench] mixed vs FP64 one-step error at 128^3")
+        mixed["error_vs_fp64"] = mixed_error_vs_fp64(args)
+        res["mixed"] = mixed
     if not args.no_cpu:
         log("[bench] timing the CPU baseline (oracle)")
         res["cpu_baseline"] = run_cpu_baseline(args.cpu_seconds)

@@ -360,7 +360,10 @@ int bin_from_staging(ohpic_ctx *ctx, int s, int64_t n, int wrap)
     HIPCHK(ctx, launch_fill_u64(cnt, nt, 0ull, ctx->st));
     int rc = clear_flags(ctx);
     if (rc) return rc;
-    HIPCHK(ctx, launch_bin_count(ctx->d_inst, 0, s, lbuf, n, ctx->T, wrap, ctx->st));
+    if (ctx->cfg.precision == OHPIC_MIXED)
+        HIPCHK(ctx, launch_bin_count_mx(ctx->d_inst, 0, s, lbuf, n, ctx->T, ctx->st));
+    else
+        HIPCHK(ctx, launch_bin_count(ctx->d_inst, 0, s, lbuf, n, ctx->T, wrap, ctx->st));
     std::vector<int64_t> F, B;
     if ((rc = download_words(ctx, cnt, F, B))) return rc;
     if ((rc = read_flags(ctx))) return rc;
@@ -370,7 +373,10 @@ int bin_from_staging(ohpic_ctx *ctx, int s, int64_t n, int wrap)
     rc = set_bins(ctx, s, F);
     if (rc) return rc;
     HIPCHK(ctx, launch_fill_u64(cnt, nt, 0ull, ctx->st));
-    HIPCHK(ctx, launch_bin_scatter(ctx->d_inst, 0, s, lbuf, n, ctx->T, wrap, ctx->st));
+    if (ctx->cfg.precision == OHPIC_MIXED)
+        HIPCHK(ctx, launch_bin_scatter_mx(ctx->d_inst, 0, s, lbuf, n, ctx->T, ctx->st));
+    else
+        HIPCHK(ctx, launch_bin_scatter(ctx->d_inst, 0, s, lbuf, n, ctx->T, wrap, ctx->st));
     rc = read_flags(ctx);
     if (rc) return rc;
     if (ctx->h_flags[kFlagOverflow])
@@ -397,8 +403,12 @@ int bins_to_staging(ohpic_ctx *ctx, int s, int64_t *nlisted)
     }
     HIPCHK(ctx, hipMemcpyAsync(ctx->d_tmp64, off.data(), nt * sizeof(int64_t),
                                hipMemcpyHostToDevice, ctx->st));
-    HIPCHK(ctx, launch_bins_to_list(ctx->d_inst, 0, s, ctx->par, ctx->par ^ 1, nt, ctx->d_tmp64,
-                                    ctx->st));
+    if (ctx->cfg.precision == OHPIC_MIXED)
+        HIPCHK(ctx, launch_bins_to_list_mx(ctx->d_inst, 0, s, ctx->par, ctx->par ^ 1, nt,
+                                           ctx->d_tmp64, ctx->st));
+    else
+        HIPCHK(ctx, launch_bins_to_list(ctx->d_inst, 0, s, ctx->par, ctx->par ^ 1, nt,
+                                        ctx->d_tmp64, ctx->st));
     HIPCHK(ctx, hipStreamSynchronize(ctx->st));
     *nlisted = tot;
     return OHPIC_OK;
@@ -576,6 +586,49 @@ void release(ohpic_ctx *ctx)
 
 }  // namespace
 
+// FP64 positions/velocities <-> the mixed-precision storage (int32 cell
+// packing ix | iy << 10 | iz << 20, FP32 offsets and velocities).  The cell
+// is the FP64 int() truncation and the offset is clamped below 1, so a
+// particle never changes cell in the conversion.  Positions outside [0, n)
+// get an invalid cell so the binning kernel rejects them.
+void to_mixed(ohpic_ctx *ctx, int64_t n, const double *const src[6], std::vector<int32_t> &cell,
+              std::vector<float> comp[6])
+{
+    cell.resize(n);
+    for (int k = 0; k < 6; ++k) comp[k].resize(n);
+    for (int64_t i = 0; i < n; ++i) {
+        int c[3];
+        bool bad = false;
+        for (int a = 0; a < 3; ++a) {
+            const double p = src[a][i];
+            const int na = ctx->n[a];
+            if (!(p >= 0.0 && p < (double)na)) {
+                bad = true;
+                c[a] = 0;
+                comp[a][i] = 0.0f;
+                continue;
+            }
+            c[a] = (int)p;
+            // keep the FP64 cell; clamp an offset that rounds up to 1
+            comp[a][i] = std::min((float)(p - (double)c[a]), 0.99999994f);
+        }
+        cell[i] = bad ? -1 : (c[0] | (c[1] << 10) | (c[2] << 20));
+        for (int a = 0; a < 3; ++a) comp[3 + a][i] = (float)src[3 + a][i];
+    }
+}
+
+void from_mixed(int64_t n, const std::vector<int32_t> &cell, const std::vector<float> comp[6],
+                double *const dst[6])
+{
+    for (int64_t i = 0; i < n; ++i) {
+        const int c[3] = {cell[i] & 1023, (cell[i] >> 10) & 1023, cell[i] >> 20};
+        for (int a = 0; a < 3; ++a) {
+            if (dst[a]) dst[a][i] = (double)c[a] + (double)comp[a][i];
+            if (dst[3 + a]) dst[3 + a][i] = (double)comp[3 + a][i];
+        }
+    }
+}
+
 // ============================================================== FFT helpers
 namespace ohpic {
 
@@ -661,9 +714,11 @@ int ohpic_create(const ohpic_config *cfg, ohpic_ctx **out)
         return fail(nullptr, OHPIC_EINVAL, "grid sizes must be >= 1");
     if (c.nspecies < 1 || c.nspecies > OHPIC_MAX_SPECIES)
         return fail(nullptr, OHPIC_EINVAL, "nspecies must be in [1, %d]", OHPIC_MAX_SPECIES);
-    if (c.precision != OHPIC_FP64)
-        return fail(nullptr, OHPIC_EINVAL, "precision mode %d not available in this build",
-                    c.precision);
+    if (c.precision != OHPIC_FP64 && c.precision != OHPIC_MIXED)
+        return fail(nullptr, OHPIC_EINVAL, "unknown precision mode %d", c.precision);
+    if (c.precision == OHPIC_MIXED && (c.nx > 1024 || c.ny > 1024 || c.nz > 1024))
+        return fail(nullptr, OHPIC_EINVAL, "mixed precision packs 10-bit cell coordinates: "
+                    "axis lengths must be <= 1024");
     if (c.weight_order != 0 && c.weight_order != 1)
         return fail(nullptr, OHPIC_EINVAL, "weight_order must be 0 or 1");
     for (int s = 0; s < c.nspecies; ++s) {
@@ -743,12 +798,20 @@ int ohpic_create(const ohpic_config *cfg, ohpic_ctx **out)
         S.capacity = c.capacity[s];
         S.alloc = species_alloc(c, s, nt);
         for (int b = 0; b < 2; ++b) {
-            for (int k = 0; k < 6; ++k) {
-                double *p = nullptr;
-                CREATE_CHK(dalloc(ctx, &p, S.alloc));
-                D.buf[b][k] = p;
+            if (c.precision == OHPIC_MIXED) {
+                for (int k = 0; k < 7; ++k) {
+                    uint32_t *p = nullptr;
+                    CREATE_CHK(dalloc(ctx, &p, S.alloc));
+                    D.buf[b][k] = p;
+                }
+            } else {
+                for (int k = 0; k < 6; ++k) {
+                    double *p = nullptr;
+                    CREATE_CHK(dalloc(ctx, &p, S.alloc));
+                    D.buf[b][k] = p;
+                }
+                D.buf[b][6] = nullptr;
             }
-            D.buf[b][6] = nullptr;
             D.pid[b] = nullptr;
             if (c.track_ids) CREATE_CHK(dalloc(ctx, &D.pid[b], S.alloc));
             CREATE_CHK(dalloc(ctx, &D.count[b], nt));
@@ -848,9 +911,22 @@ int ohpic_load_particles(ohpic_ctx *ctx, int ispec, int64_t n, const double *x, 
     const int stg = ctx->par ^ 1;
     SpecDev &D = ctx->h.sp[ispec];
     const double *src[6] = {x, y, z, vx, vy, vz};
-    for (int k = 0; k < 6 && n > 0; ++k)
-        HIPCHK(ctx, hipMemcpyAsync(D.buf[stg][k], src[k], n * sizeof(double),
-                                   hipMemcpyHostToDevice, ctx->st));
+    if (ctx->cfg.precision == OHPIC_MIXED) {
+        std::vector<int32_t> cell;
+        std::vector<float> comp[6];
+        to_mixed(ctx, n, src, cell, comp);
+        if (n > 0)
+            HIPCHK(ctx, hipMemcpyAsync(D.buf[stg][0], cell.data(), n * sizeof(int32_t),
+                                       hipMemcpyHostToDevice, ctx->st));
+        for (int k = 0; k < 6 && n > 0; ++k)
+            HIPCHK(ctx, hipMemcpyAsync(D.buf[stg][k + 1], comp[k].data(), n * sizeof(float),
+                                       hipMemcpyHostToDevice, ctx->st));
+        HIPCHK(ctx, hipStreamSynchronize(ctx->st));
+    } else {
+        for (int k = 0; k < 6 && n > 0; ++k)
+            HIPCHK(ctx, hipMemcpyAsync(D.buf[stg][k], src[k], n * sizeof(double),
+                                       hipMemcpyHostToDevice, ctx->st));
+    }
     if (ctx->cfg.track_ids && n > 0) {
         if (pid) {
             HIPCHK(ctx, hipMemcpyAsync(D.pid[stg], pid, n * sizeof(int64_t),
@@ -897,10 +973,26 @@ int ohpic_store_particles(ohpic_ctx *ctx, int ispec, int64_t cap, int64_t *n, do
     const int stg = ctx->par ^ 1;
     SpecDev &D = ctx->h.sp[ispec];
     double *dst[6] = {x, y, z, vx, vy, vz};
-    for (int k = 0; k < 6 && cnt > 0; ++k)
-        if (dst[k])
-            HIPCHK(ctx, hipMemcpyAsync(dst[k], D.buf[stg][k], cnt * sizeof(double),
+    if (ctx->cfg.precision == OHPIC_MIXED) {
+        std::vector<int32_t> cell(cnt);
+        std::vector<float> comp[6];
+        if (cnt > 0)
+            HIPCHK(ctx, hipMemcpyAsync(cell.data(), D.buf[stg][0], cnt * sizeof(int32_t),
                                        hipMemcpyDeviceToHost, ctx->st));
+        for (int k = 0; k < 6; ++k) {
+            comp[k].resize(cnt);
+            if (cnt > 0)
+                HIPCHK(ctx, hipMemcpyAsync(comp[k].data(), D.buf[stg][k + 1], cnt * sizeof(float),
+                                           hipMemcpyDeviceToHost, ctx->st));
+        }
+        HIPCHK(ctx, hipStreamSynchronize(ctx->st));
+        from_mixed(cnt, cell, comp, dst);
+    } else {
+        for (int k = 0; k < 6 && cnt > 0; ++k)
+            if (dst[k])
+                HIPCHK(ctx, hipMemcpyAsync(dst[k], D.buf[stg][k], cnt * sizeof(double),
+                                           hipMemcpyDeviceToHost, ctx->st));
+    }
     if (pid && cnt > 0) {
         if (!ctx->cfg.track_ids) return fail(ctx, OHPIC_ESTATE, "ids not tracked");
         HIPCHK(ctx, hipMemcpyAsync(pid, D.pid[stg], cnt * sizeof(int64_t), hipMemcpyDeviceToHost,
@@ -1000,11 +1092,17 @@ int ohpic_seed_uniform_maxwell(ohpic_ctx *ctx, uint64_t initstate, uint64_t init
         const int stg = ctx->par ^ 1;
         SpecDev &D = ctx->h.sp[s];
         // one random_fix draw, then 7 draws per particle
-        HIPCHK(ctx, launch_load_uniform(r0, offset + 1, np[s], ctx->n[0], ctx->n[1], ctx->n[2],
-                                        vth[s], (double *)D.buf[stg][0], (double *)D.buf[stg][1],
-                                        (double *)D.buf[stg][2], (double *)D.buf[stg][3],
-                                        (double *)D.buf[stg][4], (double *)D.buf[stg][5],
-                                        ctx->cfg.track_ids ? D.pid[stg] : nullptr, 0, ctx->st));
+        if (ctx->cfg.precision == OHPIC_MIXED)
+            HIPCHK(ctx, launch_load_uniform_mx(ctx->d_inst, 0, s, stg, r0, offset + 1, np[s],
+                                               vth[s], ctx->st));
+        else
+            HIPCHK(ctx, launch_load_uniform(r0, offset + 1, np[s], ctx->n[0], ctx->n[1],
+                                            ctx->n[2], vth[s], (double *)D.buf[stg][0],
+                                            (double *)D.buf[stg][1], (double *)D.buf[stg][2],
+                                            (double *)D.buf[stg][3], (double *)D.buf[stg][4],
+                                            (double *)D.buf[stg][5],
+                                            ctx->cfg.track_ids ? D.pid[stg] : nullptr, 0,
+                                            ctx->st));
         offset += 1 + 7 * (uint64_t)np[s];
         int rc = bin_from_staging(ctx, s, np[s], 0);
         if (rc) return rc;
@@ -1034,13 +1132,17 @@ int ohpic_seed_hump_maxwell(ohpic_ctx *ctx, uint64_t initstate, uint64_t initseq
         const int stg = ctx->par ^ 1;
         SpecDev &D = ctx->h.sp[s];
         HIPCHK(ctx, launch_fill_u64(ctx->d_cursor, 1, 0ull, ctx->st));
-        HIPCHK(ctx, launch_load_hump(r0, (uint64_t)s << 40, budget, np[s], ctx->n[0], ctx->n[1],
-                                     ctx->n[2], amp, sigma, vth[s], (double *)D.buf[stg][0],
-                                     (double *)D.buf[stg][1], (double *)D.buf[stg][2],
-                                     (double *)D.buf[stg][3], (double *)D.buf[stg][4],
-                                     (double *)D.buf[stg][5],
-                                     ctx->cfg.track_ids ? D.pid[stg] : nullptr, 0,
-                                     ctx->d_cursor, ctx->st));
+        if (ctx->cfg.precision == OHPIC_MIXED)
+            HIPCHK(ctx, launch_load_hump_mx(ctx->d_inst, 0, s, stg, r0, (uint64_t)s << 40, budget,
+                                            np[s], amp, sigma, vth[s], ctx->d_cursor, ctx->st));
+        else
+            HIPCHK(ctx, launch_load_hump(r0, (uint64_t)s << 40, budget, np[s], ctx->n[0],
+                                         ctx->n[1], ctx->n[2], amp, sigma, vth[s],
+                                         (double *)D.buf[stg][0], (double *)D.buf[stg][1],
+                                         (double *)D.buf[stg][2], (double *)D.buf[stg][3],
+                                         (double *)D.buf[stg][4], (double *)D.buf[stg][5],
+                                         ctx->cfg.track_ids ? D.pid[stg] : nullptr, 0,
+                                         ctx->d_cursor, ctx->st));
         unsigned long long failed = 0;
         HIPCHK(ctx, hipMemcpyAsync(&failed, ctx->d_cursor, sizeof failed, hipMemcpyDeviceToHost,
                                    ctx->st));
@@ -1313,8 +1415,12 @@ int ohpic_cell_counts(ohpic_ctx *ctx, int ispec, int32_t *counts)
     HIPCHK(ctx, launch_fill_i32(ctx->d_cellcounts, ctx->ng, 0, ctx->st));
     for (int s = 0; s < ctx->cfg.nspecies; ++s) {
         if (ispec >= 0 && s != ispec) continue;
-        HIPCHK(ctx, launch_cell_counts(ctx->d_inst, 0, s, ctx->par, ctx->ntiles, ctx->d_cellcounts,
-                                       ctx->st));
+        if (ctx->cfg.precision == OHPIC_MIXED)
+            HIPCHK(ctx, launch_cell_counts_mx(ctx->d_inst, 0, s, ctx->par, ctx->ntiles,
+                                              ctx->d_cellcounts, ctx->st));
+        else
+            HIPCHK(ctx, launch_cell_counts(ctx->d_inst, 0, s, ctx->par, ctx->ntiles,
+                                           ctx->d_cellcounts, ctx->st));
     }
     HIPCHK(ctx, hipMemcpyAsync(counts, ctx->d_cellcounts, ctx->ng * sizeof(int32_t),
                                hipMemcpyDeviceToHost, ctx->st));

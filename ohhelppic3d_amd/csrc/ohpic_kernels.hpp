@@ -21,6 +21,10 @@ enum ParticleMode : int {
     kModeBoundaryList = 5
 };
 
+struct Pcg32Dev {
+    uint64_t state, inc;
+};
+
 bool tile_supported(int tx, int ty, int tz);
 
 hipError_t launch_particles(int mode, int precision, int weight_order, const int T[3],
@@ -38,6 +42,19 @@ hipError_t launch_bins_to_list(const InstDev *insts, int inst, int s, int src, i
 hipError_t launch_cell_counts(const InstDev *insts, int inst, int s, int par, int ntiles,
                               int32_t *counts, hipStream_t st);
 
+// mixed-precision particles (ohpic_particles_mixed.hip)
+hipError_t launch_particles_mixed(int mode, int weight_order, const int T[3],
+                                  const InstDev *insts, const WorkItem *items, int nitems,
+                                  int par, hipStream_t st);
+hipError_t launch_bin_count_mx(const InstDev *insts, int inst, int s, int lbuf, int64_t n,
+                               const int T[3], hipStream_t st);
+hipError_t launch_bin_scatter_mx(const InstDev *insts, int inst, int s, int lbuf, int64_t n,
+                                 const int T[3], hipStream_t st);
+hipError_t launch_bins_to_list_mx(const InstDev *insts, int inst, int s, int src, int dst,
+                                  int ntiles, const int64_t *list_off, hipStream_t st);
+hipError_t launch_cell_counts_mx(const InstDev *insts, int inst, int s, int par, int ntiles,
+                                 int32_t *counts, hipStream_t st);
+
 // fields (ohpic_fields.hip)
 hipError_t launch_fill_u64(unsigned long long *p, int64_t n, unsigned long long v,
                            hipStream_t st);
@@ -54,13 +71,16 @@ hipError_t launch_efield(const double *phi, const double *rho, double *ex, doubl
 hipError_t launch_reduce_partials(const double *partial, int count, int stride, int width,
                                   double *out, int accumulate, hipStream_t st);
 
-// seeded loaders (ohpic_fields.hip)
-struct Pcg32Dev {
-    uint64_t state, inc;
-};
+// seeded loaders (ohpic_fields.hip, ohpic_particles_mixed.hip)
 hipError_t launch_load_uniform(Pcg32Dev rng0, uint64_t offset, int64_t np, int nx, int ny,
                                int nz, double vth, double *x, double *y, double *z, double *vx,
                                double *vy, double *vz, int64_t *pid, int64_t pid0,
+                               hipStream_t st);
+hipError_t launch_load_uniform_mx(const InstDev *insts, int inst, int s, int b, Pcg32Dev rng0,
+                                  uint64_t offset, int64_t np, double vth, hipStream_t st);
+hipError_t launch_load_hump_mx(const InstDev *insts, int inst, int s, int b, Pcg32Dev rng0,
+                               uint64_t offset, uint64_t budget, int64_t np, double amp,
+                               double sigma, double vth, unsigned long long *fail,
                                hipStream_t st);
 hipError_t launch_load_hump(Pcg32Dev rng0, uint64_t offset, uint64_t budget, int64_t np, int nx,
                             int ny, int nz, double amp, double sigma, double vth, double *x,
