@@ -37,7 +37,6 @@ extern "C" {
 #define OHPIC_EHIP -2     /* HIP runtime error */
 #define OHPIC_ENOMEM -3   /* device memory / bin capacity exhausted */
 #define OHPIC_ESTATE -4   /* call not valid in the current state */
-#define OHPIC_ELOST -5    /* particles lost (overflow buffer exhausted) */
 #define OHPIC_ENODEV -6   /* no usable GPU */
 
 /* precision modes */
@@ -74,7 +73,6 @@ typedef struct {
     int32_t track_ids;     /* keep a 64-bit particle id beside each particle */
     int32_t device;        /* HIP device ordinal */
     int32_t charge_bits;   /* fixed-point fraction bits of the charge grid; 0 = derived */
-    double overflow_frac;  /* overflow list size as a fraction of capacity; <= 0 = 0.05 */
 } ohpic_config;
 
 typedef struct {
@@ -161,6 +159,29 @@ int ohpic_put_field(ohpic_ctx *ctx, int which, const double *values, int nelem,
  * every axis divided by nx*ny*nz.  in and out may alias. */
 int ohpic_fft3d_r2r(int device, int nx, int ny, int nz, int backward, const double *in,
                     double *out);
+
+/* --- per-particle operators over host arrays -----------------------------------
+ * The reference's per-particle bindings, batched: each call uploads the
+ * arrays, runs one kernel on `device` and copies the results back.
+ * ohpic_boris_batch   : t_ParticleMoverBoris%move (particle_mover_boris.f90:21-64)
+ *                       eb holds 6 values per particle (Ex,Ey,Ez,Bx,By,Bz)
+ * ohpic_interp_batch  : t_LinearInterpolator%interp (linear_interpolator.f90:22-69)
+ *                       pos holds x,y,z per particle; values(nelem, lb:ub) in
+ *                       Fortran order; out holds nelem values per particle
+ * ohpic_scatter_batch : t_LinearScatter%scatter (linear_scatter.f90:22-55), adds
+ *                       amount(nelem) * W(x) into values(nelem, lb:ub)
+ * ohpic_wrap_batch    : apply_periodic (periodic_particle_boundary.f90:42-53)
+ *                       applied `applications` times to each p[i] (axis length nn)
+ * A stencil reaching outside lb:ub is an error (the reference reads out of
+ * bounds there). */
+int ohpic_boris_batch(int device, int64_t n, double *x, double *y, double *z, double *vx,
+                      double *vy, double *vz, const double *eb, double qm, double dt);
+int ohpic_interp_batch(int device, int64_t n, const double *pos, const double *values, int nelem,
+                       const int32_t lb[3], const int32_t ub[3], int weight_order, double *out);
+int ohpic_scatter_batch(int device, int64_t n, const double *pos, double *values, int nelem,
+                        const int32_t lb[3], const int32_t ub[3], const double *amount,
+                        int weight_order);
+int ohpic_wrap_batch(int device, int64_t n, double *p, int32_t nn, int applications);
 
 /* --- diagnostics ------------------------------------------------------------*/
 int ohpic_diagnostics(ohpic_ctx *ctx, ohpic_diag *out);

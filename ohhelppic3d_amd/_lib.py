@@ -16,7 +16,7 @@ LIB_PATH = os.environ.get("OHPIC_LIB", os.path.join(PKG, "libohpic.so"))
 
 MAX_SPECIES = 8
 
-OK, EINVAL, EHIP, ENOMEM, ESTATE, ELOST, ENODEV = 0, -1, -2, -3, -4, -5, -6
+OK, EINVAL, EHIP, ENOMEM, ESTATE, ENODEV = 0, -1, -2, -3, -4, -6
 FP64, MIXED = 0, 1
 WEIGHTS_REFERENCE, WEIGHTS_STANDARD = 0, 1
 FIELD_RHO, FIELD_PHI, FIELD_EX, FIELD_EY, FIELD_EZ, FIELD_EB = range(6)
@@ -30,7 +30,8 @@ EXPORTS = (
     "ohpic_field_solve", "ohpic_gather_push", "ohpic_boundary", "ohpic_step",
     "ohpic_synchronize", "ohpic_get_field", "ohpic_put_field", "ohpic_fft3d_r2r",
     "ohpic_diagnostics", "ohpic_cell_counts", "ohpic_enable_timing", "ohpic_kernel_times",
-    "ohpic_memory_footprint", "ohpic_estimate_footprint",
+    "ohpic_memory_footprint", "ohpic_estimate_footprint", "ohpic_boris_batch",
+    "ohpic_interp_batch", "ohpic_scatter_batch", "ohpic_wrap_batch",
 )
 
 
@@ -57,7 +58,6 @@ class Config(C.Structure):
         ("track_ids", C.c_int32),
         ("device", C.c_int32),
         ("charge_bits", C.c_int32),
-        ("overflow_frac", C.c_double),
     ]
 
 
@@ -135,6 +135,10 @@ def lib():
         "ohpic_kernel_times": [vp, C.c_int, vp, dp, i64p, C.POINTER(C.c_int)],
         "ohpic_memory_footprint": [vp, i64p],
         "ohpic_estimate_footprint": [C.POINTER(Config), i64p],
+        "ohpic_boris_batch": [C.c_int, i64, dp, dp, dp, dp, dp, dp, dp, dbl, dbl],
+        "ohpic_interp_batch": [C.c_int, i64, dp, dp, C.c_int, i32p, i32p, C.c_int, dp],
+        "ohpic_scatter_batch": [C.c_int, i64, dp, dp, C.c_int, i32p, i32p, dp, C.c_int],
+        "ohpic_wrap_batch": [C.c_int, i64, dp, C.c_int32, C.c_int],
     }
     for name in EXPORTS:
         f = getattr(L, name)

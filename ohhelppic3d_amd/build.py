@@ -17,7 +17,7 @@ OBJ = os.path.join(PKG, "_build")
 LIB = os.path.join(PKG, "libohpic.so")
 
 SOURCES = ["ohpic_particles.hip", "ohpic_particles_mixed.hip", "ohpic_fields.hip",
-           "ohpic_fft.hip", "ohpic_api.cpp"]
+           "ohpic_fft.hip", "ohpic_ops.hip", "ohpic_api.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("OHPIC_ARCH", "gfx950")
@@ -43,7 +43,7 @@ def _compile(src: str) -> str:
 def build(verbose: bool = False) -> str:
     """Compile every source for gfx950 and link libohpic.so; returns its path."""
     os.makedirs(OBJ, exist_ok=True)
-    with ThreadPoolExecutor(max_workers=min(5, len(SOURCES))) as ex:
+    with ThreadPoolExecutor(max_workers=min(6, len(SOURCES))) as ex:
         objs = list(ex.map(_compile, SOURCES))
     tmp = LIB + ".tmp"
     cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs]

@@ -632,6 +632,12 @@ void from_mixed(int64_t n, const std::vector<int32_t> &cell, const std::vector<f
 // ============================================================== FFT helpers
 namespace ohpic {
 
+void set_global_error(const char *msg)
+{
+    std::lock_guard<std::mutex> lk(g_err_mu);
+    g_err = msg ? msg : "";
+}
+
 bool fft_factorize(int n, int *fac, int *nfac)
 {
     if (n < 1 || n > kMaxFftLen) return false;
@@ -685,7 +691,6 @@ int ohpic_config_default(ohpic_config *cfg)
     cfg->precision = OHPIC_FP64;
     cfg->weight_order = OHPIC_WEIGHTS_REFERENCE;
     cfg->slack = 0.15;
-    cfg->overflow_frac = 0.05;
     return OHPIC_OK;
 }
 

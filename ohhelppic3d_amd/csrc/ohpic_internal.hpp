@@ -88,6 +88,9 @@ struct WorkItem {
 
 constexpr int kKeSlots = 2 * kMaxSpecies;  // per item: ke_old[8], ke_new[8]
 
+// Records the message ohpic_last_error(NULL) returns (host side).
+void set_global_error(const char *msg);
+
 // ---------------------------------------------------------------- helpers
 __device__ __forceinline__ long long fix_round(double w, double qfix)
 {

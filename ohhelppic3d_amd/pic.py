@@ -50,7 +50,6 @@ class SimConfig:
     track_ids: bool = False
     device: int = 0
     charge_bits: int = 0
-    overflow_frac: float = 0.05
 
     def to_c(self) -> L.Config:
         c = L.Config()
@@ -75,7 +74,6 @@ class SimConfig:
         c.track_ids = 1 if self.track_ids else 0
         c.device = self.device
         c.charge_bits = self.charge_bits
-        c.overflow_frac = self.overflow_frac
         return c
 
 
