@@ -56,6 +56,7 @@ extern "C" {
 #define OHPIC_FIELD_EB 5  /* 6 components: Ex, Ey, Ez, B0x, B0y, B0z */
 
 typedef struct ohpic_ctx ohpic_ctx;
+typedef struct ohpic_group ohpic_group;
 
 typedef struct {
     int32_t nx, ny, nz;
@@ -196,6 +197,47 @@ int ohpic_kernel_times(ohpic_ctx *ctx, int n, char names[][32], double *total_ms
 int ohpic_memory_footprint(ohpic_ctx *ctx, int64_t *bytes);
 /* Predicted device bytes for a configuration (used for admission). */
 int ohpic_estimate_footprint(const ohpic_config *cfg, int64_t *bytes);
+
+/* --- groups: many independent instances advanced together ------------------
+ * No counterpart in the reference (it runs one simulation per MPI job).  A
+ * group holds contexts of one device, precision, weight order and tile shape
+ * and advances all of them with one launch per kernel and step: the particle
+ * kernel walks a work list of (instance, tile, chunk) items of every member,
+ * the FFT passes a table of per-instance transforms of different sizes.  The
+ * per-step launch sequence is captured once in a hipGraph and replayed until
+ * membership or a member's bins change.  Each member's result is the same as
+ * when it is stepped alone with ohpic_step (charge deposit is integer, so the
+ * fields and particles match bit for bit; the kinetic-energy sums may differ
+ * in the last bits because particle order inside a bin is not fixed).
+ * Members stay usable through the single-context calls between group steps.
+ * Group errors: ohpic_group_last_error(g). */
+int ohpic_group_create(int device, ohpic_group **out);
+int ohpic_group_destroy(ohpic_group *g); /* detaches every member */
+int ohpic_group_attach(ohpic_group *g, ohpic_ctx *ctx);
+int ohpic_group_detach(ohpic_group *g, ohpic_ctx *ctx);
+int ohpic_group_size(const ohpic_group *g, int *n);
+/* advance every member by nsteps (same sequence as ohpic_step) */
+int ohpic_group_step(ohpic_group *g, int nsteps);
+/* enable != 0 (default): replay captured hipGraphs; 0: launch directly */
+int ohpic_group_use_graph(ohpic_group *g, int enable);
+/* steps taken, graph captures, work items of the current table */
+int ohpic_group_stats(const ohpic_group *g, int64_t *steps, int64_t *captures, int64_t *items);
+const char *ohpic_group_last_error(const ohpic_group *g);
+
+/* --- autotuner ---------------------------------------------------------------
+ * No counterpart in the reference.  Times the group step over candidate tile
+ * shapes (4x4x4, 8x8x8, 8x8x4) and chunks per tile (1, 2, 4) on up to `sample`
+ * of the n configurations (evenly spaced), each seeded with capacity[s]
+ * Maxwellian particles, and returns the fastest (tile, chunks) with its
+ * measured particle-steps/s.  The configurations' own tile/chunks are ignored. */
+typedef struct {
+    int32_t tile[3];
+    int32_t chunks;
+    double particle_steps_per_s;
+    int32_t candidates; /* candidates timed */
+} ohpic_tune_result;
+int ohpic_autotune(const ohpic_config *cfgs, int n, int sample, int steps,
+                   ohpic_tune_result *out);
 
 #ifdef __cplusplus
 }

@@ -7,13 +7,14 @@ field-solver interfaces.  A Fortran iso_c_binding mirror lives in fortran/.
 """
 from ._lib import (FIELD_EB, FIELD_EX, FIELD_EY, FIELD_EZ, FIELD_PHI, FIELD_RHO, FP64, MIXED,
                    WEIGHTS_REFERENCE, WEIGHTS_STANDARD, OhpicError, available)
-from .pic import (LinearInterpolator, LinearScatter, ParticleBoundaries, ParticleMoverBoris,
+from .pic import (Group, LinearInterpolator, LinearScatter, ParticleBoundaries, ParticleMoverBoris,
                   PoissonFieldSolver, SimConfig, Simulation, charge_per_macro_particle,
-                  create_particle_mover, estimate_footprint, fft3d_r2r, mainstep)
+                  autotune, create_particle_mover, estimate_footprint, fft3d_r2r, mainstep)
 
 __all__ = [
     "FIELD_EB", "FIELD_EX", "FIELD_EY", "FIELD_EZ", "FIELD_PHI", "FIELD_RHO", "FP64", "MIXED",
-    "WEIGHTS_REFERENCE", "WEIGHTS_STANDARD", "OhpicError", "available", "LinearInterpolator",
+    "WEIGHTS_REFERENCE", "WEIGHTS_STANDARD", "OhpicError", "available", "Group", "autotune",
+    "LinearInterpolator",
     "LinearScatter", "ParticleBoundaries", "ParticleMoverBoris", "PoissonFieldSolver",
     "SimConfig", "Simulation", "charge_per_macro_particle", "create_particle_mover",
     "estimate_footprint", "fft3d_r2r", "mainstep",

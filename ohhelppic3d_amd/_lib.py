@@ -32,6 +32,9 @@ EXPORTS = (
     "ohpic_diagnostics", "ohpic_cell_counts", "ohpic_enable_timing", "ohpic_kernel_times",
     "ohpic_memory_footprint", "ohpic_estimate_footprint", "ohpic_boris_batch",
     "ohpic_interp_batch", "ohpic_scatter_batch", "ohpic_wrap_batch",
+    "ohpic_group_create", "ohpic_group_destroy", "ohpic_group_attach", "ohpic_group_detach",
+    "ohpic_group_size", "ohpic_group_step", "ohpic_group_use_graph", "ohpic_group_stats",
+    "ohpic_group_last_error", "ohpic_autotune",
 )
 
 
@@ -72,6 +75,15 @@ class Diag(C.Structure):
         ("total_energy", C.c_double),
         ("repacks", C.c_int64),
         ("fast_particles", C.c_int64),
+    ]
+
+
+class TuneResult(C.Structure):
+    _fields_ = [
+        ("tile", C.c_int32 * 3),
+        ("chunks", C.c_int32),
+        ("particle_steps_per_s", C.c_double),
+        ("candidates", C.c_int32),
     ]
 
 
@@ -139,10 +151,19 @@ def lib():
         "ohpic_interp_batch": [C.c_int, i64, dp, dp, C.c_int, i32p, i32p, C.c_int, dp],
         "ohpic_scatter_batch": [C.c_int, i64, dp, dp, C.c_int, i32p, i32p, dp, C.c_int],
         "ohpic_wrap_batch": [C.c_int, i64, dp, C.c_int32, C.c_int],
+        "ohpic_group_create": [C.c_int, C.POINTER(vp)],
+        "ohpic_group_destroy": [vp],
+        "ohpic_group_attach": [vp, vp],
+        "ohpic_group_detach": [vp, vp],
+        "ohpic_group_size": [vp, C.POINTER(C.c_int)],
+        "ohpic_group_step": [vp, C.c_int],
+        "ohpic_group_use_graph": [vp, C.c_int],
+        "ohpic_group_stats": [vp, i64p, i64p, i64p],
+        "ohpic_autotune": [C.POINTER(Config), C.c_int, C.c_int, C.c_int, C.POINTER(TuneResult)],
     }
     for name in EXPORTS:
         f = getattr(L, name)
-        if name == "ohpic_last_error":
+        if name in ("ohpic_last_error", "ohpic_group_last_error"):
             f.argtypes = [vp]
             f.restype = C.c_char_p
             continue
@@ -155,6 +176,12 @@ def lib():
 def check(rc: int, ctx=None) -> None:
     if rc != OK:
         msg = lib().ohpic_last_error(ctx)
+        raise OhpicError(rc, msg.decode() if msg else "")
+
+
+def check_group(rc: int, group) -> None:
+    if rc != OK:
+        msg = lib().ohpic_group_last_error(group)
         raise OhpicError(rc, msg.decode() if msg else "")
 
 

@@ -11,6 +11,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -72,9 +73,17 @@ struct ohpic_ctx {
     int64_t *d_tmp64b = nullptr; // [ntiles]
     unsigned long long *d_cursor = nullptr;
     int32_t *d_cellcounts = nullptr;
-    double *d_epartial = nullptr;  // 3 * kEfieldMaxBlocks
+    int *d_par01 = nullptr;      // {0, 1}: d_par01 + par is this instance's parity
     FftAxisDev ax[3]{};
     double *d_lam[3]{};
+    FftJob job{};                // Poisson transform of rho into phi
+    FftJob *d_job = nullptr;
+    int64_t fft_grp[3]{};        // workgroups per axis pass
+    int fft_rs[3]{};             // radix set per axis
+    std::vector<WorkItem> items;  // host copy of d_items
+    uint64_t inst_ver = 0;       // bumped whenever h is uploaded
+    uint64_t items_ver = 0;      // bumped whenever the work list changes
+    ohpic_group *group = nullptr;
     unsigned long long *h_flags = nullptr;  // pinned
     double *h_diag = nullptr;               // pinned
     int par = 0;
@@ -188,8 +197,12 @@ int sync_upload_inst(ohpic_ctx *ctx)
 {
     HIPCHK(ctx, hipMemcpyAsync(ctx->d_inst, &ctx->h, sizeof(InstDev), hipMemcpyHostToDevice,
                                ctx->st));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->st));
+    ctx->inst_ver += 1;
     return OHPIC_OK;
 }
+
+const int *parity_of(const ohpic_ctx *ctx) { return ctx->d_par01 + ctx->par; }
 
 void timer_begin(ohpic_ctx *ctx, int k)
 {
@@ -281,14 +294,14 @@ int build_work_items(ohpic_ctx *ctx)
         HIPCHK(ctx, launch_fill_f64(ctx->h.ke_partial, (int64_t)items.size() * kKeSlots, 0.0,
                                     ctx->st));
         ctx->nitems_alloc = (int)items.size();
-        rc = sync_upload_inst(ctx);
-        if (rc) return rc;
     }
     ctx->nitems = (int)items.size();
+    ctx->h.nitems = ctx->nitems;
     HIPCHK(ctx, hipMemcpyAsync(ctx->d_items, items.data(), items.size() * sizeof(WorkItem),
                                hipMemcpyHostToDevice, ctx->st));
-    HIPCHK(ctx, hipStreamSynchronize(ctx->st));
-    return OHPIC_OK;
+    ctx->items = std::move(items);
+    ctx->items_ver += 1;
+    return sync_upload_inst(ctx);
 }
 
 // Assign capacities/bases from per-bin counts and upload them.
@@ -427,16 +440,12 @@ int zero_next_counts(ohpic_ctx *ctx)
 int rerun_as_list(ohpic_ctx *ctx, int list_mode)
 {
     timer_begin(ctx, 3);
-    int rc = zero_next_counts(ctx);
-    if (rc) return rc;
-    if (list_mode == kModeFusedList)
-        HIPCHK(ctx, launch_fill_u64(ctx->h.rho_acc[ctx->par ^ 1], ctx->ng, 0ull, ctx->st));
-    if ((rc = clear_flags(ctx))) return rc;
+    int rc;
+    HIPCHK(ctx, launch_prepare_next_b(ctx->d_inst, parity_of(ctx), 1, ctx->ng, ctx->ntiles,
+                                      ctx->st));
     HIPCHK(ctx, launch_particles(list_mode, ctx->cfg.precision, ctx->cfg.weight_order, ctx->T,
-                                 ctx->d_inst, ctx->d_items, ctx->nitems, ctx->par, ctx->st));
-    if (list_mode == kModeFusedList)
-        HIPCHK(ctx, launch_reduce_partials(ctx->h.ke_partial, ctx->nitems, kKeSlots, kKeSlots,
-                                           ctx->h.diag + kDiagKeOld, 0, ctx->st));
+                                 ctx->d_inst, ctx->d_items, ctx->nitems, parity_of(ctx), ctx->st));
+    if (list_mode == kModeFusedList) HIPCHK(ctx, launch_reduce_ke_b(ctx->d_inst, 1, ctx->st));
     if ((rc = read_flags(ctx))) return rc;
     if (ctx->h_flags[kFlagRange])
         return fail(ctx, OHPIC_ESTATE, "%llu particles moved more than two box lengths",
@@ -463,55 +472,53 @@ int rerun_as_list(ohpic_ctx *ctx, int list_mode)
     return build_work_items(ctx);
 }
 
-// After a kernel that re-bins into buffer par^1: 0 = fine, 1 = a bin
-// overflowed (the output is incomplete), < 0 = error.
-int check_rebin(ohpic_ctx *ctx)
-{
-    int rc = read_flags(ctx);
-    if (rc) return rc;
-    timers_collect(ctx);
-    const unsigned long long *f = ctx->h_flags;
-    if (f[kFlagRange])
-        return fail(ctx, OHPIC_ESTATE,
-                    "%llu particles moved more than two box lengths in one step",
-                    (unsigned long long)f[kFlagRange]);
-    return f[kFlagOverflow] ? 1 : 0;
-}
-
 int do_deposit(ohpic_ctx *ctx)
 {
     timer_begin(ctx, 2);
     HIPCHK(ctx, launch_fill_u64(ctx->h.rho_acc[ctx->par], ctx->ng, 0ull, ctx->st));
     HIPCHK(ctx, launch_particles(kModeDeposit, ctx->cfg.precision, ctx->cfg.weight_order, ctx->T,
-                                 ctx->d_inst, ctx->d_items, ctx->nitems, ctx->par, ctx->st));
+                                 ctx->d_inst, ctx->d_items, ctx->nitems, parity_of(ctx), ctx->st));
     timer_end(ctx, 2);
     ctx->rho_valid = true;
     return OHPIC_OK;
 }
 
+// Poisson solve of the instance table `insts` (n instances, FFT jobs `jobs`):
+// rho from the int64 grid (from_acc), five axis passes, node E + energies.
+hipError_t solve_passes(const InstDev *insts, const int *parity, const FftJob *jobs, int n,
+                        bool from_acc, int64_t max_ng, const int64_t grp[3], const int rs[3],
+                        int max_rows, hipStream_t st)
+{
+    hipError_t e = hipSuccess;
+    if (from_acc) e = launch_rho_from_acc_b(insts, parity, n, max_ng, st);
+    if (e == hipSuccess) e = fft_pass(kPassR2hcXFromSrc, jobs, n, grp[0], rs[0], st);
+    if (e == hipSuccess) e = fft_pass(kPassR2hcY, jobs, n, grp[1], rs[1], st);
+    if (e == hipSuccess) e = fft_pass(kPassPoissonZ, jobs, n, grp[2], rs[2], st);
+    if (e == hipSuccess) e = fft_pass(kPassHc2rY, jobs, n, grp[1], rs[1], st);
+    if (e == hipSuccess) e = fft_pass(kPassHc2rXScaled, jobs, n, grp[0], rs[0], st);
+    if (e == hipSuccess) e = launch_efield_b(insts, n, max_rows, st);
+    return e;
+}
+
 int do_solve(ohpic_ctx *ctx, bool from_acc)
 {
     timer_begin(ctx, 1);
-    const int nx = ctx->n[0], ny = ctx->n[1], nz = ctx->n[2];
-    double *rho = ctx->h.rho, *phi = ctx->h.phi;
-    if (from_acc)
-        HIPCHK(ctx, launch_rho_from_acc(ctx->h.rho_acc[ctx->par], rho, ctx->ng, ctx->h.fix_inv,
-                                        ctx->st));
-    const double *L0 = ctx->d_lam[0], *L1 = ctx->d_lam[1], *L2 = ctx->d_lam[2];
-    HIPCHK(ctx, fft_axis_pass(kFftR2HC, 0, rho, false, 1.0, phi, 1.0, nx, ny, nz, ctx->ax[0], L0,
-                              L1, L2, ctx->st));
-    HIPCHK(ctx, fft_axis_pass(kFftR2HC, 1, phi, false, 1.0, phi, 1.0, nx, ny, nz, ctx->ax[1], L0,
-                              L1, L2, ctx->st));
-    HIPCHK(ctx, fft_axis_pass(kFftPoisson, 2, phi, false, 1.0, phi, 1.0, nx, ny, nz, ctx->ax[2],
-                              L0, L1, L2, ctx->st));
-    HIPCHK(ctx, fft_axis_pass(kFftHC2R, 1, phi, false, 1.0, phi, 1.0, nx, ny, nz, ctx->ax[1], L0,
-                              L1, L2, ctx->st));
-    HIPCHK(ctx, fft_axis_pass(kFftHC2R, 0, phi, false, 1.0, phi, 1.0 / (double)ctx->ng, nx, ny,
-                              nz, ctx->ax[0], L0, L1, L2, ctx->st));
-    HIPCHK(ctx, launch_efield(phi, rho, ctx->h.e[0], ctx->h.e[1], ctx->h.e[2], nx, ny, nz,
-                              ctx->d_epartial, ctx->h.diag, ctx->st));
+    HIPCHK(ctx, solve_passes(ctx->d_inst, parity_of(ctx), ctx->d_job, 1, from_acc, ctx->ng,
+                             ctx->fft_grp, ctx->fft_rs, ctx->n[1] * ctx->n[2], ctx->st));
     timer_end(ctx, 1);
     return OHPIC_OK;
+}
+
+// Flags of a finished re-binning step: counts fast particles, fails on
+// out-of-range movers; 1 when a bin overflowed.
+int absorb_flags(ohpic_ctx *ctx, const unsigned long long *f)
+{
+    if (f[kFlagRange])
+        return fail(ctx, OHPIC_ESTATE,
+                    "%llu particles moved more than two box lengths in one step",
+                    (unsigned long long)f[kFlagRange]);
+    ctx->fast += (int64_t)f[kFlagFast];
+    return f[kFlagOverflow] ? 1 : 0;
 }
 
 int one_step(ohpic_ctx *ctx)
@@ -519,18 +526,17 @@ int one_step(ohpic_ctx *ctx)
     int rc;
     if (!ctx->rho_valid && (rc = do_deposit(ctx))) return rc;
     if ((rc = do_solve(ctx, true))) return rc;
-    if ((rc = zero_next_counts(ctx))) return rc;
-    HIPCHK(ctx, launch_fill_u64(ctx->h.rho_acc[ctx->par ^ 1], ctx->ng, 0ull, ctx->st));
-    if ((rc = clear_flags(ctx))) return rc;
+    HIPCHK(ctx, launch_prepare_next_b(ctx->d_inst, parity_of(ctx), 1, ctx->ng, ctx->ntiles,
+                                      ctx->st));
     timer_begin(ctx, 0);
     HIPCHK(ctx, launch_particles(kModeFused, ctx->cfg.precision, ctx->cfg.weight_order, ctx->T,
-                                 ctx->d_inst, ctx->d_items, ctx->nitems, ctx->par, ctx->st));
+                                 ctx->d_inst, ctx->d_items, ctx->nitems, parity_of(ctx), ctx->st));
     timer_end(ctx, 0);
-    HIPCHK(ctx, launch_reduce_partials(ctx->h.ke_partial, ctx->nitems, kKeSlots, kKeSlots,
-                                       ctx->h.diag + kDiagKeOld, 0, ctx->st));
-    rc = check_rebin(ctx);
+    HIPCHK(ctx, launch_reduce_ke_b(ctx->d_inst, 1, ctx->st));
+    if ((rc = read_flags(ctx))) return rc;
+    timers_collect(ctx);
+    rc = absorb_flags(ctx, ctx->h_flags);
     if (rc < 0) return rc;
-    ctx->fast += (int64_t)ctx->h_flags[kFlagFast];
     if (rc == 1) {
         // a bin overflowed: redo the step into a list and bin it into buffer par
         if ((rc = rerun_as_list(ctx, kModeFusedList))) return rc;
@@ -567,9 +573,12 @@ int make_axis(ohpic_ctx *ctx, int n, FftAxisDev &ax)
     return OHPIC_OK;
 }
 
+int group_detach(ohpic_group *g, ohpic_ctx *ctx);
+
 void release(ohpic_ctx *ctx)
 {
     if (!ctx) return;
+    if (ctx->group) (void)group_detach(ctx->group, ctx);
     (void)hipSetDevice(ctx->device);
     if (ctx->st) (void)hipStreamSynchronize(ctx->st);
     for (void *p : ctx->allocs) (void)hipFree(p);
@@ -671,6 +680,292 @@ void fft_twiddles(int n, double2 *tw)
 }
 
 }  // namespace ohpic
+
+// ================================================================== groups
+struct ohpic_group {
+    int device = 0;
+    hipStream_t st = nullptr;
+    std::vector<ohpic_ctx *> members;
+    std::vector<uint64_t> seen_inst, seen_items;  // member versions in the tables
+    std::vector<int> dev_par;                     // parity the device table holds
+    InstDev *d_insts = nullptr;
+    FftJob *d_jobs = nullptr;
+    int *d_parity = nullptr;
+    unsigned long long *d_flags = nullptr;
+    unsigned long long *h_flags = nullptr;  // pinned
+    int cap_insts = 0;
+    WorkItem *d_items = nullptr;
+    int64_t cap_items = 0;
+    int nitems = 0;
+    bool dirty = true;
+    int64_t max_ng = 0;
+    int max_ntiles = 0, max_rows = 0;
+    int64_t grp[3]{};
+    int rs[3]{};
+    int T[3]{};
+    int precision = -1, weight_order = -1;
+    bool use_graph = true;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    int64_t steps = 0, captures = 0;
+    std::string err;
+};
+
+namespace {
+
+int gfail(ohpic_group *g, int code, const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    if (g) g->err = buf;
+    std::lock_guard<std::mutex> lk(g_err_mu);
+    g_err = buf;
+    return code;
+}
+
+#define GHIP(g, expr)                                                                       \
+    do {                                                                                    \
+        hipError_t e_ = (expr);                                                             \
+        if (e_ != hipSuccess)                                                               \
+            return gfail(g, OHPIC_EHIP, "%s failed: %s (%s:%d)", #expr,                    \
+                         hipGetErrorString(e_), __FILE__, __LINE__);                        \
+    } while (0)
+
+void drop_graph(ohpic_group *g)
+{
+    if (g->exec) (void)hipGraphExecDestroy(g->exec);
+    if (g->graph) (void)hipGraphDestroy(g->graph);
+    g->exec = nullptr;
+    g->graph = nullptr;
+}
+
+template <typename T>
+void gfree(T *&p)
+{
+    if (p) (void)hipFree((void *)p);
+    p = nullptr;
+}
+
+int group_reserve(ohpic_group *g, int ninst, int64_t nitems)
+{
+    if (ninst > g->cap_insts) {
+        const int cap = std::max(ninst, std::max(16, 2 * g->cap_insts));
+        GHIP(g, hipStreamSynchronize(g->st));
+        drop_graph(g);
+        gfree(g->d_insts);
+        gfree(g->d_jobs);
+        gfree(g->d_parity);
+        gfree(g->d_flags);
+        if (g->h_flags) (void)hipHostFree(g->h_flags);
+        g->h_flags = nullptr;
+        g->cap_insts = 0;
+        GHIP(g, hipMalloc((void **)&g->d_insts, cap * sizeof(InstDev)));
+        GHIP(g, hipMalloc((void **)&g->d_jobs, cap * sizeof(FftJob)));
+        GHIP(g, hipMalloc((void **)&g->d_parity, cap * sizeof(int)));
+        GHIP(g, hipMalloc((void **)&g->d_flags, (size_t)cap * kFlagSlots * 8));
+        GHIP(g, hipHostMalloc((void **)&g->h_flags, (size_t)cap * kFlagSlots * 8,
+                              hipHostMallocDefault));
+        g->cap_insts = cap;
+    }
+    if (nitems > g->cap_items) {
+        const int64_t cap = std::max<int64_t>(nitems, std::max<int64_t>(1024, 2 * g->cap_items));
+        GHIP(g, hipStreamSynchronize(g->st));
+        drop_graph(g);
+        gfree(g->d_items);
+        g->cap_items = 0;
+        GHIP(g, hipMalloc((void **)&g->d_items, cap * sizeof(WorkItem)));
+        g->cap_items = cap;
+    }
+    return OHPIC_OK;
+}
+
+// Rebuild the device tables when membership or a member's record / work list
+// changed, and upload the parity table when it differs from the members'.
+int group_refresh(ohpic_group *g)
+{
+    const int n = (int)g->members.size();
+    bool tables = g->dirty;
+    for (int i = 0; i < n && !tables; ++i)
+        tables = g->members[i]->inst_ver != g->seen_inst[i] ||
+                 g->members[i]->items_ver != g->seen_items[i];
+    if (tables) {
+        int64_t total = 0;
+        for (ohpic_ctx *m : g->members) total += m->nitems;
+        if (total > INT32_MAX) return gfail(g, OHPIC_EINVAL, "more than 2^31 work items");
+        int rc = group_reserve(g, n, total);
+        if (rc) return rc;
+        std::vector<InstDev> insts(n);
+        std::vector<FftJob> jobs(n);
+        std::vector<WorkItem> items;
+        items.reserve((size_t)total);
+        g->max_ng = 0;
+        g->max_ntiles = g->max_rows = 0;
+        for (int a = 0; a < 3; ++a) {
+            g->grp[a] = 0;
+            g->rs[a] = 0;
+        }
+        for (int i = 0; i < n; ++i) {
+            ohpic_ctx *m = g->members[i];
+            insts[i] = m->h;
+            jobs[i] = m->job;
+            for (WorkItem w : m->items) {
+                w.inst = i;
+                items.push_back(w);
+            }
+            g->max_ng = std::max(g->max_ng, m->ng);
+            g->max_ntiles = std::max(g->max_ntiles, m->ntiles);
+            g->max_rows = std::max(g->max_rows, m->n[1] * m->n[2]);
+            for (int a = 0; a < 3; ++a) {
+                g->grp[a] = std::max(g->grp[a], m->fft_grp[a]);
+                g->rs[a] = std::max(g->rs[a], m->fft_rs[a]);
+            }
+            g->seen_inst[i] = m->inst_ver;
+            g->seen_items[i] = m->items_ver;
+        }
+        // heavy items first over the whole table: the dispatcher then fills
+        // the machine with the largest chunks of every instance
+        std::stable_sort(items.begin(), items.end(), [&](const WorkItem &a, const WorkItem &b) {
+            auto load = [&](const WorkItem &w) {
+                const ohpic_ctx *m = g->members[w.inst];
+                int64_t l = 0;
+                for (int s = 0; s < m->cfg.nspecies; ++s)
+                    if (!m->sp[s].cap.empty()) l += m->sp[s].cap[w.tile];
+                return l / w.nchunks;
+            };
+            return load(a) > load(b);
+        });
+        g->nitems = (int)items.size();
+        if (n > 0) {
+            GHIP(g, hipMemcpyAsync(g->d_insts, insts.data(), n * sizeof(InstDev),
+                                   hipMemcpyHostToDevice, g->st));
+            GHIP(g, hipMemcpyAsync(g->d_jobs, jobs.data(), n * sizeof(FftJob),
+                                   hipMemcpyHostToDevice, g->st));
+        }
+        if (!items.empty())
+            GHIP(g, hipMemcpyAsync(g->d_items, items.data(), items.size() * sizeof(WorkItem),
+                                   hipMemcpyHostToDevice, g->st));
+        GHIP(g, hipStreamSynchronize(g->st));
+        drop_graph(g);
+        g->dirty = false;
+    }
+    bool par = false;
+    for (int i = 0; i < n; ++i) par = par || g->dev_par[i] != g->members[i]->par;
+    if (par) {
+        std::vector<int> p(n);
+        for (int i = 0; i < n; ++i) p[i] = g->members[i]->par;
+        GHIP(g, hipMemcpyAsync(g->d_parity, p.data(), n * sizeof(int), hipMemcpyHostToDevice,
+                               g->st));
+        GHIP(g, hipStreamSynchronize(g->st));
+        g->dev_par = p;
+    }
+    return OHPIC_OK;
+}
+
+// One step of every member: solve, fused particle kernel, KE sums, flags.
+hipError_t group_enqueue(ohpic_group *g)
+{
+    const int n = (int)g->members.size();
+    hipError_t e = solve_passes(g->d_insts, g->d_parity, g->d_jobs, n, true, g->max_ng, g->grp,
+                                g->rs, g->max_rows, g->st);
+    if (e == hipSuccess)
+        e = launch_prepare_next_b(g->d_insts, g->d_parity, n, g->max_ng, g->max_ntiles, g->st);
+    if (e == hipSuccess)
+        e = launch_particles(kModeFused, g->precision, g->weight_order, g->T, g->d_insts,
+                             g->d_items, g->nitems, g->d_parity, g->st);
+    if (e == hipSuccess) e = launch_reduce_ke_b(g->d_insts, n, g->st);
+    if (e == hipSuccess) e = launch_finish_b(g->d_insts, g->d_parity, n, g->d_flags, g->st);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(g->h_flags, g->d_flags, (size_t)n * kFlagSlots * 8,
+                           hipMemcpyDeviceToHost, g->st);
+    return e;
+}
+
+int group_capture(ohpic_group *g)
+{
+    drop_graph(g);
+    GHIP(g, hipStreamBeginCapture(g->st, hipStreamCaptureModeThreadLocal));
+    const hipError_t e = group_enqueue(g);
+    hipGraph_t graph = nullptr;
+    const hipError_t e2 = hipStreamEndCapture(g->st, &graph);
+    if (e != hipSuccess || e2 != hipSuccess) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return gfail(g, OHPIC_EHIP, "graph capture failed: %s",
+                     hipGetErrorString(e != hipSuccess ? e : e2));
+    }
+    g->graph = graph;
+    GHIP(g, hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0));
+    g->captures += 1;
+    return OHPIC_OK;
+}
+
+int group_step(ohpic_group *g, int nsteps)
+{
+    GHIP(g, hipSetDevice(g->device));
+    const int n = (int)g->members.size();
+    for (int it = 0; it < nsteps && n > 0; ++it) {
+        for (int i = 0; i < n; ++i) {
+            ohpic_ctx *m = g->members[i];
+            if (!m->binned)
+                return gfail(g, OHPIC_ESTATE, "member %d: particles pushed but not re-binned", i);
+            if (!m->rho_valid) {
+                int rc = do_deposit(m);
+                if (rc == OHPIC_OK && hipStreamSynchronize(m->st) != hipSuccess)
+                    rc = fail(m, OHPIC_EHIP, "deposit failed");
+                if (rc) return gfail(g, rc, "member %d: %s", i, m->err.c_str());
+            }
+        }
+        int rc = group_refresh(g);
+        if (rc) return rc;
+        if (g->use_graph) {
+            if (!g->exec && (rc = group_capture(g))) return rc;
+            GHIP(g, hipGraphLaunch(g->exec, g->st));
+        } else {
+            GHIP(g, group_enqueue(g));
+        }
+        GHIP(g, hipStreamSynchronize(g->st));
+        for (int i = 0; i < n; ++i) {
+            ohpic_ctx *m = g->members[i];
+            rc = absorb_flags(m, g->h_flags + (size_t)i * kFlagSlots);
+            if (rc < 0) return gfail(g, rc, "member %d: %s", i, m->err.c_str());
+            if (rc == 1) {
+                // the device kept this member's parity: redo its step alone
+                if ((rc = rerun_as_list(m, kModeFusedList)))
+                    return gfail(g, rc, "member %d: %s", i, m->err.c_str());
+            } else {
+                m->par ^= 1;
+                g->dev_par[i] ^= 1;
+            }
+            m->step += 1;
+            m->rho_valid = true;
+        }
+        g->steps += 1;
+    }
+    return OHPIC_OK;
+}
+
+int group_detach(ohpic_group *g, ohpic_ctx *ctx)
+{
+    auto it = std::find(g->members.begin(), g->members.end(), ctx);
+    if (it == g->members.end()) return gfail(g, OHPIC_EINVAL, "context is not a member");
+    (void)hipSetDevice(g->device);
+    GHIP(g, hipStreamSynchronize(g->st));
+    const size_t i = (size_t)(it - g->members.begin());
+    g->members.erase(it);
+    g->seen_inst.erase(g->seen_inst.begin() + i);
+    g->seen_items.erase(g->seen_items.begin() + i);
+    g->dev_par.erase(g->dev_par.begin() + i);
+    ctx->group = nullptr;
+    g->dirty = true;
+    // the parity table is indexed by position: upload it again
+    for (int &p : g->dev_par) p = -1;
+    drop_graph(g);
+    return OHPIC_OK;
+}
+
+}  // namespace
 
 // ================================================================== C ABI
 extern "C" {
@@ -849,8 +1144,13 @@ int ohpic_create(const ohpic_config *cfg, ohpic_ctx **out)
     CREATE_CHK(dalloc(ctx, &ctx->d_tmp32a, nt));
     CREATE_CHK(dalloc(ctx, &ctx->d_tmp32b, nt));
     CREATE_CHK(dalloc(ctx, &ctx->d_cursor, 1));
-    CREATE_CHK(dalloc(ctx, &ctx->d_epartial, 3 * kEfieldMaxBlocks));
+    CREATE_CHK(dalloc(ctx, &h.e_partial, 3 * kEfieldMaxBlocks));
     CREATE_CHK(dalloc(ctx, &ctx->d_inst, 1));
+    CREATE_CHK(dalloc(ctx, &ctx->d_par01, 2));
+    {
+        const int p01[2] = {0, 1};
+        CREATE_HIP(hipMemcpy(ctx->d_par01, p01, sizeof p01, hipMemcpyHostToDevice));
+    }
     // FFT plans and the Green's-function denominators per axis
     for (int a = 0; a < 3; ++a) {
         CREATE_CHK(make_axis(ctx, ctx->n[a], ctx->ax[a]));
@@ -865,6 +1165,21 @@ int ohpic_create(const ohpic_config *cfg, ohpic_ctx **out)
         }
         CREATE_CHK(dalloc(ctx, &ctx->d_lam[a], n));
         CREATE_HIP(hipMemcpy(ctx->d_lam[a], lam.data(), n * sizeof(double), hipMemcpyHostToDevice));
+    }
+    {
+        FftJob &J = ctx->job;
+        J.src = h.rho;
+        J.data = h.phi;
+        for (int a = 0; a < 3; ++a) {
+            J.n[a] = ctx->n[a];
+            J.ax[a] = ctx->ax[a];
+            J.lam[a] = ctx->d_lam[a];
+            ctx->fft_grp[a] = fft_groups(a, fft_group_lines(ctx->n[a]), c.nx, c.ny, c.nz);
+            ctx->fft_rs[a] = fft_radix_set(ctx->ax[a]);
+        }
+        J.inv_n = 1.0 / (double)ctx->ng;
+        CREATE_CHK(dalloc(ctx, &ctx->d_job, 1));
+        CREATE_HIP(hipMemcpy(ctx->d_job, &J, sizeof J, hipMemcpyHostToDevice));
     }
     CREATE_HIP(hipHostMalloc((void **)&ctx->h_flags, kFlagSlots * sizeof(unsigned long long),
                              hipHostMallocDefault));
@@ -1170,8 +1485,7 @@ int ohpic_deposit(ohpic_ctx *ctx)
     HIPCHK(ctx, hipSetDevice(ctx->device));
     int rc = do_deposit(ctx);
     if (rc) return rc;
-    HIPCHK(ctx, launch_rho_from_acc(ctx->h.rho_acc[ctx->par], ctx->h.rho, ctx->ng, ctx->h.fix_inv,
-                                    ctx->st));
+    HIPCHK(ctx, launch_rho_from_acc_b(ctx->d_inst, parity_of(ctx), 1, ctx->ng, ctx->st));
     HIPCHK(ctx, hipStreamSynchronize(ctx->st));
     timers_collect(ctx);
     return OHPIC_OK;
@@ -1195,10 +1509,9 @@ int ohpic_gather_push(ohpic_ctx *ctx)
         return fail(ctx, OHPIC_ESTATE, "gather_push needs binned particles: call boundary");
     HIPCHK(ctx, hipSetDevice(ctx->device));
     HIPCHK(ctx, launch_particles(kModeGatherPush, ctx->cfg.precision, ctx->cfg.weight_order,
-                                 ctx->T, ctx->d_inst, ctx->d_items, ctx->nitems, ctx->par,
+                                 ctx->T, ctx->d_inst, ctx->d_items, ctx->nitems, parity_of(ctx),
                                  ctx->st));
-    HIPCHK(ctx, launch_reduce_partials(ctx->h.ke_partial, ctx->nitems, kKeSlots, kKeSlots,
-                                       ctx->h.diag + kDiagKeOld, 0, ctx->st));
+    HIPCHK(ctx, launch_reduce_ke_b(ctx->d_inst, 1, ctx->st));
     HIPCHK(ctx, hipStreamSynchronize(ctx->st));
     ctx->binned = false;
     ctx->rho_valid = false;
@@ -1213,8 +1526,10 @@ int ohpic_boundary(ohpic_ctx *ctx)
     if (rc) return rc;
     if ((rc = clear_flags(ctx))) return rc;
     HIPCHK(ctx, launch_particles(kModeBoundary, ctx->cfg.precision, ctx->cfg.weight_order, ctx->T,
-                                 ctx->d_inst, ctx->d_items, ctx->nitems, ctx->par, ctx->st));
-    rc = check_rebin(ctx);
+                                 ctx->d_inst, ctx->d_items, ctx->nitems, parity_of(ctx), ctx->st));
+    if ((rc = read_flags(ctx))) return rc;
+    timers_collect(ctx);
+    rc = absorb_flags(ctx, ctx->h_flags);
     if (rc < 0) return rc;
     if (rc == 1) {
         // a bin overflowed: redo the wrap into a list and bin it into buffer par
@@ -1364,16 +1679,34 @@ int ohpic_fft3d_r2r(int device, int nx, int ny, int nz, int backward, const doub
         cleanup();
         return fail(nullptr, rc, "%s", e.c_str());
     }
-    hipError_t e = hipMemcpyAsync(buf, in, ng * sizeof(double), hipMemcpyHostToDevice, st);
-    const double *nl = nullptr;
+    FftJob job{};
+    job.src = buf;
+    job.data = buf;
+    int64_t grp[3];
+    int rs[3];
+    for (int a = 0; a < 3; ++a) {
+        job.n[a] = n[a];
+        job.ax[a] = ax[a];
+        grp[a] = fft_groups(a, fft_group_lines(n[a]), nx, ny, nz);
+        rs[a] = fft_radix_set(ax[a]);
+    }
+    job.inv_n = 1.0 / (double)ng;
+    FftJob *d_job = nullptr;
+    rc = dalloc(&tmp, &d_job, 1);
+    if (rc) {
+        std::string es = tmp.err;
+        cleanup();
+        return fail(nullptr, rc, "%s", es.c_str());
+    }
+    hipError_t e = hipMemcpyAsync(d_job, &job, sizeof job, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(buf, in, ng * sizeof(double), hipMemcpyHostToDevice, st);
     if (!backward) {
-        for (int a = 0; a < 3 && e == hipSuccess; ++a)
-            e = fft_axis_pass(kFftR2HC, a, buf, false, 1.0, buf, 1.0, nx, ny, nz, ax[a], nl, nl,
-                              nl, st);
+        const FftPass p[3] = {kPassR2hcX, kPassR2hcY, kPassR2hcZ};
+        for (int a = 0; a < 3 && e == hipSuccess; ++a) e = fft_pass(p[a], d_job, 1, grp[a], rs[a], st);
     } else {
-        for (int a = 2; a >= 0 && e == hipSuccess; --a)
-            e = fft_axis_pass(kFftHC2R, a, buf, false, 1.0, buf, a == 0 ? 1.0 / (double)ng : 1.0,
-                              nx, ny, nz, ax[a], nl, nl, nl, st);
+        const FftPass p[3] = {kPassHc2rXScaled, kPassHc2rY, kPassHc2rZ};
+        for (int a = 2; a >= 0 && e == hipSuccess; --a) e = fft_pass(p[a], d_job, 1, grp[a], rs[a], st);
     }
     if (e == hipSuccess)
         e = hipMemcpyAsync(out, buf, ng * sizeof(double), hipMemcpyDeviceToHost, st);
@@ -1468,6 +1801,198 @@ int ohpic_memory_footprint(ohpic_ctx *ctx, int64_t *bytes)
 {
     if (!ctx || !bytes) return fail(ctx, OHPIC_EINVAL, "NULL argument");
     *bytes = ctx->bytes;
+    return OHPIC_OK;
+}
+
+// ------------------------------------------------------------------ groups
+int ohpic_group_create(int device, ohpic_group **out)
+{
+    if (!out) return fail(nullptr, OHPIC_EINVAL, "NULL argument");
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
+        (void)hipGetLastError();
+        return fail(nullptr, OHPIC_ENODEV, "no HIP device available");
+    }
+    if (device < 0 || device >= ndev)
+        return fail(nullptr, OHPIC_EINVAL, "device %d out of range (%d devices)", device, ndev);
+    if (hipSetDevice(device) != hipSuccess) return fail(nullptr, OHPIC_EHIP, "hipSetDevice failed");
+    ohpic_group *g = new ohpic_group();
+    g->device = device;
+    if (hipStreamCreateWithFlags(&g->st, hipStreamNonBlocking) != hipSuccess) {
+        delete g;
+        return fail(nullptr, OHPIC_EHIP, "stream creation failed");
+    }
+    *out = g;
+    return OHPIC_OK;
+}
+
+int ohpic_group_destroy(ohpic_group *g)
+{
+    if (!g) return OHPIC_OK;
+    (void)hipSetDevice(g->device);
+    while (!g->members.empty()) (void)group_detach(g, g->members.back());
+    (void)hipStreamSynchronize(g->st);
+    drop_graph(g);
+    gfree(g->d_insts);
+    gfree(g->d_jobs);
+    gfree(g->d_parity);
+    gfree(g->d_flags);
+    gfree(g->d_items);
+    if (g->h_flags) (void)hipHostFree(g->h_flags);
+    (void)hipStreamDestroy(g->st);
+    delete g;
+    return OHPIC_OK;
+}
+
+int ohpic_group_attach(ohpic_group *g, ohpic_ctx *ctx)
+{
+    if (!g || !ctx) return fail(nullptr, OHPIC_EINVAL, "NULL argument");
+    if (ctx->group) return gfail(g, OHPIC_ESTATE, "context already belongs to a group");
+    if (ctx->device != g->device)
+        return gfail(g, OHPIC_EINVAL, "context on device %d, group on device %d", ctx->device,
+                     g->device);
+    if (!ctx->binned) return gfail(g, OHPIC_ESTATE, "particles pushed but not re-binned");
+    if (!g->members.empty()) {
+        if (ctx->cfg.precision != g->precision || ctx->cfg.weight_order != g->weight_order ||
+            ctx->T[0] != g->T[0] || ctx->T[1] != g->T[1] || ctx->T[2] != g->T[2])
+            return gfail(g, OHPIC_EINVAL,
+                         "member differs from the group in precision, weight order or tile "
+                         "(group %d/%d/%dx%dx%d, context %d/%d/%dx%dx%d)",
+                         g->precision, g->weight_order, g->T[0], g->T[1], g->T[2],
+                         ctx->cfg.precision, ctx->cfg.weight_order, ctx->T[0], ctx->T[1],
+                         ctx->T[2]);
+    } else {
+        g->precision = ctx->cfg.precision;
+        g->weight_order = ctx->cfg.weight_order;
+        for (int a = 0; a < 3; ++a) g->T[a] = ctx->T[a];
+    }
+    GHIP(g, hipSetDevice(g->device));
+    GHIP(g, hipStreamSynchronize(ctx->st));
+    g->members.push_back(ctx);
+    g->seen_inst.push_back(0);
+    g->seen_items.push_back(0);
+    g->dev_par.push_back(-1);
+    ctx->group = g;
+    g->dirty = true;
+    return OHPIC_OK;
+}
+
+int ohpic_group_detach(ohpic_group *g, ohpic_ctx *ctx)
+{
+    if (!g || !ctx) return fail(nullptr, OHPIC_EINVAL, "NULL argument");
+    return group_detach(g, ctx);
+}
+
+int ohpic_group_size(const ohpic_group *g, int *n)
+{
+    if (!g || !n) return fail(nullptr, OHPIC_EINVAL, "NULL argument");
+    *n = (int)g->members.size();
+    return OHPIC_OK;
+}
+
+int ohpic_group_step(ohpic_group *g, int nsteps)
+{
+    if (!g) return fail(nullptr, OHPIC_EINVAL, "group is NULL");
+    if (nsteps < 0) return gfail(g, OHPIC_EINVAL, "nsteps < 0");
+    return group_step(g, nsteps);
+}
+
+int ohpic_group_use_graph(ohpic_group *g, int enable)
+{
+    if (!g) return fail(nullptr, OHPIC_EINVAL, "group is NULL");
+    g->use_graph = enable != 0;
+    (void)hipSetDevice(g->device);
+    (void)hipStreamSynchronize(g->st);
+    drop_graph(g);
+    return OHPIC_OK;
+}
+
+int ohpic_group_stats(const ohpic_group *g, int64_t *steps, int64_t *captures, int64_t *items)
+{
+    if (!g) return fail(nullptr, OHPIC_EINVAL, "group is NULL");
+    if (steps) *steps = g->steps;
+    if (captures) *captures = g->captures;
+    if (items) *items = g->nitems;
+    return OHPIC_OK;
+}
+
+const char *ohpic_group_last_error(const ohpic_group *g)
+{
+    if (g) return g->err.c_str();
+    return ohpic_last_error(nullptr);
+}
+
+// ---------------------------------------------------------------- autotuner
+int ohpic_autotune(const ohpic_config *cfgs, int n, int sample, int steps, ohpic_tune_result *out)
+{
+    if (!cfgs || !out || n <= 0) return fail(nullptr, OHPIC_EINVAL, "bad arguments");
+    if (steps < 1) steps = 3;
+    if (sample < 1 || sample > n) sample = n;
+    std::vector<int> pick(sample);
+    for (int k = 0; k < sample; ++k) pick[k] = (int)((int64_t)k * n / sample);
+    const int tiles[3][3] = {{4, 4, 4}, {8, 8, 8}, {8, 8, 4}};
+    const int chunks[3] = {1, 2, 4};
+    std::memset(out, 0, sizeof *out);
+    double best = -1.0;
+    int timed = 0;
+    for (const auto &t : tiles) {
+        for (int ch : chunks) {
+            ohpic_group *g = nullptr;
+            int rc = ohpic_group_create(cfgs[0].device, &g);
+            if (rc) return rc;
+            std::vector<ohpic_ctx *> ctxs;
+            double np_total = 0.0;
+            auto cleanup = [&]() {
+                ohpic_group_destroy(g);
+                for (ohpic_ctx *c : ctxs) ohpic_destroy(c);
+            };
+            for (int k = 0; k < sample && rc == OHPIC_OK; ++k) {
+                ohpic_config c = cfgs[pick[k]];
+                for (int a = 0; a < 3; ++a) c.tile[a] = t[a];
+                c.chunks = ch;
+                ohpic_ctx *ctx = nullptr;
+                if ((rc = ohpic_create(&c, &ctx))) break;
+                ctxs.push_back(ctx);
+                int64_t np[OHPIC_MAX_SPECIES];
+                double vth[OHPIC_MAX_SPECIES];
+                for (int s = 0; s < c.nspecies; ++s) {
+                    np[s] = c.capacity[s];
+                    vth[s] = s == 0 ? 1.0 : 0.1;
+                    np_total += (double)np[s];
+                }
+                if ((rc = ohpic_seed_uniform_maxwell(ctx, 42u + (uint64_t)k, 52u, np, vth)))
+                    break;
+                rc = ohpic_group_attach(g, ctx);
+            }
+            if (rc == OHPIC_OK) rc = ohpic_group_step(g, 2);
+            double rate = 0.0;
+            if (rc == OHPIC_OK) {
+                const auto t0 = std::chrono::steady_clock::now();
+                rc = ohpic_group_step(g, steps);
+                const double sec =
+                    std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+                rate = np_total * steps / std::max(sec, 1e-9);
+            }
+            if (rc != OHPIC_OK) {
+                std::string e = ohpic_last_error(nullptr);
+                cleanup();
+                if (rc == OHPIC_ENOMEM) continue;  // candidate does not fit: skip it
+                return fail(nullptr, rc, "autotune %dx%dx%d/%d: %s", t[0], t[1], t[2], ch,
+                            e.c_str());
+            }
+            cleanup();
+            timed += 1;
+            if (rate > best) {
+                best = rate;
+                for (int a = 0; a < 3; ++a) out->tile[a] = t[a];
+                out->chunks = ch;
+                out->particle_steps_per_s = rate;
+            }
+        }
+    }
+    out->candidates = timed;
+    if (timed == 0) return fail(nullptr, OHPIC_ENOMEM, "autotune: no candidate fits");
     return OHPIC_OK;
 }
 

@@ -213,9 +213,9 @@ __device__ __forceinline__ void elem_of(int e, int n, int G, int &g, int &j)
 }
 
 // Load G real lines into LDS packed as complex (line pair -> re/im).
-template <int AXIS, typename SrcT>
-__device__ void load_lines(const SrcT *__restrict__ src, double src_scale, cplx *buf, int grp,
-                           int G, int n, int64_t estride, int nx, int ny, int nz)
+template <int AXIS>
+__device__ void load_lines(const double *__restrict__ src, cplx *buf, int grp, int G, int n,
+                           int64_t estride, int nx, int ny, int nz)
 {
     double *bd = reinterpret_cast<double *>(buf);
     const int total = G * n;
@@ -224,13 +224,7 @@ __device__ void load_lines(const SrcT *__restrict__ src, double src_scale, cplx 
         elem_of<AXIS>(e, n, G, g, j);
         LineGeom lg;
         double v = 0.0;
-        if (line_of<AXIS>(grp, g, G, nx, ny, nz, lg)) {
-            if constexpr (std::is_same<SrcT, unsigned long long>::value) {
-                v = (double)(long long)src[lg.base + (int64_t)j * estride] * src_scale;
-            } else {
-                v = (double)src[lg.base + (int64_t)j * estride];
-            }
-        }
+        if (line_of<AXIS>(grp, g, G, nx, ny, nz, lg)) v = src[lg.base + (int64_t)j * estride];
         bd[((g >> 1) * n + j) * 2 + (g & 1)] = v;
     }
 }
@@ -299,20 +293,31 @@ __device__ void build_packed_inplace(cplx *buf, int n, int nlines)
     __syncthreads();
 }
 
-template <int KIND, int AXIS, int RS, typename SrcT>
-__global__ void __launch_bounds__(kFftBlock)
-    k_r2r_axis(const SrcT *__restrict__ src, double *__restrict__ dst, double src_scale,
-               double out_scale, int nx, int ny, int nz, FftAxisDev ax, int G,
-               const double *__restrict__ lamx, const double *__restrict__ lamy,
-               const double *__restrict__ lamz)
+// One axis pass for every job of the table (blockIdx.y = job, blockIdx.x =
+// line group; groups beyond a job's count exit).  FROM_SRC reads job.src
+// (rho) instead of transforming job.data in place; SCALE multiplies the
+// output by job.inv_n (the final HC2R pass).
+template <int KIND, int AXIS, int RS, bool FROM_SRC, bool SCALE>
+__global__ void __launch_bounds__(kFftBlock) k_fft_pass(const FftJob *__restrict__ jobs)
 {
     __shared__ cplx buf[kFftLdsCplx];
+    const FftJob &J = jobs[blockIdx.y];
+    const int nx = J.n[0], ny = J.n[1], nz = J.n[2];
+    const FftAxisDev &ax = J.ax[AXIS];
     const int n = ax.n;
-    const int nlines = (G + 1) / 2;
+    const int G = fft_group_lines(n);
     const int grp = blockIdx.x;
+    if ((int64_t)grp >= fft_groups(AXIS, G, nx, ny, nz)) return;
+    const double *__restrict__ src = FROM_SRC ? J.src : J.data;
+    double *__restrict__ dst = J.data;
+    const double out_scale = SCALE ? J.inv_n : 1.0;
+    const double *__restrict__ lamx = J.lam[0];
+    const double *__restrict__ lamy = J.lam[1];
+    const double *__restrict__ lamz = J.lam[2];
+    const int nlines = (G + 1) / 2;
     const int64_t estride = AXIS == 0 ? 1 : (AXIS == 1 ? (int64_t)nx : (int64_t)nx * ny);
 
-    load_lines<AXIS, SrcT>(src, src_scale, buf, grp, G, n, estride, nx, ny, nz);
+    load_lines<AXIS>(src, buf, grp, G, n, estride, nx, ny, nz);
     __syncthreads();
 
     if (KIND == kFftR2HC || KIND == kFftPoisson) {
@@ -356,7 +361,6 @@ __global__ void __launch_bounds__(kFftBlock)
         fft_lds<+1, RS>(buf, n, nlines, ax);
     }
 
-    // write back
     const int total = G * n;
     for (int e = threadIdx.x; e < total; e += blockDim.x) {
         int g, j;
@@ -376,89 +380,45 @@ __global__ void __launch_bounds__(kFftBlock)
 
 }  // namespace
 
-int fft_group_lines(int n)
-{
-    // real lines per workgroup: G/2 complex lines of n fill at most
-    // kFftLdsCplx complex values (32 KiB); at least 2 lines, at most 64
-    int g = 2 * (kFftLdsCplx / (n > 0 ? n : 1));
-    if (g > 64) g = 64;
-    if (g < 2) g = 2;
-    return g & ~1;
-}
-
-size_t fft_lds_bytes(int n, int G)
-{
-    return (size_t)((G + 1) / 2) * n * sizeof(cplx);
-}
-
-int64_t fft_groups(int axis, int G, int nx, int ny, int nz)
-{
-    if (axis == 0) return ((int64_t)ny * nz + G - 1) / G;
-    if (axis == 1) return (int64_t)((nx + G - 1) / G) * nz;
-    return ((int64_t)nx * ny + G - 1) / G;
-}
-
 int fft_radix_set(const FftAxisDev &ax)
 {
     int rs = 0;
     for (int i = 0; i < ax.nfac; ++i) {
         if (ax.fac[i] == 3 && rs < 1) rs = 1;
-        if (ax.fac[i] > 4) rs = 2;
+        if (ax.fac[i] > 4 && ax.fac[i] != 3) rs = 2;
     }
     return rs;
 }
 
-template <int KIND, int AXIS, typename SrcT>
-static hipError_t launch_one(const SrcT *src, double *dst, double src_scale, double out_scale,
-                             int nx, int ny, int nz, const FftAxisDev &ax, const double *lx,
-                             const double *ly, const double *lz, hipStream_t st)
+template <int KIND, int AXIS, bool FROM_SRC, bool SCALE>
+static hipError_t launch_pass(const FftJob *jobs, int njobs, int64_t max_groups, int rs,
+                              hipStream_t st)
 {
-    const int G = fft_group_lines(ax.n);
-    const int64_t groups = fft_groups(AXIS, G, nx, ny, nz);
-    if (groups <= 0) return hipSuccess;
-    const dim3 g((unsigned)groups), b(kFftBlock);
-    switch (fft_radix_set(ax)) {
-    case 0:
-        hipLaunchKernelGGL((k_r2r_axis<KIND, AXIS, 0, SrcT>), g, b, 0, st, src, dst, src_scale,
-                           out_scale, nx, ny, nz, ax, G, lx, ly, lz);
-        break;
-    case 1:
-        hipLaunchKernelGGL((k_r2r_axis<KIND, AXIS, 1, SrcT>), g, b, 0, st, src, dst, src_scale,
-                           out_scale, nx, ny, nz, ax, G, lx, ly, lz);
-        break;
-    default:
-        hipLaunchKernelGGL((k_r2r_axis<KIND, AXIS, 2, SrcT>), g, b, 0, st, src, dst, src_scale,
-                           out_scale, nx, ny, nz, ax, G, lx, ly, lz);
-        break;
-    }
+    if (njobs <= 0 || max_groups <= 0) return hipSuccess;
+    const dim3 g((unsigned)max_groups, (unsigned)njobs), b(kFftBlock);
+    if (rs == 0)
+        hipLaunchKernelGGL((k_fft_pass<KIND, AXIS, 0, FROM_SRC, SCALE>), g, b, 0, st, jobs);
+    else if (rs == 1)
+        hipLaunchKernelGGL((k_fft_pass<KIND, AXIS, 1, FROM_SRC, SCALE>), g, b, 0, st, jobs);
+    else
+        hipLaunchKernelGGL((k_fft_pass<KIND, AXIS, 2, FROM_SRC, SCALE>), g, b, 0, st, jobs);
     return hipGetLastError();
 }
 
-hipError_t fft_axis_pass(int kind, int axis, const void *src, bool src_is_fixed,
-                         double src_scale, double *dst, double out_scale, int nx, int ny,
-                         int nz, const FftAxisDev &ax, const double *lamx, const double *lamy,
-                         const double *lamz, hipStream_t st)
+hipError_t fft_pass(FftPass pass, const FftJob *jobs, int njobs, int64_t max_groups, int rs,
+                    hipStream_t st)
 {
-#define OHPIC_FFT_DISPATCH(K, A)                                                               \
-    if (src_is_fixed)                                                                          \
-        return launch_one<K, A, unsigned long long>(                                           \
-            (const unsigned long long *)src, dst, src_scale, out_scale, nx, ny, nz, ax, lamx,  \
-            lamy, lamz, st);                                                                   \
-    return launch_one<K, A, double>((const double *)src, dst, src_scale, out_scale, nx, ny,    \
-                                    nz, ax, lamx, lamy, lamz, st);
-    if (kind == kFftR2HC) {
-        if (axis == 0) { OHPIC_FFT_DISPATCH(kFftR2HC, 0) }
-        if (axis == 1) { OHPIC_FFT_DISPATCH(kFftR2HC, 1) }
-        OHPIC_FFT_DISPATCH(kFftR2HC, 2)
-    } else if (kind == kFftHC2R) {
-        if (axis == 0) { OHPIC_FFT_DISPATCH(kFftHC2R, 0) }
-        if (axis == 1) { OHPIC_FFT_DISPATCH(kFftHC2R, 1) }
-        OHPIC_FFT_DISPATCH(kFftHC2R, 2)
-    } else {
-        if (axis != 2) return hipErrorInvalidValue;
-        OHPIC_FFT_DISPATCH(kFftPoisson, 2)
+    switch (pass) {
+    case kPassR2hcXFromSrc: return launch_pass<kFftR2HC, 0, true, false>(jobs, njobs, max_groups, rs, st);
+    case kPassR2hcX: return launch_pass<kFftR2HC, 0, false, false>(jobs, njobs, max_groups, rs, st);
+    case kPassR2hcY: return launch_pass<kFftR2HC, 1, false, false>(jobs, njobs, max_groups, rs, st);
+    case kPassR2hcZ: return launch_pass<kFftR2HC, 2, false, false>(jobs, njobs, max_groups, rs, st);
+    case kPassPoissonZ: return launch_pass<kFftPoisson, 2, false, false>(jobs, njobs, max_groups, rs, st);
+    case kPassHc2rZ: return launch_pass<kFftHC2R, 2, false, false>(jobs, njobs, max_groups, rs, st);
+    case kPassHc2rY: return launch_pass<kFftHC2R, 1, false, false>(jobs, njobs, max_groups, rs, st);
+    case kPassHc2rXScaled: return launch_pass<kFftHC2R, 0, false, true>(jobs, njobs, max_groups, rs, st);
     }
-#undef OHPIC_FFT_DISPATCH
+    return hipErrorInvalidValue;
 }
 
 }  // namespace ohpic

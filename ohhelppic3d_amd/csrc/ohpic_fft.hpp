@@ -31,18 +31,51 @@ bool fft_factorize(int n, int *fac, int *nfac);
 // Host-side twiddle table exp(-2 pi i k/n) in long double, rounded to double.
 void fft_twiddles(int n, double2 *tw);
 
-int fft_group_lines(int n);
-size_t fft_lds_bytes(int n, int G);
-int64_t fft_groups(int axis, int G, int nx, int ny, int nz);
+// real lines per workgroup: G/2 complex lines of n fill at most kFftLdsCplx
+// complex values (32 KiB); at least 2 lines, at most 64
+__host__ __device__ inline int fft_group_lines(int n)
+{
+    int g = 2 * (kFftLdsCplx / (n > 0 ? n : 1));
+    if (g > 64) g = 64;
+    if (g < 2) g = 2;
+    return g & ~1;
+}
 
-// One pass over every line of `axis`.  kind R2HC/HC2R transform src -> dst
-// (in place allowed); kind Poisson (axis 2 only) runs forward, division by
-// lamx[kx]+lamy[ky]+lamz[kz] (DC -> 0) and inverse along z.  When
-// src_is_fixed the source is an int64 fixed-point grid multiplied by
-// src_scale on load.  Every output is multiplied by out_scale.
-hipError_t fft_axis_pass(int kind, int axis, const void *src, bool src_is_fixed,
-                         double src_scale, double *dst, double out_scale, int nx, int ny,
-                         int nz, const FftAxisDev &ax, const double *lamx, const double *lamy,
-                         const double *lamz, hipStream_t st);
+// workgroups of one axis pass (G real lines each)
+__host__ __device__ inline int64_t fft_groups(int axis, int G, int nx, int ny, int nz)
+{
+    if (axis == 0) return ((int64_t)ny * nz + G - 1) / G;
+    if (axis == 1) return (int64_t)((nx + G - 1) / G) * nz;
+    return ((int64_t)nx * ny + G - 1) / G;
+}
+
+// One transform job: an nx*ny*nz array with its per-axis plans and (for the
+// Poisson pass) the Green's-function denominators per axis.
+struct FftJob {
+    const double *src;     // source of the first Poisson pass (rho)
+    double *data;          // array transformed in place (phi)
+    int n[3];
+    FftAxisDev ax[3];
+    const double *lam[3];  // (2 sin(pi k/n))^2 per axis
+    double inv_n;          // 1 / (nx ny nz)
+};
+
+enum FftPass : int {
+    kPassR2hcXFromSrc = 0,  // R2HC along x, src -> data
+    kPassR2hcX,             // R2HC along x in place
+    kPassR2hcY,
+    kPassR2hcZ,
+    kPassPoissonZ,          // R2HC z, divide by lam sum (DC := 0), HC2R z
+    kPassHc2rZ,
+    kPassHc2rY,
+    kPassHc2rXScaled        // HC2R along x, times inv_n
+};
+
+int fft_radix_set(const FftAxisDev &ax);
+
+// One axis pass for every job of the device table `jobs`; max_groups is the
+// largest group count over the jobs, rs the largest radix set.
+hipError_t fft_pass(FftPass pass, const FftJob *jobs, int njobs, int64_t max_groups, int rs,
+                    hipStream_t st);
 
 }  // namespace ohpic

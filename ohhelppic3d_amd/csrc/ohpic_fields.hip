@@ -20,14 +20,6 @@ __global__ void k_fill(T *__restrict__ p, int64_t n, T v)
         p[i] = v;
 }
 
-__global__ void k_rho_from_acc(const unsigned long long *__restrict__ acc, double *__restrict__ rho,
-                               int64_t n, double scale)
-{
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * blockDim.x)
-        rho[i] = (double)(long long)acc[i] * scale;
-}
-
 __device__ __forceinline__ double wave_sum(double v)
 {
     for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
@@ -36,19 +28,27 @@ __device__ __forceinline__ double wave_sum(double v)
 
 // E_a(i) = 0.5*((phi(i-e_a) - phi(i)) + (phi(i) - phi(i+e_a))): the edge
 // difference of poisson_field_solver.f90:139-144 averaged onto nodes as at
-// :165-170, periodic.  Each block walks whole x rows (32-bit row indexing)
-// and writes its partial sums of rho*phi, |E|^2 and rho to partial[3*block];
-// k_reduce_partials adds them in block order.
-__global__ void __launch_bounds__(256)
-    k_efield(const double *__restrict__ phi, const double *__restrict__ rho,
-             double *__restrict__ ex, double *__restrict__ ey, double *__restrict__ ez, int nx,
-             int ny, int nz, double *__restrict__ partial)
+// :165-170, periodic.  blockIdx.y = instance; each block walks whole x rows
+// (32-bit row indexing) and writes its partial sums of rho*phi, |E|^2 and
+// rho to e_partial[3*block]; k_reduce_e_b adds them in block order.
+__global__ void __launch_bounds__(256) k_efield_b(const InstDev *__restrict__ insts)
 {
     __shared__ double red[3][4];
+    const InstDev *I = insts + blockIdx.y;
+    const double *__restrict__ phi = I->phi;
+    const double *__restrict__ rho = I->rho;
+    double *__restrict__ ex = I->e[0];
+    double *__restrict__ ey = I->e[1];
+    double *__restrict__ ez = I->e[2];
+    const int nx = I->n[0], ny = I->n[1], nz = I->n[2];
     const int rows = ny * nz;
+    // the block count depends on this instance only, so the partial sums
+    // (and the diagnostics) do not depend on the other instances
+    const int nblk = rows < kEfieldMaxBlocks ? rows : kEfieldMaxBlocks;
+    if ((int)blockIdx.x >= nblk) return;
     const int64_t sy = nx, sz = (int64_t)nx * ny;
     double we = 0.0, we2 = 0.0, qn = 0.0;
-    for (int r = blockIdx.x; r < rows; r += gridDim.x) {
+    for (int r = blockIdx.x; r < rows; r += nblk) {
         const int iy = r % ny;
         const int iz = r / ny;
         const int64_t row = (int64_t)r * nx;
@@ -90,31 +90,92 @@ __global__ void __launch_bounds__(256)
             b += red[1][i];
             c += red[2][i];
         }
-        partial[3 * blockIdx.x + 0] = a;
-        partial[3 * blockIdx.x + 1] = b;
-        partial[3 * blockIdx.x + 2] = c;
+        I->e_partial[3 * blockIdx.x + 0] = a;
+        I->e_partial[3 * blockIdx.x + 1] = b;
+        I->e_partial[3 * blockIdx.x + 2] = c;
     }
 }
 
-// out[k] (+)= sum over i of partial[i*stride + k], k < width, summed in index
-// order by one block (deterministic).
-__global__ void __launch_bounds__(256)
-    k_reduce_partials(const double *__restrict__ partial, int count, int stride, int width,
-                      double *__restrict__ out, int accumulate)
+// Sum of partial[i*stride + k] over i < count, in index order, by one block.
+__device__ double block_ordered_sum(const double *__restrict__ partial, int count, int stride,
+                                    int k, double *red)
 {
-    __shared__ double red[256];
-    for (int k = 0; k < width; ++k) {
-        double s = 0.0;
-        for (int i = threadIdx.x; i < count; i += blockDim.x) s += partial[(int64_t)i * stride + k];
-        red[threadIdx.x] = s;
-        __syncthreads();
-        for (int h = blockDim.x / 2; h > 0; h >>= 1) {
-            if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
-            __syncthreads();
-        }
-        if (threadIdx.x == 0) out[k] = (accumulate ? out[k] : 0.0) + red[0];
+    double s = 0.0;
+    for (int i = threadIdx.x; i < count; i += blockDim.x) s += partial[(int64_t)i * stride + k];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int h = blockDim.x / 2; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
         __syncthreads();
     }
+    const double r = red[0];
+    __syncthreads();
+    return r;
+}
+
+// blockIdx.x = instance: diag[kDiagWe + k] := sum of the instance's e_partial rows
+__global__ void __launch_bounds__(256) k_reduce_e_b(const InstDev *__restrict__ insts)
+{
+    __shared__ double red[256];
+    const InstDev *I = insts + blockIdx.x;
+    const int rows = I->n[1] * I->n[2];
+    const int count = rows < kEfieldMaxBlocks ? rows : kEfieldMaxBlocks;
+    for (int k = 0; k < 3; ++k) {
+        const double v = block_ordered_sum(I->e_partial, count, 3, k, red);
+        if (threadIdx.x == 0) I->diag[kDiagWe + k] = v;
+    }
+}
+
+// blockIdx.x = instance: diag[kDiagKeOld + k] := sum over items of ke_partial
+__global__ void __launch_bounds__(256) k_reduce_ke_b(const InstDev *__restrict__ insts)
+{
+    __shared__ double red[256];
+    const InstDev *I = insts + blockIdx.x;
+    for (int k = 0; k < kKeSlots; ++k) {
+        const double v = block_ordered_sum(I->ke_partial, I->nitems, kKeSlots, k, red);
+        if (threadIdx.x == 0) I->diag[kDiagKeOld + k] = v;
+    }
+}
+
+__global__ void k_rho_from_acc_b(const InstDev *__restrict__ insts, const int *__restrict__ parity)
+{
+    const InstDev *I = insts + blockIdx.y;
+    const unsigned long long *__restrict__ acc = I->rho_acc[parity[blockIdx.y]];
+    double *__restrict__ rho = I->rho;
+    const int64_t n = (int64_t)I->n[0] * I->n[1] * I->n[2];
+    const double scale = I->fix_inv;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        rho[i] = (double)(long long)acc[i] * scale;
+}
+
+__global__ void k_prepare_next_b(const InstDev *__restrict__ insts, const int *__restrict__ parity)
+{
+    const InstDev *I = insts + blockIdx.y;
+    const int nb = parity[blockIdx.y] ^ 1;
+    const int64_t n = (int64_t)I->n[0] * I->n[1] * I->n[2];
+    const int64_t t0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    unsigned long long *__restrict__ acc = I->rho_acc[nb];
+    for (int64_t i = t0; i < n; i += stride) acc[i] = 0ull;
+    for (int s = 0; s < I->nspecies; ++s) {
+        unsigned long long *__restrict__ w = I->sp[s].count[nb];
+        for (int64_t t = t0; t < I->ntiles; t += stride) w[t] = 0ull;
+    }
+    if (t0 < kFlagSlots) I->flags[t0] = 0ull;
+}
+
+// After a re-binning step: out[i*kFlagSlots + k] := flags[k] of instance i,
+// and the instance's parity flips unless a bin overflowed (the host then
+// re-runs that instance's step in list mode from the unchanged buffer).
+__global__ void k_finish_b(const InstDev *__restrict__ insts, int *__restrict__ parity, int ninst,
+                           unsigned long long *__restrict__ out)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ninst) return;
+    const unsigned long long *f = insts[i].flags;
+    for (int k = 0; k < kFlagSlots; ++k) out[i * kFlagSlots + k] = f[k];
+    if (f[kFlagOverflow] == 0ull) parity[i] ^= 1;
 }
 
 __global__ void k_load_uniform(Pcg32Dev rng0, uint64_t offset, int64_t np, int nx, int ny,
@@ -208,38 +269,56 @@ hipError_t launch_fill_f64(double *p, int64_t n, double v, hipStream_t st)
     return hipGetLastError();
 }
 
-hipError_t launch_rho_from_acc(const unsigned long long *acc, double *rho, int64_t n,
-                               double scale, hipStream_t st)
+static unsigned grid_cap(int64_t n, int64_t cap)
 {
-    if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_rho_from_acc, dim3(grid_for(n)), dim3(256), 0, st, acc, rho, n, scale);
+    int64_t b = (n + 255) / 256;
+    if (b > cap) b = cap;
+    if (b < 1) b = 1;
+    return (unsigned)b;
+}
+
+hipError_t launch_rho_from_acc_b(const InstDev *insts, const int *parity, int ninst,
+                                 int64_t max_ng, hipStream_t st)
+{
+    if (ninst <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_rho_from_acc_b, dim3(grid_cap(max_ng, 4096), ninst), dim3(256), 0, st,
+                       insts, parity);
     return hipGetLastError();
 }
 
-int efield_blocks(int ny, int nz)
+hipError_t launch_efield_b(const InstDev *insts, int ninst, int max_rows, hipStream_t st)
 {
-    const int rows = ny * nz;
-    return rows < kEfieldMaxBlocks ? rows : kEfieldMaxBlocks;
-}
-
-hipError_t launch_efield(const double *phi, const double *rho, double *ex, double *ey,
-                         double *ez, int nx, int ny, int nz, double *partial, double *diag,
-                         hipStream_t st)
-{
-    const int blocks = efield_blocks(ny, nz);
-    hipLaunchKernelGGL(k_efield, dim3(blocks), dim3(256), 0, st, phi, rho, ex, ey, ez, nx, ny,
-                       nz, partial);
-    hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(256), 0, st, partial, blocks, 3, 3,
-                       diag + kDiagWe, 0);
+    if (ninst <= 0) return hipSuccess;
+    const int blocks = max_rows < kEfieldMaxBlocks ? (max_rows > 0 ? max_rows : 1)
+                                                   : kEfieldMaxBlocks;
+    hipLaunchKernelGGL(k_efield_b, dim3(blocks, ninst), dim3(256), 0, st, insts);
+    hipLaunchKernelGGL(k_reduce_e_b, dim3(ninst), dim3(256), 0, st, insts);
     return hipGetLastError();
 }
 
-hipError_t launch_reduce_partials(const double *partial, int count, int stride, int width,
-                                  double *out, int accumulate, hipStream_t st)
+hipError_t launch_prepare_next_b(const InstDev *insts, const int *parity, int ninst,
+                                 int64_t max_ng, int max_ntiles, hipStream_t st)
 {
-    if (count <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(256), 0, st, partial, count, stride,
-                       width, out, accumulate);
+    if (ninst <= 0) return hipSuccess;
+    const int64_t n = max_ng > max_ntiles ? max_ng : max_ntiles;
+    hipLaunchKernelGGL(k_prepare_next_b, dim3(grid_cap(n, 4096), ninst), dim3(256), 0, st, insts,
+                       parity);
+    return hipGetLastError();
+}
+
+hipError_t launch_reduce_ke_b(const InstDev *insts, int ninst, hipStream_t st)
+{
+    if (ninst <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_reduce_ke_b, dim3(ninst), dim3(256), 0, st, insts);
+    return hipGetLastError();
+}
+
+hipError_t launch_finish_b(const InstDev *insts, int *parity, int ninst, unsigned long long *out,
+                           hipStream_t st)
+{
+    if (ninst <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_finish_b, dim3((ninst + 255) / 256), dim3(256), 0, st, insts, parity,
+                       ninst, out);
     return hipGetLastError();
 }
 

@@ -75,6 +75,8 @@ struct InstDev {
     double *e[3];
     double *diag;                 // kDiagSlots
     double *ke_partial;           // [items][kKeSlots], reduced in item order
+    double *e_partial;            // [kEfieldMaxBlocks][3] field-energy partial sums
+    int nitems;                   // work items of this instance (rows of ke_partial)
     unsigned long long *flags;    // kFlagSlots
 };
 
@@ -87,6 +89,7 @@ struct WorkItem {
 };
 
 constexpr int kKeSlots = 2 * kMaxSpecies;  // per item: ke_old[8], ke_new[8]
+constexpr int kEfieldMaxBlocks = 4096;     // rows of e_partial
 
 // Records the message ohpic_last_error(NULL) returns (host side).
 void set_global_error(const char *msg);

@@ -27,9 +27,10 @@ struct Pcg32Dev {
 
 bool tile_supported(int tx, int ty, int tz);
 
+// parity[item.inst] selects the current buffer of each item's instance.
 hipError_t launch_particles(int mode, int precision, int weight_order, const int T[3],
-                            const InstDev *insts, const WorkItem *items, int nitems, int par,
-                            hipStream_t st);
+                            const InstDev *insts, const WorkItem *items, int nitems,
+                            const int *parity, hipStream_t st);
 // Bin a contiguous list [0, n) of buffer lbuf into the front runs of buffer
 // lbuf^1 (count first, then scatter once capacities are set); wrap != 0
 // applies the periodic wrap twice to each position first.
@@ -45,7 +46,7 @@ hipError_t launch_cell_counts(const InstDev *insts, int inst, int s, int par, in
 // mixed-precision particles (ohpic_particles_mixed.hip)
 hipError_t launch_particles_mixed(int mode, int weight_order, const int T[3],
                                   const InstDev *insts, const WorkItem *items, int nitems,
-                                  int par, hipStream_t st);
+                                  const int *parity, hipStream_t st);
 hipError_t launch_bin_count_mx(const InstDev *insts, int inst, int s, int lbuf, int64_t n,
                                const int T[3], hipStream_t st);
 hipError_t launch_bin_scatter_mx(const InstDev *insts, int inst, int s, int lbuf, int64_t n,
@@ -60,16 +61,23 @@ hipError_t launch_fill_u64(unsigned long long *p, int64_t n, unsigned long long 
                            hipStream_t st);
 hipError_t launch_fill_i32(int32_t *p, int64_t n, int32_t v, hipStream_t st);
 hipError_t launch_fill_f64(double *p, int64_t n, double v, hipStream_t st);
-hipError_t launch_rho_from_acc(const unsigned long long *acc, double *rho, int64_t n,
-                               double scale, hipStream_t st);
-constexpr int kEfieldMaxBlocks = 4096;
-int efield_blocks(int ny, int nz);
-// partial: scratch of 3 * efield_blocks(ny, nz) doubles; writes diag[kDiagWe..kDiagQnet]
-hipError_t launch_efield(const double *phi, const double *rho, double *ex, double *ey,
-                         double *ez, int nx, int ny, int nz, double *partial, double *diag,
-                         hipStream_t st);
-hipError_t launch_reduce_partials(const double *partial, int count, int stride, int width,
-                                  double *out, int accumulate, hipStream_t st);
+// Batched grid kernels: every launch covers instances [0, ninst) of the
+// table (blockIdx.y = instance).
+// rho := int64 charge grid of the current buffer * 2^-charge_bits
+hipError_t launch_rho_from_acc_b(const InstDev *insts, const int *parity, int ninst,
+                                 int64_t max_ng, hipStream_t st);
+// node E from phi + block partials of sum rho*phi, sum E^2, sum rho; then the
+// partials are reduced into diag[kDiagWe..kDiagQnet]
+hipError_t launch_efield_b(const InstDev *insts, int ninst, int max_rows, hipStream_t st);
+// zero the next buffer's bin words and charge grid and the flags
+hipError_t launch_prepare_next_b(const InstDev *insts, const int *parity, int ninst,
+                                 int64_t max_ng, int max_ntiles, hipStream_t st);
+// diag[kDiagKeOld..] := sums of ke_partial over the instance's items
+hipError_t launch_reduce_ke_b(const InstDev *insts, int ninst, hipStream_t st);
+// out[i * kFlagSlots + k] := flags[k] of instance i; parity[i] ^= 1 unless
+// instance i overflowed a bin
+hipError_t launch_finish_b(const InstDev *insts, int *parity, int ninst, unsigned long long *out,
+                           hipStream_t st);
 
 // seeded loaders (ohpic_fields.hip, ohpic_particles_mixed.hip)
 hipError_t launch_load_uniform(Pcg32Dev rng0, uint64_t offset, int64_t np, int nx, int ny,

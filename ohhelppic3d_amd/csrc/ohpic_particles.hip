@@ -105,7 +105,7 @@ template <int TX, int TY, int TZ, int WO, int MODE>
 __global__ void __launch_bounds__(kBlock)
     __attribute__((amdgpu_waves_per_eu(OHPIC_PUSH_MIN_WAVES, 8)))
     k_particles_fp64(const InstDev *__restrict__ insts, const WorkItem *__restrict__ items,
-                     int par)
+                     const int *__restrict__ parity)
 {
     constexpr bool LIST = (MODE == kModeFusedList || MODE == kModeBoundaryList);
     constexpr bool PUSH = (MODE == kModeFused || MODE == kModeGatherPush || MODE == kModeFusedList);
@@ -122,6 +122,7 @@ __global__ void __launch_bounds__(kBlock)
 
     const WorkItem it = items[blockIdx.x];
     const InstDev *__restrict__ I = insts + it.inst;
+    const int par = parity[it.inst];
     const int nx = I->n[0], ny = I->n[1], nz = I->n[2];
     const int ntx = I->nt[0], nty = I->nt[1];
     const int tile = it.tile;
@@ -516,33 +517,33 @@ __global__ void k_cell_counts(const InstDev *__restrict__ insts, int inst, int s
 // ---------------------------------------------------------------- launchers
 template <int TX, int TY, int TZ, int WO>
 static hipError_t launch_tile(int mode, const InstDev *insts, const WorkItem *items,
-                              int nitems, int par, hipStream_t st)
+                              int nitems, const int *parity, hipStream_t st)
 {
     dim3 g(nitems), b(kBlock);
     switch (mode) {
     case kModeFused:
         hipLaunchKernelGGL((k_particles_fp64<TX, TY, TZ, WO, kModeFused>), g, b, 0, st, insts,
-                           items, par);
+                           items, parity);
         break;
     case kModeDeposit:
         hipLaunchKernelGGL((k_particles_fp64<TX, TY, TZ, WO, kModeDeposit>), g, b, 0, st,
-                           insts, items, par);
+                           insts, items, parity);
         break;
     case kModeGatherPush:
         hipLaunchKernelGGL((k_particles_fp64<TX, TY, TZ, WO, kModeGatherPush>), g, b, 0, st,
-                           insts, items, par);
+                           insts, items, parity);
         break;
     case kModeBoundary:
         hipLaunchKernelGGL((k_particles_fp64<TX, TY, TZ, WO, kModeBoundary>), g, b, 0, st,
-                           insts, items, par);
+                           insts, items, parity);
         break;
     case kModeFusedList:
         hipLaunchKernelGGL((k_particles_fp64<TX, TY, TZ, WO, kModeFusedList>), g, b, 0, st,
-                           insts, items, par);
+                           insts, items, parity);
         break;
     case kModeBoundaryList:
         hipLaunchKernelGGL((k_particles_fp64<TX, TY, TZ, WO, kModeBoundaryList>), g, b, 0, st,
-                           insts, items, par);
+                           insts, items, parity);
         break;
     default:
         return hipErrorInvalidValue;
@@ -557,18 +558,18 @@ bool tile_supported(int tx, int ty, int tz)
 }
 
 hipError_t launch_particles(int mode, int precision, int weight_order, const int T[3],
-                            const InstDev *insts, const WorkItem *items, int nitems, int par,
+                            const InstDev *insts, const WorkItem *items, int nitems, const int *parity,
                             hipStream_t st)
 {
     if (nitems <= 0) return hipSuccess;
     if (precision == 1)
-        return launch_particles_mixed(mode, weight_order, T, insts, items, nitems, par, st);
+        return launch_particles_mixed(mode, weight_order, T, insts, items, nitems, parity, st);
     if (precision != 0) return hipErrorInvalidValue;
 #define OHPIC_TILE(A, B, C)                                                     \
     if (T[0] == A && T[1] == B && T[2] == C)                                    \
         return weight_order == 0                                                \
-                   ? launch_tile<A, B, C, 0>(mode, insts, items, nitems, par, st) \
-                   : launch_tile<A, B, C, 1>(mode, insts, items, nitems, par, st);
+                   ? launch_tile<A, B, C, 0>(mode, insts, items, nitems, parity, st) \
+                   : launch_tile<A, B, C, 1>(mode, insts, items, nitems, parity, st);
     OHPIC_TILE(4, 4, 4)
     OHPIC_TILE(8, 8, 8)
     OHPIC_TILE(8, 8, 4)

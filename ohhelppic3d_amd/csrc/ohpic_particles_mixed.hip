@@ -140,7 +140,7 @@ template <int TX, int TY, int TZ, int WO, int MODE>
 __global__ void __launch_bounds__(kBlock)
     __attribute__((amdgpu_waves_per_eu(OHPIC_PUSH_MIN_WAVES, 8)))
     k_particles_mixed(const InstDev *__restrict__ insts, const WorkItem *__restrict__ items,
-                      int par)
+                      const int *__restrict__ parity)
 {
     constexpr bool LIST = (MODE == kModeFusedList || MODE == kModeBoundaryList);
     constexpr bool PUSH = (MODE == kModeFused || MODE == kModeGatherPush || MODE == kModeFusedList);
@@ -157,6 +157,7 @@ __global__ void __launch_bounds__(kBlock)
 
     const WorkItem it = items[blockIdx.x];
     const InstDev *__restrict__ I = insts + it.inst;
+    const int par = parity[it.inst];
     const int nx = I->n[0], ny = I->n[1], nz = I->n[2];
     const int ntx = I->nt[0], nty = I->nt[1];
     const int tile = it.tile;
@@ -602,13 +603,13 @@ __global__ void k_load_hump_mx(const InstDev *__restrict__ insts, int inst, int 
 // ---------------------------------------------------------------- launchers
 template <int TX, int TY, int TZ, int WO>
 static hipError_t launch_tile_mx(int mode, const InstDev *insts, const WorkItem *items,
-                                 int nitems, int par, hipStream_t st)
+                                 int nitems, const int *parity, hipStream_t st)
 {
     dim3 g(nitems), b(kBlock);
 #define OHPIC_MX_CASE(M)                                                                 \
     case M:                                                                              \
         hipLaunchKernelGGL((k_particles_mixed<TX, TY, TZ, WO, M>), g, b, 0, st, insts, items, \
-                           par);                                                         \
+                           parity);                                                      \
         break;
     switch (mode) {
         OHPIC_MX_CASE(kModeFused)
@@ -626,13 +627,13 @@ static hipError_t launch_tile_mx(int mode, const InstDev *insts, const WorkItem 
 
 hipError_t launch_particles_mixed(int mode, int weight_order, const int T[3],
                                   const InstDev *insts, const WorkItem *items, int nitems,
-                                  int par, hipStream_t st)
+                                  const int *parity, hipStream_t st)
 {
 #define OHPIC_TILE(A, B, C)                                                        \
     if (T[0] == A && T[1] == B && T[2] == C)                                       \
         return weight_order == 0                                                   \
-                   ? launch_tile_mx<A, B, C, 0>(mode, insts, items, nitems, par, st) \
-                   : launch_tile_mx<A, B, C, 1>(mode, insts, items, nitems, par, st);
+                   ? launch_tile_mx<A, B, C, 0>(mode, insts, items, nitems, parity, st) \
+                   : launch_tile_mx<A, B, C, 1>(mode, insts, items, nitems, parity, st);
     OHPIC_TILE(4, 4, 4)
     OHPIC_TILE(8, 8, 8)
     OHPIC_TILE(8, 8, 4)

@@ -249,6 +249,76 @@ class Simulation:
         return b.value
 
 
+class Group:
+    """Instances advanced together (``ohpic_group``): one launch per kernel and
+    step over every member's work items and FFT jobs, replayed from a captured
+    hipGraph.  Members must share device, precision, weight order and tile;
+    each member's result equals stepping it alone with ``Simulation.step``."""
+
+    def __init__(self, device: int = 0, use_graph: bool = True):
+        self._lib = L.lib()
+        self._g = C.c_void_p()
+        check(self._lib.ohpic_group_create(int(device), C.byref(self._g)))
+        self.members: list = []
+        if not use_graph:
+            self.use_graph(False)
+
+    def _chk(self, rc):
+        L.check_group(rc, self._g)
+
+    def close(self) -> None:
+        if self._g:
+            self._lib.ohpic_group_destroy(self._g)
+            self._g = C.c_void_p()
+            self.members = []
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def attach(self, sim: Simulation) -> None:
+        self._chk(self._lib.ohpic_group_attach(self._g, sim._ctx))
+        self.members.append(sim)
+
+    def detach(self, sim: Simulation) -> None:
+        self._chk(self._lib.ohpic_group_detach(self._g, sim._ctx))
+        self.members.remove(sim)
+
+    def __len__(self) -> int:
+        n = C.c_int()
+        self._chk(self._lib.ohpic_group_size(self._g, C.byref(n)))
+        return n.value
+
+    def step(self, nsteps: int = 1) -> None:
+        self._chk(self._lib.ohpic_group_step(self._g, int(nsteps)))
+
+    def use_graph(self, on: bool) -> None:
+        self._chk(self._lib.ohpic_group_use_graph(self._g, 1 if on else 0))
+
+    def stats(self) -> dict:
+        s, c, i = C.c_int64(), C.c_int64(), C.c_int64()
+        self._chk(self._lib.ohpic_group_stats(self._g, C.byref(s), C.byref(c), C.byref(i)))
+        return {"steps": s.value, "captures": c.value, "items": i.value}
+
+
+def autotune(cfgs: Sequence[SimConfig], sample: int = 16, steps: int = 3) -> dict:
+    """Fastest (tile, chunks) for a group of these configurations, timed on up
+    to ``sample`` of them (``ohpic_autotune``)."""
+    arr = (L.Config * len(cfgs))(*[c.to_c() for c in cfgs])
+    r = L.TuneResult()
+    check(L.lib().ohpic_autotune(arr, len(cfgs), int(sample), int(steps), C.byref(r)))
+    return {"tile": tuple(r.tile), "chunks": r.chunks,
+            "particle_steps_per_s": r.particle_steps_per_s, "candidates": r.candidates}
+
+
 def estimate_footprint(cfg: SimConfig) -> int:
     b = C.c_int64()
     c = cfg.to_c()
