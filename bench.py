@@ -198,6 +198,69 @@ def run_cpu_baseline(seconds: float):
                       f"{len(times)} steps, median step {med * 1e3:.1f} ms, 1 thread"}
 
 
+def run_ensemble_bench(args):
+    """128 independent mixed-precision instances (each axis from
+    {16,24,32,48,64}, 8..64 ppc per species, v_th,e in [0.5,2], omega_ce in
+    [0,1], dt 0.1) advanced as one group: autotuned tile/chunks, one launch
+    per kernel and step replayed from a hipGraph.  Also times the same
+    instances stepped one after another for the batching gain."""
+    from ohhelppic3d_amd import ensemble
+
+    specs = ensemble.sample_ensemble(args.ensemble_instances, seed=2024)
+    log(f"[ensemble] {len(specs)} instances, {sum(s.particles for s in specs)} particles, "
+        f"{sum(s.cells for s in specs)} cells")
+    res = ensemble.run_ensemble(specs, steps=args.ensemble_steps, progress=log)
+    tune = {"tile": tuple(res["tile"]), "chunks": res["chunks"]}
+    log("[ensemble] same instances stepped one after another")
+    seq = ensemble.run_sequential(specs, steps=args.ensemble_seq_steps, tune=tune)
+    res["sequential"] = seq
+    res["batching_gain"] = res["value"] / seq["value"]
+    res["config"] = {"workload": f"ensemble{len(specs)}_mixed", "seed": 2024,
+                     "axes": list(ensemble.AXES), "ppc_range": [8, 64], "dt": 0.1}
+    return res
+
+
+def run_serving_bench(args):
+    """Serving loop: seeded requests (grids 32^3/64^3/96^3/128^3 weighted
+    4:3:2:1, 16..64 ppc per species, 200..2000 steps, mixed precision)
+    admitted against a 240 GB budget.  Capacity is measured with every
+    request arriving at once; then Poisson arrivals at 50/80/100% of it."""
+    import ohhelppic3d_amd as ohp
+    from ohhelppic3d_amd import serve
+
+    n = args.serve_requests
+    probe = serve.sample_requests(n, rate=float("inf"), seed=7)
+    cfgs = [r.spec.config() for r in probe[:16]]
+    t = time.perf_counter()
+    tune = ohp.autotune(cfgs, sample=8, steps=2)
+    log(f"[serve] autotune: tile {tune['tile']}, chunks {tune['chunks']} "
+        f"({time.perf_counter() - t:.1f} s)")
+    loop = serve.ServingLoop(tile=tune["tile"], chunks=tune["chunks"])
+    log(f"[serve] capacity run: {n} requests at t=0")
+    rep = loop.run(probe, progress=log)
+    cap = rep.summary()
+    cap_rps = cap["done"] / cap["makespan"]
+    log(f"[serve] capacity {cap_rps:.3f} requests/s, "
+        f"{cap['particle_steps_per_s']:.3e} particle-steps/s")
+    levels = {"saturated": cap}
+    for k, load in enumerate((0.5, 0.8, 1.0)):
+        reqs = serve.sample_requests(n, rate=load * cap_rps, seed=8 + k)
+        log(f"[serve] offered load {int(load * 100)}%: {n} requests at "
+            f"{load * cap_rps:.3f}/s")
+        levels[f"load{int(load * 100)}"] = loop.run(reqs, progress=log).summary()
+    top = levels["load100"]
+    return {
+        "metric": "serving loop: p50/p99 time-to-completion and aggregate particle-steps/s",
+        "value": top["particle_steps_per_s"], "unit": "particle-steps/s", "n_gpus": 1,
+        "higher_is_better": True, "dtype": "f32+f64", "data": "synthetic (seeded requests)",
+        "config": {"workload": f"serving{n}_mixed", "grids": [list(g) for g in serve.GRIDS],
+                   "grid_weights": list(serve.GRID_WEIGHTS), "ppc_range": [16, 64],
+                   "steps_range": [200, 2000], "budget_bytes": serve.HBM_BUDGET,
+                   "tile": list(tune["tile"]), "chunks": tune["chunks"]},
+        "capacity_requests_per_s": cap_rps, "levels": levels,
+    }
+
+
 def run_copy_calibration(gib: float = 2.0, reps: int = 5):
     """A plain device copy of known size: its FETCH_SIZE / WRITE_SIZE counters
     calibrate the per-dispatch HBM byte counts of the PIC kernels."""
@@ -229,9 +292,19 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true")
+    ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--ensemble-instances", type=int, default=128)
+    ap.add_argument("--ensemble-steps", type=int, default=500)
+    ap.add_argument("--ensemble-seq-steps", type=int, default=10)
+    ap.add_argument("--serve", action="store_true",
+                    help="run the serving-loop workload instead (its own JSON line)")
+    ap.add_argument("--serve-requests", type=int, default=48)
     args = ap.parse_args()
     if args.copy_calibration:
         print(json.dumps(run_copy_calibration()), flush=True)
+        return
+    if args.serve:
+        print(json.dumps(run_serving_bench(args)), flush=True)
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -249,6 +322,8 @@ def main():
         log("[bench] mixed vs FP64 one-step error at 128^3")
         mixed["error_vs_fp64"] = mixed_error_vs_fp64(args)
         res["mixed"] = mixed
+    if not args.no_ensemble and args.workload == "uniform256_ppc64_fp64":
+        res["ensemble"] = run_ensemble_bench(args)
     if not args.no_cpu:
         log("[bench] timing the CPU baseline (oracle)")
         res["cpu_baseline"] = run_cpu_baseline(args.cpu_seconds)

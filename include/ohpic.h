@@ -216,7 +216,12 @@ int ohpic_group_destroy(ohpic_group *g); /* detaches every member */
 int ohpic_group_attach(ohpic_group *g, ohpic_ctx *ctx);
 int ohpic_group_detach(ohpic_group *g, ohpic_ctx *ctx);
 int ohpic_group_size(const ohpic_group *g, int *n);
-/* advance every member by nsteps (same sequence as ohpic_step) */
+int ohpic_group_contains(const ohpic_group *g, const ohpic_ctx *ctx, int *yes);
+/* Advance every member by nsteps (same sequence as ohpic_step).  When a
+ * member's step fails (e.g. a particle moved more than two box lengths) the
+ * other members still complete that step, the failed members are detached
+ * (their contexts keep the message and refuse further steps) and the first
+ * failure's code is returned. */
 int ohpic_group_step(ohpic_group *g, int nsteps);
 /* enable != 0 (default): replay captured hipGraphs; 0: launch directly */
 int ohpic_group_use_graph(ohpic_group *g, int enable);

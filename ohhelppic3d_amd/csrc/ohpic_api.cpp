@@ -84,6 +84,7 @@ struct ohpic_ctx {
     uint64_t inst_ver = 0;       // bumped whenever h is uploaded
     uint64_t items_ver = 0;      // bumped whenever the work list changes
     ohpic_group *group = nullptr;
+    bool failed = false;         // a step failed: the state is no longer valid
     unsigned long long *h_flags = nullptr;  // pinned
     double *h_diag = nullptr;               // pinned
     int par = 0;
@@ -95,7 +96,7 @@ struct ohpic_ctx {
     int num_cus = 256;
     bool timing = false;
     TimerSlot timers[kNumTimers];
-    std::vector<void *> allocs;
+    std::vector<std::pair<void *, int64_t>> allocs;  // device pointer, bytes
     int64_t bytes = 0;
     std::string err;
 };
@@ -135,7 +136,7 @@ int dalloc(ohpic_ctx *ctx, T **p, int64_t count)
         return fail(ctx, OHPIC_ENOMEM, "hipMalloc of %zu bytes failed: %s", bytes,
                     hipGetErrorString(e));
     }
-    ctx->allocs.push_back((void *)*p);
+    ctx->allocs.emplace_back((void *)*p, (int64_t)bytes);
     ctx->bytes += (int64_t)bytes;
     return OHPIC_OK;
 }
@@ -175,21 +176,36 @@ int derive_charge_bits(const ohpic_config &c, int64_t ng)
     return std::max(8, std::min(s1, s2));
 }
 
-int64_t estimate_bytes(const ohpic_config &c)
+// Work items a context builds (build_work_items): chunks per tile given, or
+// the automatic choice, which never exceeds 8 items per CU plus one per tile.
+int64_t max_work_items(const ohpic_config &c, int64_t ntiles, int num_cus)
+{
+    if (c.chunks > 0) return ntiles * c.chunks;
+    return ntiles + (int64_t)num_cus * 8;
+}
+
+// Device bytes ohpic_create allocates for c (plus the lazily allocated
+// per-cell count buffer), allocation by allocation.
+int64_t estimate_bytes(const ohpic_config &c, int num_cus)
 {
     int T[3];
     pick_tile(c, T);
     const int64_t nt = (int64_t)((c.nx + T[0] - 1) / T[0]) * ((c.ny + T[1] - 1) / T[1]) *
                        ((c.nz + T[2] - 1) / T[2]);
     const int64_t ng = (int64_t)c.nx * c.ny * c.nz;
-    const int per_particle = (c.precision == OHPIC_MIXED ? 28 : 48) + (c.track_ids ? 8 : 0);
+    const int64_t per_particle = (c.precision == OHPIC_MIXED ? 7 * 4 : 6 * 8) + (c.track_ids ? 8 : 0);
     int64_t b = 0;
     for (int s = 0; s < c.nspecies; ++s) {
-        b += 2 * species_alloc(c, s, (int)nt) * per_particle;
-        b += nt * (8 + 4 + 8 + 8);
+        b += 2 * std::max<int64_t>(species_alloc(c, s, (int)nt), 1) * per_particle;
+        b += nt * (2 * 8 + 8 + 4);  // bin words x2, bin base, bin capacity
     }
-    b += ng * (8 * 2 + 8 * 5) + ng * 4;  // acc x2, rho, phi, E x3, cell counts
-    b += nt * (16 + 8 + 8 + 4 + 4) * 2;
+    b += ng * (2 * 8 + 5 * 8 + 4);           // int64 charge x2, rho, phi, E, cell counts
+    b += nt * (8 + 8 + 4 + 4) + 8;           // host-transfer scratch, cursor
+    b += 3 * kEfieldMaxBlocks * 8;           // field-energy partials
+    b += kDiagSlots * 8 + kFlagSlots * 8 + (int64_t)sizeof(InstDev) + 2 * 4 +
+         (int64_t)sizeof(FftJob);
+    b += (int64_t)(c.nx + c.ny + c.nz) * (16 + 8);  // twiddles, Green's-function tables
+    b += max_work_items(c, nt, num_cus) * ((int64_t)sizeof(WorkItem) + kKeSlots * 8);
     return b;
 }
 
@@ -283,7 +299,10 @@ int build_work_items(ohpic_ctx *ctx)
             if (!old) continue;
             HIPCHK(ctx, hipStreamSynchronize(ctx->st));
             (void)hipFree(old);
-            ctx->allocs.erase(std::find(ctx->allocs.begin(), ctx->allocs.end(), old));
+            auto it = std::find_if(ctx->allocs.begin(), ctx->allocs.end(),
+                                   [&](const std::pair<void *, int64_t> &a) { return a.first == old; });
+            ctx->bytes -= it->second;
+            ctx->allocs.erase(it);
         }
         ctx->d_items = nullptr;
         ctx->h.ke_partial = nullptr;
@@ -581,7 +600,7 @@ void release(ohpic_ctx *ctx)
     if (ctx->group) (void)group_detach(ctx->group, ctx);
     (void)hipSetDevice(ctx->device);
     if (ctx->st) (void)hipStreamSynchronize(ctx->st);
-    for (void *p : ctx->allocs) (void)hipFree(p);
+    for (auto &a : ctx->allocs) (void)hipFree(a.first);
     ctx->allocs.clear();
     if (ctx->h_flags) (void)hipHostFree(ctx->h_flags);
     if (ctx->h_diag) (void)hipHostFree(ctx->h_diag);
@@ -926,22 +945,38 @@ int group_step(ohpic_group *g, int nsteps)
             GHIP(g, group_enqueue(g));
         }
         GHIP(g, hipStreamSynchronize(g->st));
+        // every member's step is accounted for before a failure is reported:
+        // failed members leave the group, the others have completed the step
+        int first_rc = OHPIC_OK;
+        std::string msg;
+        std::vector<ohpic_ctx *> failed;
         for (int i = 0; i < n; ++i) {
             ohpic_ctx *m = g->members[i];
             rc = absorb_flags(m, g->h_flags + (size_t)i * kFlagSlots);
-            if (rc < 0) return gfail(g, rc, "member %d: %s", i, m->err.c_str());
             if (rc == 1) {
                 // the device kept this member's parity: redo its step alone
-                if ((rc = rerun_as_list(m, kModeFusedList)))
-                    return gfail(g, rc, "member %d: %s", i, m->err.c_str());
-            } else {
+                rc = rerun_as_list(m, kModeFusedList);
+            } else if (rc == 0) {
                 m->par ^= 1;
                 g->dev_par[i] ^= 1;
+            }
+            if (rc < 0) {
+                m->failed = true;
+                failed.push_back(m);
+                if (first_rc == OHPIC_OK) first_rc = rc;
+                char buf[64];
+                snprintf(buf, sizeof buf, "%smember %d: ", msg.empty() ? "" : "; ", i);
+                msg += buf + m->err;
+                continue;
             }
             m->step += 1;
             m->rho_valid = true;
         }
         g->steps += 1;
+        if (first_rc != OHPIC_OK) {
+            for (ohpic_ctx *m : failed) (void)group_detach(g, m);
+            return gfail(g, first_rc, "%s (failed members were detached)", msg.c_str());
+        }
     }
     return OHPIC_OK;
 }
@@ -1001,7 +1036,13 @@ const char *ohpic_last_error(const ohpic_ctx *ctx)
 int ohpic_estimate_footprint(const ohpic_config *cfg, int64_t *bytes)
 {
     if (!cfg || !bytes) return fail(nullptr, OHPIC_EINVAL, "NULL argument");
-    *bytes = estimate_bytes(*cfg);
+    int num_cus = 256;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, cfg->device) == hipSuccess)
+        num_cus = prop.multiProcessorCount;
+    else
+        (void)hipGetLastError();
+    *bytes = estimate_bytes(*cfg, num_cus);
     return OHPIC_OK;
 }
 
@@ -1546,12 +1587,16 @@ int ohpic_step(ohpic_ctx *ctx, int nsteps)
 {
     if (!ctx) return fail(nullptr, OHPIC_EINVAL, "ctx is NULL");
     if (nsteps < 0) return fail(ctx, OHPIC_EINVAL, "nsteps < 0");
+    if (ctx->failed) return fail(ctx, OHPIC_ESTATE, "an earlier step failed: %s", ctx->err.c_str());
     if (!ctx->binned)
         return fail(ctx, OHPIC_ESTATE, "particles pushed but not re-binned: call boundary");
     HIPCHK(ctx, hipSetDevice(ctx->device));
     for (int i = 0; i < nsteps; ++i) {
         int rc = one_step(ctx);
-        if (rc) return rc;
+        if (rc) {
+            ctx->failed = true;
+            return rc;
+        }
     }
     return OHPIC_OK;
 }
@@ -1664,7 +1709,7 @@ int ohpic_fft3d_r2r(int device, int nx, int ny, int nz, int backward, const doub
     const int n[3] = {nx, ny, nz};
     hipStream_t st = nullptr;
     auto cleanup = [&]() {
-        for (void *p : tmp.allocs) (void)hipFree(p);
+        for (auto &a : tmp.allocs) (void)hipFree(a.first);
         tmp.allocs.clear();
         if (st) (void)hipStreamDestroy(st);
     };
@@ -1853,6 +1898,7 @@ int ohpic_group_attach(ohpic_group *g, ohpic_ctx *ctx)
         return gfail(g, OHPIC_EINVAL, "context on device %d, group on device %d", ctx->device,
                      g->device);
     if (!ctx->binned) return gfail(g, OHPIC_ESTATE, "particles pushed but not re-binned");
+    if (ctx->failed) return gfail(g, OHPIC_ESTATE, "context failed an earlier step");
     if (!g->members.empty()) {
         if (ctx->cfg.precision != g->precision || ctx->cfg.weight_order != g->weight_order ||
             ctx->T[0] != g->T[0] || ctx->T[1] != g->T[1] || ctx->T[2] != g->T[2])
@@ -1882,6 +1928,13 @@ int ohpic_group_detach(ohpic_group *g, ohpic_ctx *ctx)
 {
     if (!g || !ctx) return fail(nullptr, OHPIC_EINVAL, "NULL argument");
     return group_detach(g, ctx);
+}
+
+int ohpic_group_contains(const ohpic_group *g, const ohpic_ctx *ctx, int *yes)
+{
+    if (!g || !ctx || !yes) return fail(nullptr, OHPIC_EINVAL, "NULL argument");
+    *yes = ctx->group == g ? 1 : 0;
+    return OHPIC_OK;
 }
 
 int ohpic_group_size(const ohpic_group *g, int *n)

@@ -81,6 +81,11 @@ __device__ __forceinline__ void cic_wf(float r, float &w0, float &w1)
 __device__ __forceinline__ bool recell(float &o, int &c, int n)
 {
     if (o >= 0.0f && o < 1.0f) return true;
+    if (!(fabsf(o) < 1.0e9f)) {
+        // non-finite or absurd displacement: reported by the caller as out of range
+        o = 0.0f;
+        return false;
+    }
     const float f = floorf(o);
     int d = (int)f;
     o = o - f;

@@ -1,0 +1,156 @@
+"""Batched ensemble of independent periodic PIC instances (one group).
+
+Each instance is the reference's two-species plasma (electrons q/m -1, ions
+q/m +1/100, macro charges -1/ppc and +1/ppc so omega_pe = 1 and the box is
+neutral, uniform B0 along z with omega_ce = B0, dt 0.1) on its own non-cubic
+periodic grid.  All instances are advanced together by one ``Group``: per
+step one launch of each field kernel over a table of per-instance FFT jobs
+and one launch of the particle kernel over the work items of every instance,
+replayed from a captured hipGraph.  Tile shape and chunks per tile come from
+``autotune``.
+"""
+from __future__ import annotations
+
+import time
+from dataclasses import dataclass
+from typing import Callable, List, Optional, Sequence, Tuple
+
+import numpy as np
+
+from . import _lib as L
+from .pic import Group, SimConfig, Simulation, autotune
+
+AXES = (16, 24, 32, 48, 64)
+
+
+@dataclass
+class InstanceSpec:
+    n: Tuple[int, int, int]
+    ppc: Tuple[int, int]        # particles per cell, electrons and ions
+    vth_e: float                # ion thermal speed is vth_e / 10
+    omega_ce: float             # electron cyclotron frequency = B0z
+    seed: int                   # PCG32 initstate of the seeded loader
+    dt: float = 0.1
+
+    @property
+    def cells(self) -> int:
+        return self.n[0] * self.n[1] * self.n[2]
+
+    @property
+    def counts(self) -> Tuple[int, int]:
+        return (self.ppc[0] * self.cells, self.ppc[1] * self.cells)
+
+    @property
+    def particles(self) -> int:
+        return sum(self.counts)
+
+    def config(self, tile=(0, 0, 0), chunks: int = 0, precision: int = L.MIXED,
+               track_ids: bool = False, device: int = 0) -> SimConfig:
+        return SimConfig(n=self.n, qm=(-1.0, 0.01),
+                         q_macro=(-1.0 / self.ppc[0], 1.0 / self.ppc[1]),
+                         capacity=self.counts, dt=self.dt, b0=(0.0, 0.0, self.omega_ce),
+                         precision=precision, tile=tuple(tile), chunks=chunks,
+                         track_ids=track_ids, device=device)
+
+    def create(self, tile=(0, 0, 0), chunks: int = 0, precision: int = L.MIXED,
+               track_ids: bool = False, device: int = 0) -> Simulation:
+        sim = Simulation(self.config(tile, chunks, precision, track_ids, device))
+        try:
+            sim.seed_uniform_maxwell(list(self.counts), [self.vth_e, self.vth_e / 10.0],
+                                     seed=(self.seed, 52))
+        except Exception:
+            sim.close()
+            raise
+        return sim
+
+
+def sample_ensemble(count: int = 128, seed: int = 2024, axes: Sequence[int] = AXES,
+                    ppc_range=(8, 64)) -> List[InstanceSpec]:
+    """Instances drawn from one seed: each axis length independently from
+    ``axes``, particles per cell uniform in ppc_range per species, v_th,e
+    uniform in [0.5, 2], omega_ce uniform in [0, 1]."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(count):
+        n = tuple(int(v) for v in rng.choice(np.asarray(axes), 3))
+        ppc = tuple(int(v) for v in rng.integers(ppc_range[0], ppc_range[1] + 1, size=2))
+        vth_e = float(rng.uniform(0.5, 2.0))
+        omega_ce = float(rng.uniform(0.0, 1.0))
+        out.append(InstanceSpec(n=n, ppc=ppc, vth_e=vth_e, omega_ce=omega_ce, seed=1000 + i))
+    return out
+
+
+def run_ensemble(specs: Sequence[InstanceSpec], steps: int, warmup: int = 2,
+                 tune: Optional[dict] = None, tune_sample: int = 16, use_graph: bool = True,
+                 precision: int = L.MIXED,
+                 progress: Optional[Callable[[str], None]] = None) -> dict:
+    """Advance every instance ``steps`` steps as one group and time each group
+    step (host wall clock around the synchronous step)."""
+    say = progress or (lambda s: None)
+    t0 = time.perf_counter()
+    if tune is None:
+        cfgs = [s.config(precision=precision) for s in specs]
+        tune = autotune(cfgs, sample=tune_sample, steps=3)
+        say(f"[ensemble] autotune: tile {tune['tile']}, chunks {tune['chunks']} "
+            f"({tune['candidates']} candidates, {time.perf_counter() - t0:.1f} s)")
+    t1 = time.perf_counter()
+    sims = []
+    grp = Group(use_graph=use_graph)
+    try:
+        for s in specs:
+            sims.append(s.create(tune["tile"], tune["chunks"], precision))
+            grp.attach(sims[-1])
+        say(f"[ensemble] {len(sims)} instances created and seeded in "
+            f"{time.perf_counter() - t1:.1f} s")
+        for _ in range(warmup):
+            grp.step(1)
+        per_step = []
+        for k in range(steps):
+            ts = time.perf_counter()
+            grp.step(1)
+            per_step.append(time.perf_counter() - ts)
+            if (k + 1) % max(1, steps // 5) == 0:
+                say(f"[ensemble] step {k + 1}/{steps}: {per_step[-1] * 1e3:.2f} ms")
+        stats = grp.stats()
+        energy = [sim.diagnostics()["total_energy"] for sim in sims]
+    finally:
+        grp.close()
+        for sim in sims:
+            sim.close()
+    total = float(np.sum(per_step))
+    npart = sum(s.particles for s in specs)
+    a = np.asarray(per_step) * 1e3
+    return {
+        "instances": len(specs), "particles": npart, "steps": steps, "warmup": warmup,
+        "value": npart * steps / total, "unit": "particle-steps/s",
+        "step_ms_p50": float(np.percentile(a, 50)), "step_ms_p99": float(np.percentile(a, 99)),
+        "step_ms_max": float(a.max()), "completion_s": total,
+        "tile": list(tune["tile"]), "chunks": tune["chunks"],
+        "autotune_particle_steps_per_s": tune.get("particle_steps_per_s"),
+        "graph": use_graph, "graph_captures": stats["captures"], "work_items": stats["items"],
+        "energy_finite": bool(np.all(np.isfinite(energy))),
+    }
+
+
+def run_sequential(specs: Sequence[InstanceSpec], steps: int, tune: dict,
+                   precision: int = L.MIXED) -> dict:
+    """The same instances stepped one after another with Simulation.step
+    (no group): the batching gain is value(group) / value(sequential)."""
+    sims = [s.create(tune["tile"], tune["chunks"], precision) for s in specs]
+    try:
+        for sim in sims:
+            sim.step(1)
+            sim.synchronize()
+        t = time.perf_counter()
+        for _ in range(steps):
+            for sim in sims:
+                sim.step(1)
+        for sim in sims:
+            sim.synchronize()
+        total = time.perf_counter() - t
+    finally:
+        for sim in sims:
+            sim.close()
+    npart = sum(s.particles for s in specs)
+    return {"value": npart * steps / total, "steps": steps,
+            "step_ms": total / steps * 1e3}

@@ -290,15 +290,28 @@ class Group:
 
     def detach(self, sim: Simulation) -> None:
         self._chk(self._lib.ohpic_group_detach(self._g, sim._ctx))
-        self.members.remove(sim)
+        if sim in self.members:
+            self.members.remove(sim)
 
     def __len__(self) -> int:
         n = C.c_int()
         self._chk(self._lib.ohpic_group_size(self._g, C.byref(n)))
         return n.value
 
+    def contains(self, sim: Simulation) -> bool:
+        if not sim._ctx:
+            return False
+        y = C.c_int()
+        self._chk(self._lib.ohpic_group_contains(self._g, sim._ctx, C.byref(y)))
+        return bool(y.value)
+
     def step(self, nsteps: int = 1) -> None:
-        self._chk(self._lib.ohpic_group_step(self._g, int(nsteps)))
+        """Advance every member; on a member failure the others complete the
+        step, failed members are detached, and OhpicError is raised."""
+        rc = self._lib.ohpic_group_step(self._g, int(nsteps))
+        if rc != L.OK:
+            self.members = [m for m in self.members if self.contains(m)]
+        self._chk(rc)
 
     def use_graph(self, on: bool) -> None:
         self._chk(self._lib.ohpic_group_use_graph(self._g, 1 if on else 0))

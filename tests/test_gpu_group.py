@@ -181,6 +181,30 @@ def test_group_attach_errors(ohp):
         a.step(1)
 
 
+def test_failed_member_is_detached_and_others_continue(ohp):
+    # FP64: particles that move more than two box lengths in one step are an
+    # error (the reference's wrap is applied twice)
+    with make(ohp, (16, 16, 16), 2, 1, ohp.FP64) as ok, \
+            make(ohp, (16, 16, 16), 2, 2, ohp.FP64, vth=(5000.0, 0.1)) as hot, \
+            make(ohp, (16, 16, 16), 2, 1, ohp.FP64) as ref:
+        ref.step(2)
+        want = snapshot(ohp, ref)
+        with ohp.Group() as grp:
+            grp.attach(ok)
+            grp.attach(hot)
+            with pytest.raises(ohp.OhpicError) as e:
+                grp.step(1)
+            assert "member 1" in str(e.value) and "box lengths" in str(e.value)
+            assert grp.contains(ok) and not grp.contains(hot) and len(grp) == 1
+            grp.step(1)
+            with pytest.raises(ohp.OhpicError) as e:
+                hot.step(1)
+            assert e.value.code == -4
+            with pytest.raises(ohp.OhpicError):
+                grp.attach(hot)
+        assert_same(snapshot(ohp, ok), want, "surviving member")
+
+
 def test_destroying_a_member_detaches_it(ohp):
     a = make(ohp, (16, 16, 16), 2, 1, ohp.MIXED)
     b = make(ohp, (16, 24, 16), 2, 2, ohp.MIXED)
