@@ -582,6 +582,7 @@ int make_axis(ohpic_ctx *ctx, int n, FftAxisDev &ax)
     if (!fft_factorize(n, ax.fac, &ax.nfac))
         return fail(ctx, OHPIC_EINVAL, "axis length %d: unsupported (max %d, prime factors <= %d)",
                     n, kMaxFftLen, kMaxRadix);
+    fft_plan_divisors(ax);
     std::vector<double2> tw(n);
     fft_twiddles(n, tw.data());
     double2 *d = nullptr;

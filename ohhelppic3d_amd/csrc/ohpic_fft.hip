@@ -30,6 +30,11 @@ struct cplx {
     double re, im;
 };
 
+__device__ __forceinline__ int fastdiv(int t, uint32_t m)
+{
+    return m ? (int)__umulhi((uint32_t)t, m) : t;
+}
+
 __device__ __forceinline__ cplx cmul(cplx a, cplx b)
 {
     return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
@@ -93,7 +98,7 @@ __device__ __forceinline__ void butterfly(cplx *a, const double2 *__restrict__ t
 
 template <int SIGN, int P>
 __device__ void fft_stage(cplx *__restrict__ buf, int n, int nlines, int Ls,
-                          const double2 *__restrict__ tw)
+                          const double2 *__restrict__ tw, uint32_t mg_tpl, uint32_t mg_Ls)
 {
     constexpr int KMAX = (kFftLdsCplx / kFftBlock + P - 1) / P;  // tasks per thread
     const int L = Ls * P;
@@ -105,9 +110,9 @@ __device__ void fft_stage(cplx *__restrict__ buf, int n, int nlines, int Ls,
     for (int k = 0; k < KMAX; ++k) {
         const int t = threadIdx.x + k * kFftBlock;
         if (t < total) {
-            const int line = t / tpl;
+            const int line = fastdiv(t, mg_tpl);
             const int i = t - line * tpl;
-            const int row = i / Ls;
+            const int row = fastdiv(i, mg_Ls);
             const int f = i - row * Ls;
             const cplx *s = buf + line * n;
 #pragma unroll
@@ -124,9 +129,9 @@ __device__ void fft_stage(cplx *__restrict__ buf, int n, int nlines, int Ls,
     for (int k = 0; k < KMAX; ++k) {
         const int t = threadIdx.x + k * kFftBlock;
         if (t < total) {
-            const int line = t / tpl;
+            const int line = fastdiv(t, mg_tpl);
             const int i = t - line * tpl;
-            const int row = i / Ls;
+            const int row = fastdiv(i, mg_Ls);
             const int f = i - row * Ls;
             cplx *d = buf + line * n + row * L + f;
 #pragma unroll
@@ -145,88 +150,108 @@ __device__ void fft_lds(cplx *buf, int n, int nlines, const FftAxisDev &ax)
     int Ls = 1;
     for (int s = 0; s < ax.nfac; ++s) {
         const int p = ax.fac[s];
+        const uint32_t mt = ax.mg_tpl[s], ml = ax.mg_Ls[s];
         if (p == 4) {
-            fft_stage<SIGN, 4>(buf, n, nlines, Ls, ax.tw);
+            fft_stage<SIGN, 4>(buf, n, nlines, Ls, ax.tw, mt, ml);
         } else if (p == 2) {
-            fft_stage<SIGN, 2>(buf, n, nlines, Ls, ax.tw);
+            fft_stage<SIGN, 2>(buf, n, nlines, Ls, ax.tw, mt, ml);
         } else if (RS >= 1 && p == 3) {
-            fft_stage<SIGN, 3>(buf, n, nlines, Ls, ax.tw);
+            fft_stage<SIGN, 3>(buf, n, nlines, Ls, ax.tw, mt, ml);
         } else if (RS >= 2) {
             switch (p) {
-            case 5: fft_stage<SIGN, 5>(buf, n, nlines, Ls, ax.tw); break;
-            case 7: fft_stage<SIGN, 7>(buf, n, nlines, Ls, ax.tw); break;
-            case 11: fft_stage<SIGN, 11>(buf, n, nlines, Ls, ax.tw); break;
-            default: fft_stage<SIGN, 13>(buf, n, nlines, Ls, ax.tw); break;
+            case 5: fft_stage<SIGN, 5>(buf, n, nlines, Ls, ax.tw, mt, ml); break;
+            case 7: fft_stage<SIGN, 7>(buf, n, nlines, Ls, ax.tw, mt, ml); break;
+            case 11: fft_stage<SIGN, 11>(buf, n, nlines, Ls, ax.tw, mt, ml); break;
+            default: fft_stage<SIGN, 13>(buf, n, nlines, Ls, ax.tw, mt, ml); break;
             }
         }
         Ls *= p;
     }
 }
 
-// Line addressing for an axis pass.  Returns the base element of line g of
-// group grp (or -1 when that line does not exist) and the element stride.
-struct LineGeom {
+// The G lines of one workgroup: line g (< nvalid) starts at element
+// base + g * gstride and its elements are estride apart.  x: G consecutive
+// rows; y: G consecutive x positions of one z plane; z: G consecutive (x, y)
+// positions.
+struct LineBlock {
     int64_t base;
-    int ix, iy;  // coordinates used by the Poisson division (z pass)
+    int64_t estride;
+    int gstride;
+    int nvalid;
 };
 
 template <int AXIS>
-__device__ __forceinline__ bool line_of(int grp, int g, int G, int nx, int ny, int nz,
-                                        LineGeom &lg)
+__device__ __forceinline__ LineBlock line_block(int grp, int G, int nx, int ny, int nz)
 {
+    LineBlock b;
     if (AXIS == 0) {
-        const int64_t row = (int64_t)grp * G + g;
-        if (row >= (int64_t)ny * nz) return false;
-        lg.base = row * nx;
-        lg.ix = 0;
-        lg.iy = (int)(row % ny);
-        return true;
+        const int64_t row0 = (int64_t)grp * G;
+        b.base = row0 * nx;
+        b.gstride = nx;
+        b.estride = 1;
+        b.nvalid = (int)min<int64_t>(G, (int64_t)ny * nz - row0);
     } else if (AXIS == 1) {
         const int gpp = (nx + G - 1) / G;
         const int iz = grp / gpp;
-        const int ix = (grp - iz * gpp) * G + g;
-        if (ix >= nx || iz >= nz) return false;
-        lg.base = ix + (int64_t)nx * ny * iz;
-        lg.ix = ix;
-        lg.iy = 0;
-        return true;
+        const int ix0 = (grp - iz * gpp) * G;
+        b.base = ix0 + (int64_t)nx * ny * iz;
+        b.gstride = 1;
+        b.estride = nx;
+        b.nvalid = min(G, nx - ix0);
     } else {
-        const int64_t l = (int64_t)grp * G + g;
-        if (l >= (int64_t)nx * ny) return false;
-        lg.base = l;
-        lg.ix = (int)(l % nx);
-        lg.iy = (int)(l / nx);
-        return true;
+        const int64_t l0 = (int64_t)grp * G;
+        b.base = l0;
+        b.gstride = 1;
+        b.estride = (int64_t)nx * ny;
+        b.nvalid = (int)min<int64_t>(G, (int64_t)nx * ny - l0);
     }
+    return b;
 }
 
+// element e of the group -> (line g, position j): x lines are read row by
+// row (g = e / n), y and z lines across the G lines first (j = e / G) so a
+// wavefront touches G consecutive doubles per position
 template <int AXIS>
-__device__ __forceinline__ void elem_of(int e, int n, int G, int &g, int &j)
+__device__ __forceinline__ void elem_of(int e, int n, int G, const FftAxisDev &ax, int &g, int &j)
 {
     if (AXIS == 0) {
-        g = e / n;
+        g = fastdiv(e, ax.mg_n);
         j = e - g * n;
     } else {
-        j = e / G;
+        j = fastdiv(e, ax.mg_G);
         g = e - j * G;
     }
 }
 
-// Load G real lines into LDS packed as complex (line pair -> re/im).
+constexpr int kLineElems = 2 * kFftLdsCplx / kFftBlock;  // real elements per thread
+
+// Load G real lines into LDS packed as complex (line pair -> re/im).  Every
+// thread issues all of its (up to kLineElems) global loads before the first
+// LDS write, so each lane keeps that many reads in flight.
 template <int AXIS>
-__device__ void load_lines(const double *__restrict__ src, cplx *buf, int grp, int G, int n,
-                           int64_t estride, int nx, int ny, int nz)
+__device__ void load_lines(const double *__restrict__ src, cplx *buf, const LineBlock &lb,
+                           int G, int n, const FftAxisDev &ax)
 {
     double *bd = reinterpret_cast<double *>(buf);
     const int total = G * n;
-    for (int e = threadIdx.x; e < total; e += blockDim.x) {
-        int g, j;
-        elem_of<AXIS>(e, n, G, g, j);
-        LineGeom lg;
-        double v = 0.0;
-        if (line_of<AXIS>(grp, g, G, nx, ny, nz, lg)) v = src[lg.base + (int64_t)j * estride];
-        bd[((g >> 1) * n + j) * 2 + (g & 1)] = v;
+    double v[kLineElems];
+    int at[kLineElems];
+#pragma unroll
+    for (int k = 0; k < kLineElems; ++k) {
+        const int e = threadIdx.x + k * kFftBlock;
+        v[k] = 0.0;
+        at[k] = -1;
+        if (e < total) {
+            int g, j;
+            elem_of<AXIS>(e, n, G, ax, g, j);
+            if (g < lb.nvalid)
+                v[k] = src[lb.base + (int64_t)g * lb.gstride + (int64_t)j * lb.estride];
+            at[k] = ((g >> 1) * n + j) * 2 + (g & 1);
+        }
     }
+#pragma unroll
+    for (int k = 0; k < kLineElems; ++k)
+        if (at[k] >= 0) bd[at[k]] = v[k];
 }
 
 // Halfcomplex value at position m of line (pair c, which) from the complex
@@ -272,7 +297,7 @@ __device__ __forceinline__ cplx packed_from_hc(const cplx *H, int n, int c, int 
 constexpr int kElemMax = kFftLdsCplx / kFftBlock;  // LDS elements per thread
 
 // In place: buf <- packed spectrum built from the halfcomplex pairs in buf.
-__device__ void build_packed_inplace(cplx *buf, int n, int nlines)
+__device__ void build_packed_inplace(cplx *buf, int n, int nlines, uint32_t mg_n)
 {
     const int total = n * nlines;
     cplx z[kElemMax];
@@ -280,7 +305,7 @@ __device__ void build_packed_inplace(cplx *buf, int n, int nlines)
     for (int k = 0; k < kElemMax; ++k) {
         const int t = threadIdx.x + k * kFftBlock;
         if (t < total) {
-            const int c = t / n;
+            const int c = fastdiv(t, mg_n);
             z[k] = packed_from_hc(buf, n, c, t - c * n);
         }
     }
@@ -301,6 +326,7 @@ template <int KIND, int AXIS, int RS, bool FROM_SRC, bool SCALE>
 __global__ void __launch_bounds__(kFftBlock) k_fft_pass(const FftJob *__restrict__ jobs)
 {
     __shared__ cplx buf[kFftLdsCplx];
+    __shared__ double sLamXY[KIND == kFftPoisson ? 64 : 1];
     const FftJob &J = jobs[blockIdx.y];
     const int nx = J.n[0], ny = J.n[1], nz = J.n[2];
     const FftAxisDev &ax = J.ax[AXIS];
@@ -311,38 +337,47 @@ __global__ void __launch_bounds__(kFftBlock) k_fft_pass(const FftJob *__restrict
     const double *__restrict__ src = FROM_SRC ? J.src : J.data;
     double *__restrict__ dst = J.data;
     const double out_scale = SCALE ? J.inv_n : 1.0;
-    const double *__restrict__ lamx = J.lam[0];
-    const double *__restrict__ lamy = J.lam[1];
-    const double *__restrict__ lamz = J.lam[2];
     const int nlines = (G + 1) / 2;
-    const int64_t estride = AXIS == 0 ? 1 : (AXIS == 1 ? (int64_t)nx : (int64_t)nx * ny);
+    const LineBlock lb = line_block<AXIS>(grp, G, nx, ny, nz);
 
-    load_lines<AXIS>(src, buf, grp, G, n, estride, nx, ny, nz);
+    if (KIND == kFftPoisson && (int)threadIdx.x < G) {
+        // lam_x(ix) + lam_y(iy) of z line g; the DC line (ix = iy = 0) is marked
+        // by -1 (its k_z = 0 mode is set to zero)
+        const int g = threadIdx.x;
+        double v = 0.0;
+        if (g < lb.nvalid) {
+            const int l = (int)lb.base + g;
+            const int iy = l / nx;
+            const int ix = l - iy * nx;
+            v = (l == 0) ? -1.0 : J.lam[0][ix] + J.lam[1][iy];
+        }
+        sLamXY[g] = v;
+    }
+    load_lines<AXIS>(src, buf, lb, G, n, ax);
     __syncthreads();
 
     if (KIND == kFftR2HC || KIND == kFftPoisson) {
         fft_lds<-1, RS>(buf, n, nlines, ax);
         if (KIND == kFftPoisson) {
             // halfcomplex pairs divided by the Green's-function denominator
+            const double *__restrict__ lamz = J.lam[2];
             const int total = n * nlines;
             cplx h[kElemMax];
 #pragma unroll
             for (int k = 0; k < kElemMax; ++k) {
                 const int t = threadIdx.x + k * kFftBlock;
                 if (t < total) {
-                    const int c = t / n;
+                    const int c = fastdiv(t, ax.mg_n);
                     const int m = t - c * n;
                     double va = hc_value(buf, n, c, 0, m);
                     double vb = hc_value(buf, n, c, 1, m);
-                    LineGeom la, lb;
-                    if (line_of<AXIS>(grp, 2 * c, G, nx, ny, nz, la)) {
-                        const double lam = lamx[la.ix] + lamy[la.iy] + lamz[m];
-                        va = (la.ix == 0 && la.iy == 0 && m == 0) ? 0.0 : va / lam;
-                    }
-                    if (line_of<AXIS>(grp, 2 * c + 1, G, nx, ny, nz, lb)) {
-                        const double lam = lamx[lb.ix] + lamy[lb.iy] + lamz[m];
-                        vb = (lb.ix == 0 && lb.iy == 0 && m == 0) ? 0.0 : vb / lam;
-                    }
+                    const double la = sLamXY[2 * c];
+                    const double lbv = sLamXY[2 * c + 1];
+                    const double lz = lamz[m];
+                    if (2 * c < lb.nvalid)
+                        va = (la < 0.0 && m == 0) ? 0.0 : va / ((la < 0.0 ? 0.0 : la) + lz);
+                    if (2 * c + 1 < lb.nvalid)
+                        vb = (lbv < 0.0 && m == 0) ? 0.0 : vb / ((lbv < 0.0 ? 0.0 : lbv) + lz);
                     h[k] = {va, vb};
                 }
             }
@@ -353,20 +388,22 @@ __global__ void __launch_bounds__(kFftBlock) k_fft_pass(const FftJob *__restrict
                 if (t < total) buf[t] = h[k];
             }
             __syncthreads();
-            build_packed_inplace(buf, n, nlines);
+            build_packed_inplace(buf, n, nlines, ax.mg_n);
             fft_lds<+1, RS>(buf, n, nlines, ax);
         }
     } else {
-        build_packed_inplace(buf, n, nlines);
+        build_packed_inplace(buf, n, nlines, ax.mg_n);
         fft_lds<+1, RS>(buf, n, nlines, ax);
     }
 
     const int total = G * n;
-    for (int e = threadIdx.x; e < total; e += blockDim.x) {
+#pragma unroll
+    for (int k = 0; k < kLineElems; ++k) {
+        const int e = threadIdx.x + k * kFftBlock;
+        if (e >= total) continue;
         int g, j;
-        elem_of<AXIS>(e, n, G, g, j);
-        LineGeom lg;
-        if (!line_of<AXIS>(grp, g, G, nx, ny, nz, lg)) continue;
+        elem_of<AXIS>(e, n, G, ax, g, j);
+        if (g >= lb.nvalid) continue;
         double v;
         if (KIND == kFftR2HC) {
             v = hc_value(buf, n, g >> 1, g & 1, j);
@@ -374,11 +411,24 @@ __global__ void __launch_bounds__(kFftBlock) k_fft_pass(const FftJob *__restrict
             const cplx z = buf[(g >> 1) * n + j];
             v = (g & 1) ? z.im : z.re;
         }
-        dst[lg.base + (int64_t)j * estride] = v * out_scale;
+        dst[lb.base + (int64_t)g * lb.gstride + (int64_t)j * lb.estride] = v * out_scale;
     }
 }
 
 }  // namespace
+
+void fft_plan_divisors(FftAxisDev &ax)
+{
+    ax.mg_n = fastdiv_magic((uint32_t)ax.n);
+    ax.mg_G = fastdiv_magic((uint32_t)fft_group_lines(ax.n));
+    int Ls = 1;
+    for (int s = 0; s < kMaxFactors; ++s) {
+        const int p = s < ax.nfac ? ax.fac[s] : 1;
+        ax.mg_tpl[s] = fastdiv_magic((uint32_t)(ax.n / p));
+        ax.mg_Ls[s] = fastdiv_magic((uint32_t)Ls);
+        Ls *= p;
+    }
+}
 
 int fft_radix_set(const FftAxisDev &ax)
 {

@@ -17,16 +17,33 @@ constexpr int kMaxFftLen = 2048;
 
 enum FftKind : int { kFftR2HC = 0, kFftHC2R = 1, kFftPoisson = 2 };
 
+// q = t / d for 0 <= t < 2^16 and 1 <= d <= 2^16 as one high multiply by
+// m = ceil(2^32 / d) (m = 0 encodes d = 1): with t = q d + r, t m / 2^32 =
+// q + r/d + t e/(d 2^32) where m d = 2^32 + e, e < d, and t d < 2^32 keeps the
+// last term below 1/d, so the floor is exact.
+__host__ __device__ inline uint32_t fastdiv_magic(uint32_t d)
+{
+    return d <= 1 ? 0u : (uint32_t)((0x100000000ull + d - 1) / d);
+}
+
 struct FftAxisDev {
     int n;
     int nfac;
     int fac[kMaxFactors];
     const double2 *tw;  // tw[k] = exp(-2 pi i k / n), k in [0, n)
+    // divisors of the LDS passes as fastdiv multipliers (fft_plan_divisors)
+    uint32_t mg_n;                 // n
+    uint32_t mg_G;                 // fft_group_lines(n)
+    uint32_t mg_tpl[kMaxFactors];  // n / fac[s]: butterflies per line in stage s
+    uint32_t mg_Ls[kMaxFactors];   // fac[0] * ... * fac[s-1]
 };
 
 // Factorises n (radix 4 first, then 2, 3, 5, ...); false if a prime factor
 // exceeds kMaxRadix or n is out of range.
 bool fft_factorize(int n, int *fac, int *nfac);
+
+// Fills the fastdiv multipliers of a factorised axis plan.
+void fft_plan_divisors(FftAxisDev &ax);
 
 // Host-side twiddle table exp(-2 pi i k/n) in long double, rounded to double.
 void fft_twiddles(int n, double2 *tw);

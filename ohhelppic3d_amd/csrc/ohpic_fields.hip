@@ -126,15 +126,16 @@ __global__ void __launch_bounds__(256) k_reduce_e_b(const InstDev *__restrict__ 
     }
 }
 
-// blockIdx.x = instance: diag[kDiagKeOld + k] := sum over items of ke_partial
+// blockIdx.y = instance, blockIdx.x = slot k: diag[kDiagKeOld + k] := sum
+// over the instance's items of ke_partial[item][k] (slots of absent species
+// sum zeros)
 __global__ void __launch_bounds__(256) k_reduce_ke_b(const InstDev *__restrict__ insts)
 {
     __shared__ double red[256];
-    const InstDev *I = insts + blockIdx.x;
-    for (int k = 0; k < kKeSlots; ++k) {
-        const double v = block_ordered_sum(I->ke_partial, I->nitems, kKeSlots, k, red);
-        if (threadIdx.x == 0) I->diag[kDiagKeOld + k] = v;
-    }
+    const InstDev *I = insts + blockIdx.y;
+    const int k = blockIdx.x;
+    const double v = block_ordered_sum(I->ke_partial, I->nitems, kKeSlots, k, red);
+    if (threadIdx.x == 0) I->diag[kDiagKeOld + k] = v;
 }
 
 __global__ void k_rho_from_acc_b(const InstDev *__restrict__ insts, const int *__restrict__ parity)
@@ -309,7 +310,7 @@ hipError_t launch_prepare_next_b(const InstDev *insts, const int *parity, int ni
 hipError_t launch_reduce_ke_b(const InstDev *insts, int ninst, hipStream_t st)
 {
     if (ninst <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_reduce_ke_b, dim3(ninst), dim3(256), 0, st, insts);
+    hipLaunchKernelGGL(k_reduce_ke_b, dim3(kKeSlots, ninst), dim3(256), 0, st, insts);
     return hipGetLastError();
 }
 
