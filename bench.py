@@ -97,7 +97,9 @@ def run_gpu(args, workload=None):
     push_ms, push_n = kt.get("push_fused", (0.0, 0))
     solve_ms, solve_n = kt.get("field_solve", (0.0, 0))
     push_avg = push_ms / max(push_n, 1)
-    sp_bytes = 48 if wl["precision"] == 0 else 28
+    # particle state read and written per step: FP64 x, v (48 B); mixed
+    # position words + FP32 v (24 B, the cell is implicit in the bin)
+    sp_bytes = 48 if wl["precision"] == 0 else 24
     alg_bytes = 2 * sp_bytes * total_p + 24 * ng + 8 * ng
     achieved = alg_bytes / (push_avg * 1e-3) / 1e9 if push_avg > 0 else 0.0
     traffic = None

@@ -615,45 +615,56 @@ void release(ohpic_ctx *ctx)
 
 }  // namespace
 
-// FP64 positions/velocities <-> the mixed-precision storage (int32 cell
-// packing ix | iy << 10 | iz << 20, FP32 offsets and velocities).  The cell
-// is the FP64 int() truncation and the offset is clamped below 1, so a
-// particle never changes cell in the conversion.  Positions outside [0, n)
-// get an invalid cell so the binning kernel rejects them.
-void to_mixed(ohpic_ctx *ctx, int64_t n, const double *const src[6], std::vector<int32_t> &cell,
-              std::vector<float> comp[6])
+// FP64 positions/velocities <-> the mixed-precision list form: per particle
+// the tile index, per axis a word holding the cell relative to the tile
+// origin (top 8 bits) and the in-cell offset as a 24-bit fraction, and FP32
+// velocities.  The cell is the FP64 int() truncation and an offset that
+// rounds up to 1 is clamped below it, so a particle never changes cell in the
+// conversion.  Positions outside [0, n) get tile -1, which the binning kernel
+// rejects.
+void to_mixed(ohpic_ctx *ctx, int64_t n, const double *const src[6], std::vector<int32_t> &tile,
+              std::vector<uint32_t> word[3], std::vector<float> vel[3])
 {
-    cell.resize(n);
-    for (int k = 0; k < 6; ++k) comp[k].resize(n);
+    tile.resize(n);
+    for (int a = 0; a < 3; ++a) {
+        word[a].resize(n);
+        vel[a].resize(n);
+    }
     for (int64_t i = 0; i < n; ++i) {
-        int c[3];
+        int tc[3];
         bool bad = false;
         for (int a = 0; a < 3; ++a) {
             const double p = src[a][i];
             const int na = ctx->n[a];
             if (!(p >= 0.0 && p < (double)na)) {
                 bad = true;
-                c[a] = 0;
-                comp[a][i] = 0.0f;
+                tc[a] = 0;
+                word[a][i] = 0u;
                 continue;
             }
-            c[a] = (int)p;
-            // keep the FP64 cell; clamp an offset that rounds up to 1
-            comp[a][i] = std::min((float)(p - (double)c[a]), 0.99999994f);
+            const int c = (int)p;
+            const double f = std::min(std::rint((p - (double)c) * 16777216.0), 16777215.0);
+            tc[a] = c / ctx->T[a];
+            word[a][i] = ((uint32_t)(c - tc[a] * ctx->T[a]) << 24) | (uint32_t)f;
         }
-        cell[i] = bad ? -1 : (c[0] | (c[1] << 10) | (c[2] << 20));
-        for (int a = 0; a < 3; ++a) comp[3 + a][i] = (float)src[3 + a][i];
+        tile[i] = bad ? -1 : tc[0] + ctx->nt[0] * (tc[1] + ctx->nt[1] * tc[2]);
+        for (int a = 0; a < 3; ++a) vel[a][i] = (float)src[3 + a][i];
     }
 }
 
-void from_mixed(int64_t n, const std::vector<int32_t> &cell, const std::vector<float> comp[6],
+void from_mixed(const ohpic_ctx *ctx, int64_t n, const std::vector<int32_t> &tile,
+                const std::vector<uint32_t> word[3], const std::vector<float> vel[3],
                 double *const dst[6])
 {
     for (int64_t i = 0; i < n; ++i) {
-        const int c[3] = {cell[i] & 1023, (cell[i] >> 10) & 1023, cell[i] >> 20};
+        const int t = tile[i];
+        const int tc[3] = {t % ctx->nt[0], (t / ctx->nt[0]) % ctx->nt[1],
+                           t / (ctx->nt[0] * ctx->nt[1])};
         for (int a = 0; a < 3; ++a) {
-            if (dst[a]) dst[a][i] = (double)c[a] + (double)comp[a][i];
-            if (dst[3 + a]) dst[3 + a][i] = (double)comp[3 + a][i];
+            const uint32_t w = word[a][i];
+            const int c = tc[a] * ctx->T[a] + ((int32_t)w >> 24);
+            if (dst[a]) dst[a][i] = (double)c + (double)(w & 0xffffffu) * 0x1p-24;
+            if (dst[3 + a]) dst[3 + a][i] = (double)vel[a][i];
         }
     }
 }
@@ -1274,15 +1285,20 @@ int ohpic_load_particles(ohpic_ctx *ctx, int ispec, int64_t n, const double *x, 
     SpecDev &D = ctx->h.sp[ispec];
     const double *src[6] = {x, y, z, vx, vy, vz};
     if (ctx->cfg.precision == OHPIC_MIXED) {
-        std::vector<int32_t> cell;
-        std::vector<float> comp[6];
-        to_mixed(ctx, n, src, cell, comp);
-        if (n > 0)
-            HIPCHK(ctx, hipMemcpyAsync(D.buf[stg][0], cell.data(), n * sizeof(int32_t),
+        std::vector<int32_t> tile;
+        std::vector<uint32_t> word[3];
+        std::vector<float> vel[3];
+        to_mixed(ctx, n, src, tile, word, vel);
+        if (n > 0) {
+            HIPCHK(ctx, hipMemcpyAsync(D.buf[stg][0], tile.data(), n * sizeof(int32_t),
                                        hipMemcpyHostToDevice, ctx->st));
-        for (int k = 0; k < 6 && n > 0; ++k)
-            HIPCHK(ctx, hipMemcpyAsync(D.buf[stg][k + 1], comp[k].data(), n * sizeof(float),
-                                       hipMemcpyHostToDevice, ctx->st));
+            for (int a = 0; a < 3; ++a) {
+                HIPCHK(ctx, hipMemcpyAsync(D.buf[stg][1 + a], word[a].data(),
+                                           n * sizeof(uint32_t), hipMemcpyHostToDevice, ctx->st));
+                HIPCHK(ctx, hipMemcpyAsync(D.buf[stg][4 + a], vel[a].data(), n * sizeof(float),
+                                           hipMemcpyHostToDevice, ctx->st));
+            }
+        }
         HIPCHK(ctx, hipStreamSynchronize(ctx->st));
     } else {
         for (int k = 0; k < 6 && n > 0; ++k)
@@ -1336,19 +1352,24 @@ int ohpic_store_particles(ohpic_ctx *ctx, int ispec, int64_t cap, int64_t *n, do
     SpecDev &D = ctx->h.sp[ispec];
     double *dst[6] = {x, y, z, vx, vy, vz};
     if (ctx->cfg.precision == OHPIC_MIXED) {
-        std::vector<int32_t> cell(cnt);
-        std::vector<float> comp[6];
+        std::vector<int32_t> tile(cnt);
+        std::vector<uint32_t> word[3];
+        std::vector<float> vel[3];
         if (cnt > 0)
-            HIPCHK(ctx, hipMemcpyAsync(cell.data(), D.buf[stg][0], cnt * sizeof(int32_t),
+            HIPCHK(ctx, hipMemcpyAsync(tile.data(), D.buf[stg][0], cnt * sizeof(int32_t),
                                        hipMemcpyDeviceToHost, ctx->st));
-        for (int k = 0; k < 6; ++k) {
-            comp[k].resize(cnt);
-            if (cnt > 0)
-                HIPCHK(ctx, hipMemcpyAsync(comp[k].data(), D.buf[stg][k + 1], cnt * sizeof(float),
+        for (int a = 0; a < 3; ++a) {
+            word[a].resize(cnt);
+            vel[a].resize(cnt);
+            if (cnt > 0) {
+                HIPCHK(ctx, hipMemcpyAsync(word[a].data(), D.buf[stg][1 + a],
+                                           cnt * sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->st));
+                HIPCHK(ctx, hipMemcpyAsync(vel[a].data(), D.buf[stg][4 + a], cnt * sizeof(float),
                                            hipMemcpyDeviceToHost, ctx->st));
+            }
         }
         HIPCHK(ctx, hipStreamSynchronize(ctx->st));
-        from_mixed(cnt, cell, comp, dst);
+        from_mixed(ctx, cnt, tile, word, vel, dst);
     } else {
         for (int k = 0; k < 6 && cnt > 0; ++k)
             if (dst[k])
@@ -1456,7 +1477,7 @@ int ohpic_seed_uniform_maxwell(ohpic_ctx *ctx, uint64_t initstate, uint64_t init
         // one random_fix draw, then 7 draws per particle
         if (ctx->cfg.precision == OHPIC_MIXED)
             HIPCHK(ctx, launch_load_uniform_mx(ctx->d_inst, 0, s, stg, r0, offset + 1, np[s],
-                                               vth[s], ctx->st));
+                                               vth[s], ctx->T, ctx->st));
         else
             HIPCHK(ctx, launch_load_uniform(r0, offset + 1, np[s], ctx->n[0], ctx->n[1],
                                             ctx->n[2], vth[s], (double *)D.buf[stg][0],
@@ -1496,7 +1517,8 @@ int ohpic_seed_hump_maxwell(ohpic_ctx *ctx, uint64_t initstate, uint64_t initseq
         HIPCHK(ctx, launch_fill_u64(ctx->d_cursor, 1, 0ull, ctx->st));
         if (ctx->cfg.precision == OHPIC_MIXED)
             HIPCHK(ctx, launch_load_hump_mx(ctx->d_inst, 0, s, stg, r0, (uint64_t)s << 40, budget,
-                                            np[s], amp, sigma, vth[s], ctx->d_cursor, ctx->st));
+                                            np[s], amp, sigma, vth[s], ctx->d_cursor, ctx->T,
+                                            ctx->st));
         else
             HIPCHK(ctx, launch_load_hump(r0, (uint64_t)s << 40, budget, np[s], ctx->n[0],
                                          ctx->n[1], ctx->n[2], amp, sigma, vth[s],
@@ -1801,7 +1823,7 @@ int ohpic_cell_counts(ohpic_ctx *ctx, int ispec, int32_t *counts)
         if (ispec >= 0 && s != ispec) continue;
         if (ctx->cfg.precision == OHPIC_MIXED)
             HIPCHK(ctx, launch_cell_counts_mx(ctx->d_inst, 0, s, ctx->par, ctx->ntiles,
-                                              ctx->d_cellcounts, ctx->st));
+                                              ctx->d_cellcounts, ctx->T, ctx->st));
         else
             HIPCHK(ctx, launch_cell_counts(ctx->d_inst, 0, s, ctx->par, ctx->ntiles,
                                            ctx->d_cellcounts, ctx->st));

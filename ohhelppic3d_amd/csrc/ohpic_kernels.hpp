@@ -54,7 +54,7 @@ hipError_t launch_bin_scatter_mx(const InstDev *insts, int inst, int s, int lbuf
 hipError_t launch_bins_to_list_mx(const InstDev *insts, int inst, int s, int src, int dst,
                                   int ntiles, const int64_t *list_off, hipStream_t st);
 hipError_t launch_cell_counts_mx(const InstDev *insts, int inst, int s, int par, int ntiles,
-                                 int32_t *counts, hipStream_t st);
+                                 int32_t *counts, const int T[3], hipStream_t st);
 
 // fields (ohpic_fields.hip)
 hipError_t launch_fill_u64(unsigned long long *p, int64_t n, unsigned long long v,
@@ -85,11 +85,12 @@ hipError_t launch_load_uniform(Pcg32Dev rng0, uint64_t offset, int64_t np, int n
                                double *vy, double *vz, int64_t *pid, int64_t pid0,
                                hipStream_t st);
 hipError_t launch_load_uniform_mx(const InstDev *insts, int inst, int s, int b, Pcg32Dev rng0,
-                                  uint64_t offset, int64_t np, double vth, hipStream_t st);
+                                  uint64_t offset, int64_t np, double vth, const int T[3],
+                                  hipStream_t st);
 hipError_t launch_load_hump_mx(const InstDev *insts, int inst, int s, int b, Pcg32Dev rng0,
                                uint64_t offset, uint64_t budget, int64_t np, double amp,
                                double sigma, double vth, unsigned long long *fail,
-                               hipStream_t st);
+                               const int T[3], hipStream_t st);
 hipError_t launch_load_hump(Pcg32Dev rng0, uint64_t offset, uint64_t budget, int64_t np, int nx,
                             int ny, int nz, double amp, double sigma, double vth, double *x,
                             double *y, double *z, double *vx, double *vy, double *vz,

@@ -36,25 +36,25 @@ struct P64 {
 __device__ __forceinline__ void store_p64(const SpecDev &S, int nbuf, int64_t o, const P64 &p,
                                           bool ids)
 {
-    ((double *)S.buf[nbuf][0])[o] = p.x;
-    ((double *)S.buf[nbuf][1])[o] = p.y;
-    ((double *)S.buf[nbuf][2])[o] = p.z;
-    ((double *)S.buf[nbuf][3])[o] = p.vx;
-    ((double *)S.buf[nbuf][4])[o] = p.vy;
-    ((double *)S.buf[nbuf][5])[o] = p.vz;
-    if (ids) S.pid[nbuf][o] = p.id;
+    gstore((double *)S.buf[nbuf][0], o, p.x);
+    gstore((double *)S.buf[nbuf][1], o, p.y);
+    gstore((double *)S.buf[nbuf][2], o, p.z);
+    gstore((double *)S.buf[nbuf][3], o, p.vx);
+    gstore((double *)S.buf[nbuf][4], o, p.vy);
+    gstore((double *)S.buf[nbuf][5], o, p.vz);
+    if (ids) gstore(S.pid[nbuf], o, p.id);
 }
 
 __device__ __forceinline__ P64 load_p64(const SpecDev &S, int b, int64_t o, bool ids)
 {
     P64 q;
-    q.x = ((const double *)S.buf[b][0])[o];
-    q.y = ((const double *)S.buf[b][1])[o];
-    q.z = ((const double *)S.buf[b][2])[o];
-    q.vx = ((const double *)S.buf[b][3])[o];
-    q.vy = ((const double *)S.buf[b][4])[o];
-    q.vz = ((const double *)S.buf[b][5])[o];
-    q.id = ids ? S.pid[b][o] : 0;
+    q.x = gload((const double *)S.buf[b][0], o);
+    q.y = gload((const double *)S.buf[b][1], o);
+    q.z = gload((const double *)S.buf[b][2], o);
+    q.vx = gload((const double *)S.buf[b][3], o);
+    q.vy = gload((const double *)S.buf[b][4], o);
+    q.vz = gload((const double *)S.buf[b][5], o);
+    q.id = ids ? gload(S.pid[b], o) : 0;
     return q;
 }
 
@@ -62,13 +62,13 @@ __device__ __forceinline__ P64 load_p64(const SpecDev &S, int b, int64_t o, bool
 __device__ __forceinline__ void route_back(const SpecDev &S, int nbuf, int t, const P64 &q,
                                            bool ids, unsigned long long *flags)
 {
-    const unsigned long long old = atomicAdd(&S.count[nbuf][t], 1ull << 32);
+    const unsigned long long old = gatomic_add(&S.count[nbuf][t], 1ull << 32);
     const int f = bin_front(old), b = bin_back(old);
-    const int cap = S.bin_cap[t];
+    const int cap = gload(S.bin_cap, t);
     if (b + f < cap)
-        store_p64(S, nbuf, S.bin_base[t] + (cap - 1 - b), q, ids);
+        store_p64(S, nbuf, gload(S.bin_base, t) + (cap - 1 - b), q, ids);
     else
-        atomicAdd(&flags[kFlagOverflow], 1ull);
+        gatomic_add(&flags[kFlagOverflow], 1ull);
 }
 
 struct LeaveBuf {
@@ -142,9 +142,9 @@ __global__ void __launch_bounds__(kBlock)
             const int lx = l % EX, ly = (l / EX) % EY, lz = l / (EX * EY);
             const int gx = (lox + lx) % nx, gy = (loy + ly) % ny, gz = (loz + lz) % nz;
             const int64_t g = gx + (int64_t)nx * (gy + (int64_t)ny * gz);
-            sE[l] = ex[g];
-            sE[NE + l] = ey[g];
-            sE[2 * NE + l] = ez[g];
+            sE[l] = gload(ex, g);
+            sE[NE + l] = gload(ey, g);
+            sE[2 * NE + l] = gload(ez, g);
         }
     }
     if (DEP)
@@ -162,12 +162,12 @@ __global__ void __launch_bounds__(kBlock)
 
     for (int s = 0; s < I->nspecies; ++s) {
         const SpecDev &S = I->sp[s];
-        const int cap = S.bin_cap[tile];
-        const unsigned long long word = S.count[par][tile];
+        const int cap = gload(S.bin_cap, tile);
+        const unsigned long long word = gload(S.count[par], tile);
         const int F = min(bin_front(word), cap);
         const int B = min(bin_back(word), cap - F);
         const int cnt = F + B;
-        const int64_t base = S.bin_base[tile];
+        const int64_t base = gload(S.bin_base, tile);
         int per = (cnt + it.nchunks - 1) / it.nchunks;
         per = (per + 63) & ~63;
         const int start = it.chunk * per;
@@ -178,7 +178,7 @@ __global__ void __launch_bounds__(kBlock)
                 if (!LIST && excl && cnt > 0) {
                     // reserve the whole front run: arrivals from neighbours stay behind it
                     const unsigned long long old =
-                        atomicAdd(&S.count[nbuf][tile], (unsigned long long)cnt);
+                        gatomic_add(&S.count[nbuf][tile], (unsigned long long)cnt);
                     lim = cap - bin_back(old);
                 }
                 sFront = 0;
@@ -258,7 +258,7 @@ __global__ void __launch_bounds__(kBlock)
                 q.z = apply_periodic(apply_periodic(q.z, nz), nz);
                 if (!(q.x >= 0.0 && q.x < nx && q.y >= 0.0 && q.y < ny && q.z >= 0.0 &&
                       q.z < nz)) {
-                    atomicAdd(&flags[kFlagRange], 1ull);
+                    gatomic_add(&flags[kFlagRange], 1ull);
                     q.x = fmin(fmax(q.x - floor(q.x / nx) * nx, 0.0), nx - 1e-9);
                     q.y = fmin(fmax(q.y - floor(q.y / ny) * ny, 0.0), ny - 1e-9);
                     q.z = fmin(fmax(q.z - floor(q.z / nz) * nz, 0.0), nz - 1e-9);
@@ -289,7 +289,7 @@ __global__ void __launch_bounds__(kBlock)
                                           (unsigned long long)v);
                             }
                 } else {
-                    atomicAdd(&flags[kFlagFast], 1ull);
+                    gatomic_add(&flags[kFlagFast], 1ull);
 #pragma unroll
                     for (int i = 0; i < 2; ++i)
 #pragma unroll
@@ -299,7 +299,7 @@ __global__ void __launch_bounds__(kBlock)
                                 const int gx = (ix + i) % nx, gy = (iy + j) % ny,
                                           gz = (iz + k) % nz;
                                 const long long v = fix_round(wx[i] * wy[j] * wz[k], qfix);
-                                atomicAdd(&racc[gx + (int64_t)nx * (gy + (int64_t)ny * gz)],
+                                gatomic_add(&racc[gx + (int64_t)nx * (gy + (int64_t)ny * gz)],
                                           (unsigned long long)v);
                             }
                 }
@@ -311,7 +311,7 @@ __global__ void __launch_bounds__(kBlock)
                     const int leader = __ffsll((unsigned long long)m) - 1;
                     unsigned long long b = 0;
                     if (lane_id() == leader)
-                        b = atomicAdd(&S.count[nbuf][0], (unsigned long long)__popcll(m));
+                        b = gatomic_add(&S.count[nbuf][0], (unsigned long long)__popcll(m));
                     b = __shfl(b, leader, 64);
                     if (active) store_p64(S, nbuf, (int64_t)b + __popcll(m & lanemask_lt), q, ids);
                 }
@@ -328,7 +328,7 @@ __global__ void __launch_bounds__(kBlock)
                         lim = sFrontLimit;
                     } else if (lane_id() == leader) {
                         const unsigned long long old =
-                            atomicAdd(&S.count[nbuf][tile], (unsigned long long)nst);
+                            gatomic_add(&S.count[nbuf][tile], (unsigned long long)nst);
                         b = bin_front(old);
                         lim = cap - bin_back(old);
                     }
@@ -339,7 +339,7 @@ __global__ void __launch_bounds__(kBlock)
                         if (slot < lim)
                             store_p64(S, nbuf, base + slot, q, ids);
                         else
-                            atomicAdd(&flags[kFlagOverflow], 1ull);
+                            gatomic_add(&flags[kFlagOverflow], 1ull);
                     }
                 }
                 // particles that left: per-wave LDS buffer, routed in batches
@@ -379,7 +379,7 @@ __global__ void __launch_bounds__(kBlock)
             if (tid < kWaves) sLcnt[tid] = 0;
             if (tid == 0 && excl && cnt > sFront) {
                 // release the reserved but unused part of the front run
-                atomicAdd(&S.count[nbuf][tile],
+                gatomic_add(&S.count[nbuf][tile],
                           (unsigned long long)(-(long long)(cnt - sFront)));
             }
             __syncthreads();
@@ -388,8 +388,9 @@ __global__ void __launch_bounds__(kBlock)
             const double so = block_sum(keo, sRed);
             const double sn = block_sum(ken, sRed);
             if (tid == 0) {
-                I->ke_partial[(int64_t)it.slot * kKeSlots + s] = 0.5 * S.mass * so;
-                I->ke_partial[(int64_t)it.slot * kKeSlots + kMaxSpecies + s] = 0.5 * S.mass * sn;
+                gstore(I->ke_partial, (int64_t)it.slot * kKeSlots + s, 0.5 * S.mass * so);
+                gstore(I->ke_partial, (int64_t)it.slot * kKeSlots + kMaxSpecies + s,
+                       0.5 * S.mass * sn);
             }
         }
     }
@@ -403,7 +404,7 @@ __global__ void __launch_bounds__(kBlock)
             const int gx = ((lox - 1 + lx) % nx + nx) % nx;
             const int gy = ((loy - 1 + ly) % ny + ny) % ny;
             const int gz = ((loz - 1 + lz) % nz + nz) % nz;
-            atomicAdd(&racc[gx + (int64_t)nx * (gy + (int64_t)ny * gz)], v);
+            gatomic_add(&racc[gx + (int64_t)nx * (gy + (int64_t)ny * gz)], v);
         }
     }
 }
@@ -432,7 +433,7 @@ __global__ void k_bin_count(const InstDev *__restrict__ insts, int inst, int s, 
         z = apply_periodic(apply_periodic(z, I->n[2]), I->n[2]);
     }
     if (!in_box(I, x, y, z)) {
-        atomicAdd(&I->flags[kFlagRange], 1ull);
+        gatomic_add(&I->flags[kFlagRange], 1ull);
         return;
     }
     const int t = ((int)x) / tx_ + I->nt[0] * (((int)y) / ty_ + I->nt[1] * (((int)z) / tz_));
@@ -463,7 +464,7 @@ __global__ void k_bin_scatter(const InstDev *__restrict__ insts, int inst, int s
     if (slot < S.bin_cap[t])
         store_p64(S, lbuf ^ 1, S.bin_base[t] + slot, q, ids);
     else
-        atomicAdd(&I->flags[kFlagOverflow], 1ull);
+        gatomic_add(&I->flags[kFlagOverflow], 1ull);
 }
 
 // Compact both runs of every bin of buffer src into a contiguous list of

@@ -1,13 +1,20 @@
 // ohpic_particles_mixed.hip -- mixed-precision particle kernels for gfx950.
 //
-// Storage per particle (28 B): an int32 cell index packing ix | iy << 10 |
-// iz << 20, FP32 in-cell offsets ox, oy, oz in [0, 1) and FP32 velocities.
-// Gather, Boris, drift and the deposit weights run in FP32; every deposit
-// contribution is rounded into the same int64 fixed-point charge grid the
-// FP64 path uses, so rho, the Poisson solve, E and the diagnostics stay FP64.
-// The periodic boundary is the integer wrap of the cell index (the position
-// never leaves its normalised cell + offset form).  Work decomposition,
-// LDS staging, deposit blocks and re-binning are those of the FP64 kernel.
+// Storage per binned particle (24 B): for each axis one 32-bit word holding
+// the particle's cell relative to its tile origin in the top 8 bits (signed)
+// and its in-cell offset as a 24-bit fixed-point fraction in the low 24 bits,
+// and FP32 velocities.  The tile is the bin the particle sits in, so the cell
+// index costs no storage; the offset f24 * 2^-24 converts exactly to FP32.
+// Contiguous particle lists (host transfers, re-binning input, the overflow
+// retry) carry the tile index in a seventh word.
+//
+// Gather, Boris, drift and the deposit weights run in FP32; the new offset is
+// rounded to the 24-bit grid (carry into the cell) and the deposit uses that
+// stored position; every deposit contribution is rounded into the same int64
+// fixed-point charge grid the FP64 path uses, so rho, the Poisson solve, E
+// and the diagnostics stay FP64.  The periodic boundary is an integer wrap of
+// the cell index.  Work decomposition, LDS staging, deposit blocks and
+// re-binning are those of the FP64 kernel.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -22,46 +29,87 @@ namespace ohpic {
 
 namespace {
 
-constexpr int kCellBits = 10;
-constexpr int kCellMask = (1 << kCellBits) - 1;
-constexpr float kOffsetMax = 0.99999994f;  // largest float below 1
+#ifndef OHPIC_MX_PREFETCH
+#define OHPIC_MX_PREFETCH 1
+#endif
+// Minimum waves per SIMD of the fused mixed kernel: 5 (96 VGPRs) measured
+// faster than the 4 the compiler picks on its own at 256^3 x 64 ppc.
+#ifndef OHPIC_MX_MIN_WAVES
+#define OHPIC_MX_MIN_WAVES 5
+#endif
+constexpr int kMxPrefetch = OHPIC_MX_PREFETCH;  // particles per lane loaded ahead
 
-__device__ __forceinline__ int cell_pack(int x, int y, int z)
+constexpr int kFracBits = 24;
+constexpr uint32_t kFracOne = 1u << kFracBits;
+constexpr uint32_t kFracMask = kFracOne - 1u;
+constexpr float kFracScale = 16777216.0f;            // 2^24
+constexpr float kFracUnit = 5.9604644775390625e-8f;  // 2^-24
+
+__device__ __forceinline__ uint32_t pos_word(int rel, uint32_t f24)
 {
-    return x | (y << kCellBits) | (z << (2 * kCellBits));
+    return ((uint32_t)rel << kFracBits) | f24;
+}
+__device__ __forceinline__ int pos_rel(uint32_t w) { return (int32_t)w >> kFracBits; }
+__device__ __forceinline__ float pos_frac(uint32_t w) { return (float)(w & kFracMask) * kFracUnit; }
+
+// o = offset from the particle's current cell (any finite value) -> integer
+// cell shift d and the 24-bit fraction of o - d rounded to nearest (a
+// fraction that rounds up to 1 carries into d).  False for a non-finite or
+// absurd offset (d = 0, fraction 0).
+__device__ __forceinline__ bool quantize_offset(float o, int &d, uint32_t &f24)
+{
+    if (!(fabsf(o) < 1.0e9f)) {
+        d = 0;
+        f24 = 0u;
+        return false;
+    }
+    const float fl = floorf(o);
+    d = (int)fl;
+    uint32_t q = (uint32_t)rintf((o - fl) * kFracScale);
+    if (q >= kFracOne) {
+        q -= kFracOne;
+        d += 1;
+    }
+    f24 = q;
+    return true;
 }
 
 struct PMx {
-    int cell;
-    float ox, oy, oz, vx, vy, vz;
+    uint32_t w[3];
+    float v[3];
     int64_t id;
 };
 
+// binned form: buf[b][1..3] position words, buf[b][4..6] velocities
 __device__ __forceinline__ PMx load_pmx(const SpecDev &S, int b, int64_t o, bool ids)
 {
     PMx q;
-    q.cell = ((const int32_t *)S.buf[b][0])[o];
-    q.ox = ((const float *)S.buf[b][1])[o];
-    q.oy = ((const float *)S.buf[b][2])[o];
-    q.oz = ((const float *)S.buf[b][3])[o];
-    q.vx = ((const float *)S.buf[b][4])[o];
-    q.vy = ((const float *)S.buf[b][5])[o];
-    q.vz = ((const float *)S.buf[b][6])[o];
-    q.id = ids ? S.pid[b][o] : 0;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        q.w[a] = gload((const uint32_t *)S.buf[b][1 + a], o);
+        q.v[a] = gload((const float *)S.buf[b][4 + a], o);
+    }
+    q.id = ids ? gload(S.pid[b], o) : 0;
     return q;
 }
 
 __device__ __forceinline__ void store_pmx(const SpecDev &S, int b, int64_t o, const PMx &q,
                                           bool ids)
 {
-    ((int32_t *)S.buf[b][0])[o] = q.cell;
-    ((float *)S.buf[b][1])[o] = q.ox;
-    ((float *)S.buf[b][2])[o] = q.oy;
-    ((float *)S.buf[b][3])[o] = q.oz;
-    ((float *)S.buf[b][4])[o] = q.vx;
-    ((float *)S.buf[b][5])[o] = q.vy;
-    ((float *)S.buf[b][6])[o] = q.vz;
-    if (ids) S.pid[b][o] = q.id;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        gstore((uint32_t *)S.buf[b][1 + a], o, q.w[a]);
+        gstore((float *)S.buf[b][4 + a], o, q.v[a]);
+    }
+    if (ids) gstore(S.pid[b], o, q.id);
+}
+
+// list form: the binned words plus the tile index in buf[b][0]
+__device__ __forceinline__ void store_list_mx(const SpecDev &S, int b, int64_t o, const PMx &q,
+                                              int tile, bool ids)
+{
+    gstore((int32_t *)S.buf[b][0], o, (int32_t)tile);
+    store_pmx(S, b, o, q, ids);
 }
 
 template <int WO>
@@ -76,43 +124,27 @@ __device__ __forceinline__ void cic_wf(float r, float &w0, float &w1)
     }
 }
 
-// Move the offset back into [0, 1) and shift the cell coordinate with the
-// periodic integer wrap.  Returns false when the offset is not finite.
-__device__ __forceinline__ bool recell(float &o, int &c, int n)
+__device__ __forceinline__ int wrap_cell(int g, int n)
 {
-    if (o >= 0.0f && o < 1.0f) return true;
-    if (!(fabsf(o) < 1.0e9f)) {
-        // non-finite or absurd displacement: reported by the caller as out of range
-        o = 0.0f;
-        return false;
-    }
-    const float f = floorf(o);
-    int d = (int)f;
-    o = o - f;
-    if (o >= 1.0f) {
-        o -= 1.0f;
-        d += 1;
-    }
-    c += d;
-    if (c < 0 || c >= n) c = ((c % n) + n) % n;
-    return o >= 0.0f && o < 1.0f;
+    if (g < 0 || g >= n) g = ((g % n) + n) % n;
+    return g;
 }
 
 __device__ __forceinline__ void route_back_mx(const SpecDev &S, int nbuf, int t, const PMx &q,
                                               bool ids, unsigned long long *flags)
 {
-    const unsigned long long old = atomicAdd(&S.count[nbuf][t], 1ull << 32);
+    const unsigned long long old = gatomic_add(&S.count[nbuf][t], 1ull << 32);
     const int f = bin_front(old), b = bin_back(old);
-    const int cap = S.bin_cap[t];
+    const int cap = gload(S.bin_cap, t);
     if (b + f < cap)
-        store_pmx(S, nbuf, S.bin_base[t] + (cap - 1 - b), q, ids);
+        store_pmx(S, nbuf, gload(S.bin_base, t) + (cap - 1 - b), q, ids);
     else
-        atomicAdd(&flags[kFlagOverflow], 1ull);
+        gatomic_add(&flags[kFlagOverflow], 1ull);
 }
 
 struct LeaveBufMx {
-    float c[6][kLeaveCap];
-    int32_t cell[kLeaveCap];
+    uint32_t w[3][kLeaveCap];
+    float v[3][kLeaveCap];
     int64_t id[kLeaveCap];
     int32_t dest[kLeaveCap];
 };
@@ -126,13 +158,11 @@ __device__ __forceinline__ void flush_leavers_mx(LeaveBufMx &lb, int cnt, const 
     const int l = lane_id();
     if (l < cnt) {
         PMx q;
-        q.cell = lb.cell[l];
-        q.ox = lb.c[0][l];
-        q.oy = lb.c[1][l];
-        q.oz = lb.c[2][l];
-        q.vx = lb.c[3][l];
-        q.vy = lb.c[4][l];
-        q.vz = lb.c[5][l];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            q.w[a] = lb.w[a][l];
+            q.v[a] = lb.v[a][l];
+        }
         q.id = lb.id[l];
         route_back_mx(S, nbuf, lb.dest[l], q, ids, flags);
     }
@@ -143,7 +173,7 @@ __device__ __forceinline__ void flush_leavers_mx(LeaveBufMx &lb, int cnt, const 
 
 template <int TX, int TY, int TZ, int WO, int MODE>
 __global__ void __launch_bounds__(kBlock)
-    __attribute__((amdgpu_waves_per_eu(OHPIC_PUSH_MIN_WAVES, 8)))
+    __attribute__((amdgpu_waves_per_eu(OHPIC_MX_MIN_WAVES, 8)))
     k_particles_mixed(const InstDev *__restrict__ insts, const WorkItem *__restrict__ items,
                       const int *__restrict__ parity)
 {
@@ -182,9 +212,9 @@ __global__ void __launch_bounds__(kBlock)
             const int lx = l % EX, ly = (l / EX) % EY, lz = l / (EX * EY);
             const int gx = (lox + lx) % nx, gy = (loy + ly) % ny, gz = (loz + lz) % nz;
             const int64_t g = gx + (int64_t)nx * (gy + (int64_t)ny * gz);
-            sE[l] = (float)ex[g];
-            sE[NE + l] = (float)ey[g];
-            sE[2 * NE + l] = (float)ez[g];
+            sE[l] = (float)gload(ex, g);
+            sE[NE + l] = (float)gload(ey, g);
+            sE[2 * NE + l] = (float)gload(ez, g);
         }
     }
     if (DEP)
@@ -202,12 +232,12 @@ __global__ void __launch_bounds__(kBlock)
 
     for (int s = 0; s < I->nspecies; ++s) {
         const SpecDev &S = I->sp[s];
-        const int cap = S.bin_cap[tile];
-        const unsigned long long word = S.count[par][tile];
+        const int cap = gload(S.bin_cap, tile);
+        const unsigned long long word = gload(S.count[par], tile);
         const int F = min(bin_front(word), cap);
         const int B = min(bin_back(word), cap - F);
         const int cnt = F + B;
-        const int64_t base = S.bin_base[tile];
+        const int64_t base = gload(S.bin_base, tile);
         int per = (cnt + it.nchunks - 1) / it.nchunks;
         per = (per + 63) & ~63;
         const int start = it.chunk * per;
@@ -217,7 +247,7 @@ __global__ void __launch_bounds__(kBlock)
                 int lim = cap;
                 if (!LIST && excl && cnt > 0) {
                     const unsigned long long old =
-                        atomicAdd(&S.count[nbuf][tile], (unsigned long long)cnt);
+                        gatomic_add(&S.count[nbuf][tile], (unsigned long long)cnt);
                     lim = cap - bin_back(old);
                 }
                 sFront = 0;
@@ -233,25 +263,37 @@ __global__ void __launch_bounds__(kBlock)
                     s2 = 2.0f * t2 / (1.0f + tt);
         double keo = 0.0, ken = 0.0;
 
-        PMx nextq{0, 0, 0, 0, 0, 0, 0, 0};
-        if (start + tid < end)
-            nextq = load_pmx(S, par, base + bin_slot(start + tid, F, B, cap), ids);
+        // the next kMxPrefetch particles of this lane are loaded ahead, so each
+        // lane keeps that many particle reads in flight
+        PMx ahead[kMxPrefetch];
+#pragma unroll
+        for (int k = 0; k < kMxPrefetch; ++k) {
+            ahead[k] = PMx{{0u, 0u, 0u}, {0.0f, 0.0f, 0.0f}, 0};
+            if (start + tid + k * kBlock < end)
+                ahead[k] = load_pmx(S, par, base + bin_slot(start + tid + k * kBlock, F, B, cap),
+                                    ids);
+        }
         for (int p0 = start; p0 < end; p0 += kBlock) {
             const int p = p0 + tid;
             const bool active = p < end;
-            PMx q = nextq;
-            if (p + kBlock < end)
-                nextq = load_pmx(S, par, base + bin_slot(p + kBlock, F, B, cap), ids);
-            int cx = q.cell & kCellMask, cy = (q.cell >> kCellBits) & kCellMask,
-                cz = q.cell >> (2 * kCellBits);
+            PMx q = ahead[0];
+#pragma unroll
+            for (int k = 0; k + 1 < kMxPrefetch; ++k) ahead[k] = ahead[k + 1];
+            if (p + kMxPrefetch * kBlock < end)
+                ahead[kMxPrefetch - 1] =
+                    load_pmx(S, par, base + bin_slot(p + kMxPrefetch * kBlock, F, B, cap), ids);
+            // cell relative to this tile's origin and in-cell offset per axis
+            int rx = pos_rel(q.w[0]), ry = pos_rel(q.w[1]), rz = pos_rel(q.w[2]);
+            float ox = pos_frac(q.w[0]), oy = pos_frac(q.w[1]), oz = pos_frac(q.w[2]);
+            bool ok = true;
             if (PUSH && active) {
                 float wx0, wx1, wy0, wy1, wz0, wz1;
-                cic_wf<WO>(q.ox, wx0, wx1);
-                cic_wf<WO>(q.oy, wy0, wy1);
-                cic_wf<WO>(q.oz, wz0, wz1);
-                const int lx = min(max(cx - lox, 0), TX - 1);
-                const int ly = min(max(cy - loy, 0), TY - 1);
-                const int lz = min(max(cz - loz, 0), TZ - 1);
+                cic_wf<WO>(ox, wx0, wx1);
+                cic_wf<WO>(oy, wy0, wy1);
+                cic_wf<WO>(oz, wz0, wz1);
+                const int lx = min(max(rx, 0), TX - 1);
+                const int ly = min(max(ry, 0), TY - 1);
+                const int lz = min(max(rz, 0), TZ - 1);
                 const int l0 = lx + EX * (ly + EY * lz);
                 float e0 = 0.0f, e1 = 0.0f, e2 = 0.0f;
                 const float wx[2] = {wx0, wx1}, wy[2] = {wy0, wy1}, wz[2] = {wz0, wz1};
@@ -267,45 +309,73 @@ __global__ void __launch_bounds__(kBlock)
                             e1 += sE[NE + l] * w;
                             e2 += sE[2 * NE + l] * w;
                         }
-                keo += (double)(q.vx * q.vx + q.vy * q.vy + q.vz * q.vz);
-                const float um0 = q.vx + qm * e0 * dt2;
-                const float um1 = q.vy + qm * e1 * dt2;
-                const float um2 = q.vz + qm * e2 * dt2;
+                const float vx = q.v[0], vy = q.v[1], vz = q.v[2];
+                keo += (double)(vx * vx + vy * vy + vz * vz);
+                const float um0 = vx + qm * e0 * dt2;
+                const float um1 = vy + qm * e1 * dt2;
+                const float um2 = vz + qm * e2 * dt2;
                 const float ua0 = um0 + (um1 * t2 - um2 * t1);
                 const float ua1 = um1 + (um2 * t0 - um0 * t2);
                 const float ua2 = um2 + (um0 * t1 - um1 * t0);
                 const float up0 = um0 + (ua1 * s2 - ua2 * s1);
                 const float up1 = um1 + (ua2 * s0 - ua0 * s2);
                 const float up2 = um2 + (ua0 * s1 - ua1 * s0);
-                q.vx = up0 + qm * e0 * dt2;
-                q.vy = up1 + qm * e1 * dt2;
-                q.vz = up2 + qm * e2 * dt2;
-                ken += (double)(q.vx * q.vx + q.vy * q.vy + q.vz * q.vz);
-                q.ox = q.ox + q.vx * dt;
-                q.oy = q.oy + q.vy * dt;
-                q.oz = q.oz + q.vz * dt;
-                const bool ok = recell(q.ox, cx, nx) & recell(q.oy, cy, ny) &
-                                recell(q.oz, cz, nz);
-                if (!ok) {
-                    atomicAdd(&flags[kFlagRange], 1ull);
-                    q.ox = q.oy = q.oz = 0.0f;
+                q.v[0] = up0 + qm * e0 * dt2;
+                q.v[1] = up1 + qm * e1 * dt2;
+                q.v[2] = up2 + qm * e2 * dt2;
+                ken += (double)(q.v[0] * q.v[0] + q.v[1] * q.v[1] + q.v[2] * q.v[2]);
+                int dx, dy, dz;
+                uint32_t fx, fy, fz;
+                ok = quantize_offset(ox + q.v[0] * dt, dx, fx) &
+                     quantize_offset(oy + q.v[1] * dt, dy, fy) &
+                     quantize_offset(oz + q.v[2] * dt, dz, fz);
+                rx += dx;
+                ry += dy;
+                rz += dz;
+                ox = (float)fx * kFracUnit;
+                oy = (float)fy * kFracUnit;
+                oz = (float)fz * kFracUnit;
+                if (MODE == kModeGatherPush) {
+                    // not re-binned: keep the cell relative to this tile (int8)
+                    if (rx < -128 || rx > 127 || ry < -128 || ry > 127 || rz < -128 || rz > 127)
+                        ok = false;
+                    if (!ok) {
+                        gatomic_add(&flags[kFlagRange], 1ull);
+                        rx = min(max(rx, 0), TX - 1);
+                        ry = min(max(ry, 0), TY - 1);
+                        rz = min(max(rz, 0), TZ - 1);
+                        fx = fy = fz = 0u;
+                    }
+                    q.w[0] = pos_word(rx, fx);
+                    q.w[1] = pos_word(ry, fy);
+                    q.w[2] = pos_word(rz, fz);
+                    store_pmx(S, par, base + bin_slot(p, F, B, cap), q, ids);
                 }
-                q.cell = cell_pack(cx, cy, cz);
+                if (!ok && MODE != kModeGatherPush) {
+                    gatomic_add(&flags[kFlagRange], 1ull);
+                    ox = oy = oz = 0.0f;
+                }
             }
-            if (MODE == kModeGatherPush) {
-                if (active) store_pmx(S, par, base + bin_slot(p, F, B, cap), q, ids);
-                continue;
+            if (MODE == kModeGatherPush) continue;
+            // global cell, periodic; new tile and the cell relative to it
+            const int gx = wrap_cell(lox + rx, nx), gy = wrap_cell(loy + ry, ny),
+                      gz = wrap_cell(loz + rz, nz);
+            const int ctx_t = gx / TX, cty = gy / TY, ctz = gz / TZ;
+            const int ntile = ctx_t + ntx * (cty + nty * ctz);
+            if (REBIN) {
+                q.w[0] = pos_word(gx - ctx_t * TX, (uint32_t)(ox * kFracScale));
+                q.w[1] = pos_word(gy - cty * TY, (uint32_t)(oy * kFracScale));
+                q.w[2] = pos_word(gz - ctz * TZ, (uint32_t)(oz * kFracScale));
             }
-            const int ntile = cx / TX + ntx * (cy / TY + nty * (cz / TZ));
             if (DEP && active) {
                 float wx0, wx1, wy0, wy1, wz0, wz1;
-                cic_wf<WO>(q.ox, wx0, wx1);
-                cic_wf<WO>(q.oy, wy0, wy1);
-                cic_wf<WO>(q.oz, wz0, wz1);
+                cic_wf<WO>(ox, wx0, wx1);
+                cic_wf<WO>(oy, wy0, wy1);
+                cic_wf<WO>(oz, wz0, wz1);
                 const float wx[2] = {wx0, wx1}, wy[2] = {wy0, wy1}, wz[2] = {wz0, wz1};
-                const int dx = wrap_index(cx - lox + 1, nx);
-                const int dy = wrap_index(cy - loy + 1, ny);
-                const int dz = wrap_index(cz - loz + 1, nz);
+                const int dx = wrap_index(gx - lox + 1, nx);
+                const int dy = wrap_index(gy - loy + 1, ny);
+                const int dz = wrap_index(gz - loz + 1, nz);
                 if (dx <= TX + 1 && dy <= TY + 1 && dz <= TZ + 1) {
                     const int l0 = dx + RX * (dy + RY * dz);
 #pragma unroll
@@ -320,19 +390,19 @@ __global__ void __launch_bounds__(kBlock)
                                           (unsigned long long)v);
                             }
                 } else {
-                    atomicAdd(&flags[kFlagFast], 1ull);
+                    gatomic_add(&flags[kFlagFast], 1ull);
 #pragma unroll
                     for (int i = 0; i < 2; ++i)
 #pragma unroll
                         for (int j = 0; j < 2; ++j)
 #pragma unroll
                             for (int k = 0; k < 2; ++k) {
-                                const int gx = (cx + i) % nx, gy = (cy + j) % ny,
-                                          gz = (cz + k) % nz;
+                                const int hx = (gx + i) % nx, hy = (gy + j) % ny,
+                                          hz = (gz + k) % nz;
                                 const long long v =
                                     fix_round((double)(wx[i] * wy[j] * wz[k]), qfix);
-                                atomicAdd(&racc[gx + (int64_t)nx * (gy + (int64_t)ny * gz)],
-                                          (unsigned long long)v);
+                                gatomic_add(&racc[hx + (int64_t)nx * (hy + (int64_t)ny * hz)],
+                                            (unsigned long long)v);
                             }
                 }
             }
@@ -342,10 +412,11 @@ __global__ void __launch_bounds__(kBlock)
                     const int leader = __ffsll((unsigned long long)m) - 1;
                     unsigned long long b = 0;
                     if (lane_id() == leader)
-                        b = atomicAdd(&S.count[nbuf][0], (unsigned long long)__popcll(m));
+                        b = gatomic_add(&S.count[nbuf][0], (unsigned long long)__popcll(m));
                     b = __shfl(b, leader, 64);
                     if (active)
-                        store_pmx(S, nbuf, (int64_t)b + __popcll(m & lanemask_lt), q, ids);
+                        store_list_mx(S, nbuf, (int64_t)b + __popcll(m & lanemask_lt), q, ntile,
+                                      ids);
                 }
             } else if (REBIN) {
                 const bool stay = active && ntile == tile;
@@ -359,7 +430,7 @@ __global__ void __launch_bounds__(kBlock)
                         lim = sFrontLimit;
                     } else if (lane_id() == leader) {
                         const unsigned long long old =
-                            atomicAdd(&S.count[nbuf][tile], (unsigned long long)nst);
+                            gatomic_add(&S.count[nbuf][tile], (unsigned long long)nst);
                         b = bin_front(old);
                         lim = cap - bin_back(old);
                     }
@@ -370,7 +441,7 @@ __global__ void __launch_bounds__(kBlock)
                         if (slot < lim)
                             store_pmx(S, nbuf, base + slot, q, ids);
                         else
-                            atomicAdd(&flags[kFlagOverflow], 1ull);
+                            gatomic_add(&flags[kFlagOverflow], 1ull);
                     }
                 }
                 const bool leave = active && ntile != tile;
@@ -386,13 +457,11 @@ __global__ void __launch_bounds__(kBlock)
                         if (leave) {
                             const int slot = c + (int)__popcll(lm & lanemask_lt);
                             LeaveBufMx &lb = sLeave[wv];
-                            lb.cell[slot] = q.cell;
-                            lb.c[0][slot] = q.ox;
-                            lb.c[1][slot] = q.oy;
-                            lb.c[2][slot] = q.oz;
-                            lb.c[3][slot] = q.vx;
-                            lb.c[4][slot] = q.vy;
-                            lb.c[5][slot] = q.vz;
+#pragma unroll
+                            for (int a = 0; a < 3; ++a) {
+                                lb.w[a][slot] = q.w[a];
+                                lb.v[a][slot] = q.v[a];
+                            }
                             lb.id[slot] = q.id;
                             lb.dest[slot] = ntile;
                         }
@@ -409,7 +478,7 @@ __global__ void __launch_bounds__(kBlock)
             __syncthreads();
             if (tid < kWaves) sLcnt[tid] = 0;
             if (tid == 0 && excl && cnt > sFront)
-                atomicAdd(&S.count[nbuf][tile],
+                gatomic_add(&S.count[nbuf][tile],
                           (unsigned long long)(-(long long)(cnt - sFront)));
             __syncthreads();
         }
@@ -417,8 +486,9 @@ __global__ void __launch_bounds__(kBlock)
             const double so = block_sum(keo, sRed);
             const double sn = block_sum(ken, sRed);
             if (tid == 0) {
-                I->ke_partial[(int64_t)it.slot * kKeSlots + s] = 0.5 * S.mass * so;
-                I->ke_partial[(int64_t)it.slot * kKeSlots + kMaxSpecies + s] = 0.5 * S.mass * sn;
+                gstore(I->ke_partial, (int64_t)it.slot * kKeSlots + s, 0.5 * S.mass * so);
+                gstore(I->ke_partial, (int64_t)it.slot * kKeSlots + kMaxSpecies + s,
+                       0.5 * S.mass * sn);
             }
         }
     }
@@ -432,25 +502,26 @@ __global__ void __launch_bounds__(kBlock)
             const int gx = ((lox - 1 + lx) % nx + nx) % nx;
             const int gy = ((loy - 1 + ly) % ny + ny) % ny;
             const int gz = ((loz - 1 + lz) % nz + nz) % nz;
-            atomicAdd(&racc[gx + (int64_t)nx * (gy + (int64_t)ny * gz)], v);
+            gatomic_add(&racc[gx + (int64_t)nx * (gy + (int64_t)ny * gz)], v);
         }
     }
 }
 
 // ---------------------------------------------------------------- binning
-__device__ __forceinline__ bool valid_mx(const InstDev *I, const PMx &q)
+// A list entry (tile, words) names a cell inside the box: tile in range, the
+// relative cell inside the tile, the global cell inside the (possibly partly
+// covered) last tile, the fraction below 1 by construction.
+__device__ __forceinline__ bool list_valid(const InstDev *I, int tile, const PMx &q, int tx_,
+                                           int ty_, int tz_)
 {
-    const int cx = q.cell & kCellMask, cy = (q.cell >> kCellBits) & kCellMask,
-              cz = q.cell >> (2 * kCellBits);
-    return q.cell >= 0 && cx < I->n[0] && cy < I->n[1] && cz < I->n[2] && q.ox >= 0.0f &&
-           q.ox < 1.0f && q.oy >= 0.0f && q.oy < 1.0f && q.oz >= 0.0f && q.oz < 1.0f;
-}
-
-__device__ __forceinline__ int tile_of_mx(const InstDev *I, int cell, int tx_, int ty_, int tz_)
-{
-    const int cx = cell & kCellMask, cy = (cell >> kCellBits) & kCellMask,
-              cz = cell >> (2 * kCellBits);
-    return cx / tx_ + I->nt[0] * (cy / ty_ + I->nt[1] * (cz / tz_));
+    if (tile < 0 || tile >= I->ntiles) return false;
+    const int t3[3] = {tile % I->nt[0], (tile / I->nt[0]) % I->nt[1], tile / (I->nt[0] * I->nt[1])};
+    const int T3[3] = {tx_, ty_, tz_};
+    for (int a = 0; a < 3; ++a) {
+        const int r = pos_rel(q.w[a]);
+        if (r < 0 || r >= T3[a] || t3[a] * T3[a] + r >= I->n[a]) return false;
+    }
+    return true;
 }
 
 __global__ void k_bin_count_mx(const InstDev *__restrict__ insts, int inst, int s, int lbuf,
@@ -461,11 +532,12 @@ __global__ void k_bin_count_mx(const InstDev *__restrict__ insts, int inst, int 
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const PMx q = load_pmx(S, lbuf, i, false);
-    if (!valid_mx(I, q)) {
-        atomicAdd(&I->flags[kFlagRange], 1ull);
+    const int t = ((const int32_t *)S.buf[lbuf][0])[i];
+    if (!list_valid(I, t, q, tx_, ty_, tz_)) {
+        gatomic_add(&I->flags[kFlagRange], 1ull);
         return;
     }
-    atomicAdd(&S.count[lbuf ^ 1][tile_of_mx(I, q.cell, tx_, ty_, tz_)], 1ull);
+    atomicAdd(&S.count[lbuf ^ 1][t], 1ull);
 }
 
 __global__ void k_bin_scatter_mx(const InstDev *__restrict__ insts, int inst, int s, int lbuf,
@@ -477,14 +549,14 @@ __global__ void k_bin_scatter_mx(const InstDev *__restrict__ insts, int inst, in
     if (i >= n) return;
     const bool ids = I->track_ids != 0;
     const PMx q = load_pmx(S, lbuf, i, ids);
-    if (!valid_mx(I, q)) return;
-    const int t = tile_of_mx(I, q.cell, tx_, ty_, tz_);
+    const int t = ((const int32_t *)S.buf[lbuf][0])[i];
+    if (!list_valid(I, t, q, tx_, ty_, tz_)) return;
     const unsigned long long old = atomicAdd(&S.count[lbuf ^ 1][t], 1ull);
     const int slot = bin_front(old);
     if (slot < S.bin_cap[t])
         store_pmx(S, lbuf ^ 1, S.bin_base[t] + slot, q, ids);
     else
-        atomicAdd(&I->flags[kFlagOverflow], 1ull);
+        gatomic_add(&I->flags[kFlagOverflow], 1ull);
 }
 
 __global__ void k_bins_to_list_mx(const InstDev *__restrict__ insts, int inst, int s, int src,
@@ -501,16 +573,13 @@ __global__ void k_bins_to_list_mx(const InstDev *__restrict__ insts, int inst, i
     const int64_t ob = list_off[t];
     const bool ids = I->track_ids != 0;
     for (int v = threadIdx.x; v < F + B; v += blockDim.x) {
-        const int64_t o = ib + bin_slot(v, F, B, cap);
-#pragma unroll
-        for (int c = 0; c < 7; ++c)
-            ((uint32_t *)S.buf[dst][c])[ob + v] = ((const uint32_t *)S.buf[src][c])[o];
-        if (ids) S.pid[dst][ob + v] = S.pid[src][o];
+        const PMx q = load_pmx(S, src, ib + bin_slot(v, F, B, cap), ids);
+        store_list_mx(S, dst, ob + v, q, t, ids);
     }
 }
 
 __global__ void k_cell_counts_mx(const InstDev *__restrict__ insts, int inst, int s, int par,
-                                 int32_t *__restrict__ counts)
+                                 int32_t *__restrict__ counts, int tx_, int ty_, int tz_)
 {
     const InstDev *I = insts + inst;
     const SpecDev &S = I->sp[s];
@@ -521,35 +590,48 @@ __global__ void k_cell_counts_mx(const InstDev *__restrict__ insts, int inst, in
     const int B = min(bin_back(w), cap - F);
     const int64_t b = S.bin_base[t];
     const int nx = I->n[0], ny = I->n[1];
+    const int ox = (t % I->nt[0]) * tx_, oy = ((t / I->nt[0]) % I->nt[1]) * ty_,
+              oz = (t / (I->nt[0] * I->nt[1])) * tz_;
     for (int v = threadIdx.x; v < F + B; v += blockDim.x) {
-        const int cell = ((const int32_t *)S.buf[par][0])[b + bin_slot(v, F, B, cap)];
-        const int cx = cell & kCellMask, cy = (cell >> kCellBits) & kCellMask,
-                  cz = cell >> (2 * kCellBits);
+        const int64_t o = b + bin_slot(v, F, B, cap);
+        const int cx = ox + pos_rel(((const uint32_t *)S.buf[par][1])[o]);
+        const int cy = oy + pos_rel(((const uint32_t *)S.buf[par][2])[o]);
+        const int cz = oz + pos_rel(((const uint32_t *)S.buf[par][3])[o]);
         atomicAdd(&counts[cx + (int64_t)nx * (cy + (int64_t)ny * cz)], 1);
     }
 }
 
 // ---------------------------------------------------------------- loaders
-__device__ __forceinline__ void store_loaded_mx(const SpecDev &S, int b, int64_t i, double x,
-                                                double y, double z, double vx, double vy,
-                                                double vz, const int n[3], int64_t id, bool ids)
+// FP64 position -> list entry.  The cell is the FP64 int() truncation; an
+// offset that rounds up to 1 on the 24-bit grid is clamped below it, so the
+// particle stays on the same side of the cell face as in FP64 (the mirrored
+// CIC weights jump there).
+__device__ __forceinline__ void store_loaded_mx(const InstDev *I, const SpecDev &S, int b,
+                                                int64_t i, double x, double y, double z,
+                                                double vx, double vy, double vz, int64_t id,
+                                                bool ids, int tx_, int ty_, int tz_)
 {
     const double p[3] = {x, y, z};
-    int c[3];
-    float o[3];
+    const int T3[3] = {tx_, ty_, tz_};
+    int tc[3];
+    PMx q;
     for (int a = 0; a < 3; ++a) {
-        // keep the FP64 cell (int() truncation): an offset that rounds up to
-        // 1 is clamped below it, so the particle stays on the same side of
-        // the cell face as in FP64 (the mirrored CIC weights jump there)
-        c[a] = (int)p[a];
-        o[a] = fminf((float)(p[a] - (double)c[a]), kOffsetMax);
+        const int c = (int)p[a];
+        double f = rint((p[a] - (double)c) * 16777216.0);
+        if (f > 16777215.0) f = 16777215.0;
+        tc[a] = c / T3[a];
+        q.w[a] = pos_word(c - tc[a] * T3[a], (uint32_t)f);
     }
-    PMx q{cell_pack(c[0], c[1], c[2]), o[0], o[1], o[2], (float)vx, (float)vy, (float)vz, id};
-    store_pmx(S, b, i, q, ids);
+    q.v[0] = (float)vx;
+    q.v[1] = (float)vy;
+    q.v[2] = (float)vz;
+    q.id = id;
+    store_list_mx(S, b, i, q, tc[0] + I->nt[0] * (tc[1] + I->nt[1] * tc[2]), ids);
 }
 
 __global__ void k_load_uniform_mx(const InstDev *__restrict__ insts, int inst, int s, int b,
-                                  Pcg32Dev rng0, uint64_t offset, int64_t np, double vth)
+                                  Pcg32Dev rng0, uint64_t offset, int64_t np, double vth,
+                                  int tx_, int ty_, int tz_)
 {
     const InstDev *I = insts + inst;
     const SpecDev &S = I->sp[s];
@@ -562,13 +644,13 @@ __global__ void k_load_uniform_mx(const InstDev *__restrict__ insts, int inst, i
     const double z = pcg_double(r) * (double)I->n[2] + 0.0;
     double vx, vy, vz;
     maxwell3(r, vth, vx, vy, vz);
-    store_loaded_mx(S, b, i, x, y, z, vx, vy, vz, I->n, i, I->track_ids != 0);
+    store_loaded_mx(I, S, b, i, x, y, z, vx, vy, vz, i, I->track_ids != 0, tx_, ty_, tz_);
 }
 
 __global__ void k_load_hump_mx(const InstDev *__restrict__ insts, int inst, int s, int b,
                                Pcg32Dev rng0, uint64_t offset, uint64_t budget, int64_t np,
                                double amp, double sigma, double vth,
-                               unsigned long long *__restrict__ fail)
+                               unsigned long long *__restrict__ fail, int tx_, int ty_, int tz_)
 {
     const InstDev *I = insts + inst;
     const SpecDev &S = I->sp[s];
@@ -600,7 +682,7 @@ __global__ void k_load_hump_mx(const InstDev *__restrict__ insts, int inst, int 
     if (!ok) atomicAdd(fail, 1ull);
     double vx, vy, vz;
     maxwell3(r, vth, vx, vy, vz);
-    store_loaded_mx(S, b, i, x, y, z, vx, vy, vz, I->n, i, I->track_ids != 0);
+    store_loaded_mx(I, S, b, i, x, y, z, vx, vy, vz, i, I->track_ids != 0, tx_, ty_, tz_);
 }
 
 }  // namespace
@@ -675,31 +757,33 @@ hipError_t launch_bins_to_list_mx(const InstDev *insts, int inst, int s, int src
 }
 
 hipError_t launch_cell_counts_mx(const InstDev *insts, int inst, int s, int par, int ntiles,
-                                 int32_t *counts, hipStream_t st)
+                                 int32_t *counts, const int T[3], hipStream_t st)
 {
     if (ntiles <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_cell_counts_mx, dim3(ntiles), dim3(256), 0, st, insts, inst, s, par,
-                       counts);
+                       counts, T[0], T[1], T[2]);
     return hipGetLastError();
 }
 
 hipError_t launch_load_uniform_mx(const InstDev *insts, int inst, int s, int b, Pcg32Dev rng0,
-                                  uint64_t offset, int64_t np, double vth, hipStream_t st)
+                                  uint64_t offset, int64_t np, double vth, const int T[3],
+                                  hipStream_t st)
 {
     if (np <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_load_uniform_mx, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, st,
-                       insts, inst, s, b, rng0, offset, np, vth);
+                       insts, inst, s, b, rng0, offset, np, vth, T[0], T[1], T[2]);
     return hipGetLastError();
 }
 
 hipError_t launch_load_hump_mx(const InstDev *insts, int inst, int s, int b, Pcg32Dev rng0,
                                uint64_t offset, uint64_t budget, int64_t np, double amp,
                                double sigma, double vth, unsigned long long *fail,
-                               hipStream_t st)
+                               const int T[3], hipStream_t st)
 {
     if (np <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_load_hump_mx, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, st,
-                       insts, inst, s, b, rng0, offset, budget, np, amp, sigma, vth, fail);
+                       insts, inst, s, b, rng0, offset, budget, np, amp, sigma, vth, fail, T[0],
+                       T[1], T[2]);
     return hipGetLastError();
 }
 
