@@ -467,7 +467,9 @@ int rerun_as_list(ohpic_ctx *ctx, int list_mode)
     if (list_mode == kModeFusedList) HIPCHK(ctx, launch_reduce_ke_b(ctx->d_inst, 1, ctx->st));
     if ((rc = read_flags(ctx))) return rc;
     if (ctx->h_flags[kFlagRange])
-        return fail(ctx, OHPIC_ESTATE, "%llu particles moved more than two box lengths",
+        return fail(ctx, OHPIC_ESTATE,
+                    "%llu particles moved too far (more than two box lengths; in mixed "
+                    "precision 64 cells) or have non-finite coordinates",
                     (unsigned long long)ctx->h_flags[kFlagRange]);
     std::vector<int64_t> nlist(ctx->cfg.nspecies);
     for (int s = 0; s < ctx->cfg.nspecies; ++s) {
@@ -534,7 +536,8 @@ int absorb_flags(ohpic_ctx *ctx, const unsigned long long *f)
 {
     if (f[kFlagRange])
         return fail(ctx, OHPIC_ESTATE,
-                    "%llu particles moved more than two box lengths in one step",
+                    "%llu particles moved too far in one step (more than two box lengths; in "
+                    "mixed precision 64 cells) or have non-finite coordinates",
                     (unsigned long long)f[kFlagRange]);
     ctx->fast += (int64_t)f[kFlagFast];
     return f[kFlagOverflow] ? 1 : 0;

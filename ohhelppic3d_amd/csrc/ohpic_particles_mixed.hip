@@ -44,6 +44,7 @@ constexpr uint32_t kFracOne = 1u << kFracBits;
 constexpr uint32_t kFracMask = kFracOne - 1u;
 constexpr float kFracScale = 16777216.0f;            // 2^24
 constexpr float kFracUnit = 5.9604644775390625e-8f;  // 2^-24
+constexpr float kMaxShift = 64.0f * kFracScale;      // largest drift per step: 64 cells
 
 __device__ __forceinline__ uint32_t pos_word(int rel, uint32_t f24)
 {
@@ -51,28 +52,6 @@ __device__ __forceinline__ uint32_t pos_word(int rel, uint32_t f24)
 }
 __device__ __forceinline__ int pos_rel(uint32_t w) { return (int32_t)w >> kFracBits; }
 __device__ __forceinline__ float pos_frac(uint32_t w) { return (float)(w & kFracMask) * kFracUnit; }
-
-// o = offset from the particle's current cell (any finite value) -> integer
-// cell shift d and the 24-bit fraction of o - d rounded to nearest (a
-// fraction that rounds up to 1 carries into d).  False for a non-finite or
-// absurd offset (d = 0, fraction 0).
-__device__ __forceinline__ bool quantize_offset(float o, int &d, uint32_t &f24)
-{
-    if (!(fabsf(o) < 1.0e9f)) {
-        d = 0;
-        f24 = 0u;
-        return false;
-    }
-    const float fl = floorf(o);
-    d = (int)fl;
-    uint32_t q = (uint32_t)rintf((o - fl) * kFracScale);
-    if (q >= kFracOne) {
-        q -= kFracOne;
-        d += 1;
-    }
-    f24 = q;
-    return true;
-}
 
 struct PMx {
     uint32_t w[3];
@@ -102,6 +81,53 @@ __device__ __forceinline__ void store_pmx(const SpecDev &S, int b, int64_t o, co
         gstore((float *)S.buf[b][4 + a], o, q.v[a]);
     }
     if (ids) gstore(S.pid[b], o, q.id);
+}
+
+// The bin of one tile in one buffer: array bases offset to the bin's first
+// element (uniform per workgroup, so they live in scalar registers) and
+// particles addressed by a 32-bit byte offset (a bin holds < 2^30 particles),
+// which the global instructions take directly (no 64-bit address arithmetic
+// per particle).
+struct BinView {
+    const char *w[3];
+    const char *v[3];
+    const int64_t *id;
+};
+
+__device__ __forceinline__ BinView bin_view(const SpecDev &S, int b, int64_t base, bool ids)
+{
+    BinView r;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        r.w[a] = (const char *)((const uint32_t *)S.buf[b][1 + a] + base);
+        r.v[a] = (const char *)((const float *)S.buf[b][4 + a] + base);
+    }
+    r.id = ids ? S.pid[b] + base : nullptr;
+    return r;
+}
+
+__device__ __forceinline__ PMx load_bin(const BinView &bv, uint32_t slot, bool ids)
+{
+    const uint32_t off = slot * 4u;
+    PMx q;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        q.w[a] = *as_global((const uint32_t *)(bv.w[a] + off));
+        q.v[a] = *as_global((const float *)(bv.v[a] + off));
+    }
+    q.id = ids ? gload(bv.id, (int64_t)slot) : 0;
+    return q;
+}
+
+__device__ __forceinline__ void store_bin(const BinView &bv, uint32_t slot, const PMx &q, bool ids)
+{
+    const uint32_t off = slot * 4u;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        *as_global((uint32_t *)(bv.w[a] + off)) = q.w[a];
+        *as_global((float *)(bv.v[a] + off)) = q.v[a];
+    }
+    if (ids) gstore(const_cast<int64_t *>(bv.id), (int64_t)slot, q.id);
 }
 
 // list form: the binned words plus the tile index in buf[b][0]
@@ -224,6 +250,7 @@ __global__ void __launch_bounds__(kBlock)
 
     const float dt = (float)I->dt;
     const float dt2 = 0.5f * dt;
+    const float dts = dt * kFracScale;  // cells per unit velocity, in 2^-24 units
     const float bx = (float)I->b0[0], by = (float)I->b0[1], bz = (float)I->b0[2];
     const int nbuf = par ^ 1;
     const unsigned long long lanemask_lt = (1ull << lane_id()) - 1ull;
@@ -265,12 +292,14 @@ __global__ void __launch_bounds__(kBlock)
 
         // the next kMxPrefetch particles of this lane are loaded ahead, so each
         // lane keeps that many particle reads in flight
+        const BinView cur = bin_view(S, par, base, ids);
+        const BinView nxt = bin_view(S, nbuf, base, ids);
         PMx ahead[kMxPrefetch];
 #pragma unroll
         for (int k = 0; k < kMxPrefetch; ++k) {
             ahead[k] = PMx{{0u, 0u, 0u}, {0.0f, 0.0f, 0.0f}, 0};
             if (start + tid + k * kBlock < end)
-                ahead[k] = load_pmx(S, par, base + bin_slot(start + tid + k * kBlock, F, B, cap),
+                ahead[k] = load_bin(cur, (uint32_t)bin_slot(start + tid + k * kBlock, F, B, cap),
                                     ids);
         }
         for (int p0 = start; p0 < end; p0 += kBlock) {
@@ -281,11 +310,10 @@ __global__ void __launch_bounds__(kBlock)
             for (int k = 0; k + 1 < kMxPrefetch; ++k) ahead[k] = ahead[k + 1];
             if (p + kMxPrefetch * kBlock < end)
                 ahead[kMxPrefetch - 1] =
-                    load_pmx(S, par, base + bin_slot(p + kMxPrefetch * kBlock, F, B, cap), ids);
+                    load_bin(cur, (uint32_t)bin_slot(p + kMxPrefetch * kBlock, F, B, cap), ids);
             // cell relative to this tile's origin and in-cell offset per axis
             int rx = pos_rel(q.w[0]), ry = pos_rel(q.w[1]), rz = pos_rel(q.w[2]);
             float ox = pos_frac(q.w[0]), oy = pos_frac(q.w[1]), oz = pos_frac(q.w[2]);
-            bool ok = true;
             if (PUSH && active) {
                 float wx0, wx1, wy0, wy1, wz0, wz1;
                 cic_wf<WO>(ox, wx0, wx1);
@@ -324,36 +352,28 @@ __global__ void __launch_bounds__(kBlock)
                 q.v[1] = up1 + qm * e1 * dt2;
                 q.v[2] = up2 + qm * e2 * dt2;
                 ken += (double)(q.v[0] * q.v[0] + q.v[1] * q.v[1] + q.v[2] * q.v[2]);
-                int dx, dy, dz;
-                uint32_t fx, fy, fz;
-                ok = quantize_offset(ox + q.v[0] * dt, dx, fx) &
-                     quantize_offset(oy + q.v[1] * dt, dy, fy) &
-                     quantize_offset(oz + q.v[2] * dt, dz, fz);
-                rx += dx;
-                ry += dy;
-                rz += dz;
-                ox = (float)fx * kFracUnit;
-                oy = (float)fy * kFracUnit;
-                oz = (float)fz * kFracUnit;
-                if (MODE == kModeGatherPush) {
-                    // not re-binned: keep the cell relative to this tile (int8)
-                    if (rx < -128 || rx > 127 || ry < -128 || ry > 127 || rz < -128 || rz > 127)
-                        ok = false;
-                    if (!ok) {
-                        gatomic_add(&flags[kFlagRange], 1ull);
-                        rx = min(max(rx, 0), TX - 1);
-                        ry = min(max(ry, 0), TY - 1);
-                        rz = min(max(rz, 0), TZ - 1);
-                        fx = fy = fz = 0u;
-                    }
-                    q.w[0] = pos_word(rx, fx);
-                    q.w[1] = pos_word(ry, fy);
-                    q.w[2] = pos_word(rz, fz);
-                    store_pmx(S, par, base + bin_slot(p, F, B, cap), q, ids);
-                }
-                if (!ok && MODE != kModeGatherPush) {
+                // drift in fixed point: the tile-relative position word (8.24)
+                // plus the displacement rounded to 2^-24 of a cell, exact in
+                // int32 while |v dt| < 64 cells (farther is reported)
+                const float sx = q.v[0] * dts, sy = q.v[1] * dts, sz = q.v[2] * dts;
+                const bool ok =
+                    fabsf(sx) < kMaxShift && fabsf(sy) < kMaxShift && fabsf(sz) < kMaxShift;
+                if (!ok) {
                     gatomic_add(&flags[kFlagRange], 1ull);
-                    ox = oy = oz = 0.0f;
+                } else {
+                    q.w[0] = (uint32_t)((int32_t)q.w[0] + (int32_t)rintf(sx));
+                    q.w[1] = (uint32_t)((int32_t)q.w[1] + (int32_t)rintf(sy));
+                    q.w[2] = (uint32_t)((int32_t)q.w[2] + (int32_t)rintf(sz));
+                }
+                rx = pos_rel(q.w[0]);
+                ry = pos_rel(q.w[1]);
+                rz = pos_rel(q.w[2]);
+                ox = pos_frac(q.w[0]);
+                oy = pos_frac(q.w[1]);
+                oz = pos_frac(q.w[2]);
+                if (MODE == kModeGatherPush) {
+                    // not re-binned: the cell stays relative to this tile
+                    store_bin(cur, (uint32_t)bin_slot(p, F, B, cap), q, ids);
                 }
             }
             if (MODE == kModeGatherPush) continue;
@@ -363,9 +383,9 @@ __global__ void __launch_bounds__(kBlock)
             const int ctx_t = gx / TX, cty = gy / TY, ctz = gz / TZ;
             const int ntile = ctx_t + ntx * (cty + nty * ctz);
             if (REBIN) {
-                q.w[0] = pos_word(gx - ctx_t * TX, (uint32_t)(ox * kFracScale));
-                q.w[1] = pos_word(gy - cty * TY, (uint32_t)(oy * kFracScale));
-                q.w[2] = pos_word(gz - ctz * TZ, (uint32_t)(oz * kFracScale));
+                q.w[0] = pos_word(gx - ctx_t * TX, q.w[0] & kFracMask);
+                q.w[1] = pos_word(gy - cty * TY, q.w[1] & kFracMask);
+                q.w[2] = pos_word(gz - ctz * TZ, q.w[2] & kFracMask);
             }
             if (DEP && active) {
                 float wx0, wx1, wy0, wy1, wz0, wz1;
@@ -439,7 +459,7 @@ __global__ void __launch_bounds__(kBlock)
                     if (stay) {
                         const int slot = b + (int)__popcll(m & lanemask_lt);
                         if (slot < lim)
-                            store_pmx(S, nbuf, base + slot, q, ids);
+                            store_bin(nxt, (uint32_t)slot, q, ids);
                         else
                             gatomic_add(&flags[kFlagOverflow], 1ull);
                     }
