@@ -225,6 +225,10 @@ int ohpic_group_contains(const ohpic_group *g, const ohpic_ctx *ctx, int *yes);
 int ohpic_group_step(ohpic_group *g, int nsteps);
 /* enable != 0 (default): replay captured hipGraphs; 0: launch directly */
 int ohpic_group_use_graph(ohpic_group *g, int enable);
+/* enable != 0 (default): the particle step of all members is one persistent
+ * launch (a resident grid whose workgroups claim work items, heaviest first,
+ * from a device counter); 0: one workgroup per work item */
+int ohpic_group_use_persistent(ohpic_group *g, int enable);
 /* steps taken, graph captures, work items of the current table */
 int ohpic_group_stats(const ohpic_group *g, int64_t *steps, int64_t *captures, int64_t *items);
 const char *ohpic_group_last_error(const ohpic_group *g);

@@ -33,7 +33,8 @@ EXPORTS = (
     "ohpic_memory_footprint", "ohpic_estimate_footprint", "ohpic_boris_batch",
     "ohpic_interp_batch", "ohpic_scatter_batch", "ohpic_wrap_batch",
     "ohpic_group_create", "ohpic_group_destroy", "ohpic_group_attach", "ohpic_group_detach",
-    "ohpic_group_size", "ohpic_group_contains", "ohpic_group_step", "ohpic_group_use_graph", "ohpic_group_stats",
+    "ohpic_group_size", "ohpic_group_contains", "ohpic_group_step", "ohpic_group_use_graph",
+    "ohpic_group_use_persistent", "ohpic_group_stats",
     "ohpic_group_last_error", "ohpic_autotune",
 )
 
@@ -159,6 +160,7 @@ def lib():
         "ohpic_group_contains": [vp, vp, C.POINTER(C.c_int)],
         "ohpic_group_step": [vp, C.c_int],
         "ohpic_group_use_graph": [vp, C.c_int],
+        "ohpic_group_use_persistent": [vp, C.c_int],
         "ohpic_group_stats": [vp, i64p, i64p, i64p],
         "ohpic_autotune": [C.POINTER(Config), C.c_int, C.c_int, C.c_int, C.POINTER(TuneResult)],
     }

@@ -461,7 +461,7 @@ int rerun_as_list(ohpic_ctx *ctx, int list_mode)
     timer_begin(ctx, 3);
     int rc;
     HIPCHK(ctx, launch_prepare_next_b(ctx->d_inst, parity_of(ctx), 1, ctx->ng, ctx->ntiles,
-                                      ctx->st));
+                                      nullptr, ctx->st));
     HIPCHK(ctx, launch_particles(list_mode, ctx->cfg.precision, ctx->cfg.weight_order, ctx->T,
                                  ctx->d_inst, ctx->d_items, ctx->nitems, parity_of(ctx), ctx->st));
     if (list_mode == kModeFusedList) HIPCHK(ctx, launch_reduce_ke_b(ctx->d_inst, 1, ctx->st));
@@ -549,7 +549,7 @@ int one_step(ohpic_ctx *ctx)
     if (!ctx->rho_valid && (rc = do_deposit(ctx))) return rc;
     if ((rc = do_solve(ctx, true))) return rc;
     HIPCHK(ctx, launch_prepare_next_b(ctx->d_inst, parity_of(ctx), 1, ctx->ng, ctx->ntiles,
-                                      ctx->st));
+                                      nullptr, ctx->st));
     timer_begin(ctx, 0);
     HIPCHK(ctx, launch_particles(kModeFused, ctx->cfg.precision, ctx->cfg.weight_order, ctx->T,
                                  ctx->d_inst, ctx->d_items, ctx->nitems, parity_of(ctx), ctx->st));
@@ -739,6 +739,9 @@ struct ohpic_group {
     int T[3]{};
     int precision = -1, weight_order = -1;
     bool use_graph = true;
+    bool persistent = true;          // queued particle launch over a resident grid
+    unsigned int *d_queue = nullptr;  // its work counter
+    int queue_blocks = 0;             // resident workgroups of that launch
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     int64_t steps = 0, captures = 0;
@@ -872,6 +875,8 @@ int group_refresh(ohpic_group *g)
             return load(a) > load(b);
         });
         g->nitems = (int)items.size();
+        if (n > 0 && g->queue_blocks == 0)
+            g->queue_blocks = particles_queue_blocks(g->precision, g->weight_order, g->T, g->device);
         if (n > 0) {
             GHIP(g, hipMemcpyAsync(g->d_insts, insts.data(), n * sizeof(InstDev),
                                    hipMemcpyHostToDevice, g->st));
@@ -905,10 +910,17 @@ hipError_t group_enqueue(ohpic_group *g)
     hipError_t e = solve_passes(g->d_insts, g->d_parity, g->d_jobs, n, true, g->max_ng, g->grp,
                                 g->rs, g->max_rows, g->st);
     if (e == hipSuccess)
-        e = launch_prepare_next_b(g->d_insts, g->d_parity, n, g->max_ng, g->max_ntiles, g->st);
-    if (e == hipSuccess)
-        e = launch_particles(kModeFused, g->precision, g->weight_order, g->T, g->d_insts,
-                             g->d_items, g->nitems, g->d_parity, g->st);
+        e = launch_prepare_next_b(g->d_insts, g->d_parity, n, g->max_ng, g->max_ntiles,
+                                  g->persistent ? g->d_queue : nullptr, g->st);
+    if (e == hipSuccess) {
+        if (g->persistent)
+            e = launch_particles_queue(g->precision, g->weight_order, g->T, g->d_insts,
+                                       g->d_items, g->nitems, g->d_parity, g->d_queue,
+                                       g->queue_blocks, g->st);
+        else
+            e = launch_particles(kModeFused, g->precision, g->weight_order, g->T, g->d_insts,
+                                 g->d_items, g->nitems, g->d_parity, g->st);
+    }
     if (e == hipSuccess) e = launch_reduce_ke_b(g->d_insts, n, g->st);
     if (e == hipSuccess) e = launch_finish_b(g->d_insts, g->d_parity, n, g->d_flags, g->st);
     if (e == hipSuccess)
@@ -1894,6 +1906,14 @@ int ohpic_group_create(int device, ohpic_group **out)
         delete g;
         return fail(nullptr, OHPIC_EHIP, "stream creation failed");
     }
+    if (hipMalloc((void **)&g->d_queue, sizeof(unsigned int)) != hipSuccess ||
+        hipMemset(g->d_queue, 0, sizeof(unsigned int)) != hipSuccess) {
+        (void)hipGetLastError();
+        if (g->d_queue) (void)hipFree(g->d_queue);
+        (void)hipStreamDestroy(g->st);
+        delete g;
+        return fail(nullptr, OHPIC_ENOMEM, "group work counter allocation failed");
+    }
     *out = g;
     return OHPIC_OK;
 }
@@ -1910,6 +1930,7 @@ int ohpic_group_destroy(ohpic_group *g)
     gfree(g->d_parity);
     gfree(g->d_flags);
     gfree(g->d_items);
+    gfree(g->d_queue);
     if (g->h_flags) (void)hipHostFree(g->h_flags);
     (void)hipStreamDestroy(g->st);
     delete g;
@@ -1938,6 +1959,7 @@ int ohpic_group_attach(ohpic_group *g, ohpic_ctx *ctx)
         g->precision = ctx->cfg.precision;
         g->weight_order = ctx->cfg.weight_order;
         for (int a = 0; a < 3; ++a) g->T[a] = ctx->T[a];
+        g->queue_blocks = 0;
     }
     GHIP(g, hipSetDevice(g->device));
     GHIP(g, hipStreamSynchronize(ctx->st));
@@ -1981,6 +2003,16 @@ int ohpic_group_use_graph(ohpic_group *g, int enable)
 {
     if (!g) return fail(nullptr, OHPIC_EINVAL, "group is NULL");
     g->use_graph = enable != 0;
+    (void)hipSetDevice(g->device);
+    (void)hipStreamSynchronize(g->st);
+    drop_graph(g);
+    return OHPIC_OK;
+}
+
+int ohpic_group_use_persistent(ohpic_group *g, int enable)
+{
+    if (!g) return fail(nullptr, OHPIC_EINVAL, "group is NULL");
+    g->persistent = enable != 0;
     (void)hipSetDevice(g->device);
     (void)hipStreamSynchronize(g->st);
     drop_graph(g);

@@ -150,8 +150,11 @@ __global__ void k_rho_from_acc_b(const InstDev *__restrict__ insts, const int *_
         rho[i] = (double)(long long)acc[i] * scale;
 }
 
-__global__ void k_prepare_next_b(const InstDev *__restrict__ insts, const int *__restrict__ parity)
+__global__ void k_prepare_next_b(const InstDev *__restrict__ insts, const int *__restrict__ parity,
+                                 unsigned int *__restrict__ queue)
 {
+    // work counter of the queued particle launch that follows
+    if (queue && blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) *queue = 0u;
     const InstDev *I = insts + blockIdx.y;
     const int nb = parity[blockIdx.y] ^ 1;
     const int64_t n = (int64_t)I->n[0] * I->n[1] * I->n[2];
@@ -298,12 +301,13 @@ hipError_t launch_efield_b(const InstDev *insts, int ninst, int max_rows, hipStr
 }
 
 hipError_t launch_prepare_next_b(const InstDev *insts, const int *parity, int ninst,
-                                 int64_t max_ng, int max_ntiles, hipStream_t st)
+                                 int64_t max_ng, int max_ntiles, unsigned int *queue,
+                                 hipStream_t st)
 {
     if (ninst <= 0) return hipSuccess;
     const int64_t n = max_ng > max_ntiles ? max_ng : max_ntiles;
     hipLaunchKernelGGL(k_prepare_next_b, dim3(grid_cap(n, 4096), ninst), dim3(256), 0, st, insts,
-                       parity);
+                       parity, queue);
     return hipGetLastError();
 }
 

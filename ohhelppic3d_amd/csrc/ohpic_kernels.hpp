@@ -18,7 +18,10 @@ enum ParticleMode : int {
     // list [0, count[par^1][0]) of buffer par^1 instead of a bin (the retry
     // after a bin overflowed)
     kModeFusedList = 4,
-    kModeBoundaryList = 5
+    kModeBoundaryList = 5,
+    // as kModeFused over a resident grid: workgroups claim work items from a
+    // device counter (launch_particles_queue)
+    kModeFusedQueue = 6
 };
 
 struct Pcg32Dev {
@@ -43,10 +46,24 @@ hipError_t launch_bins_to_list(const InstDev *insts, int inst, int s, int src, i
 hipError_t launch_cell_counts(const InstDev *insts, int inst, int s, int par, int ntiles,
                               int32_t *counts, hipStream_t st);
 
+// The fused step as one resident grid of `blocks` workgroups that claim the
+// nitems work items in list order from *queue (which must be 0 at launch:
+// launch_prepare_next_b zeroes it).  particles_queue_blocks: workgroups of that kernel
+// resident on the whole device.
+hipError_t launch_particles_queue(int precision, int weight_order, const int T[3],
+                                  const InstDev *insts, const WorkItem *items, int nitems,
+                                  const int *parity, unsigned int *queue, int blocks,
+                                  hipStream_t st);
+int particles_queue_blocks(int precision, int weight_order, const int T[3], int device);
+
 // mixed-precision particles (ohpic_particles_mixed.hip)
 hipError_t launch_particles_mixed(int mode, int weight_order, const int T[3],
                                   const InstDev *insts, const WorkItem *items, int nitems,
                                   const int *parity, hipStream_t st);
+hipError_t launch_particles_mixed_queue(int weight_order, const int T[3], const InstDev *insts,
+                                        const WorkItem *items, int nitems, const int *parity,
+                                        unsigned int *queue, int blocks, hipStream_t st);
+int particles_mixed_queue_blocks(int weight_order, const int T[3]);
 hipError_t launch_bin_count_mx(const InstDev *insts, int inst, int s, int lbuf, int64_t n,
                                const int T[3], hipStream_t st);
 hipError_t launch_bin_scatter_mx(const InstDev *insts, int inst, int s, int lbuf, int64_t n,
@@ -69,9 +86,11 @@ hipError_t launch_rho_from_acc_b(const InstDev *insts, const int *parity, int ni
 // node E from phi + block partials of sum rho*phi, sum E^2, sum rho; then the
 // partials are reduced into diag[kDiagWe..kDiagQnet]
 hipError_t launch_efield_b(const InstDev *insts, int ninst, int max_rows, hipStream_t st);
-// zero the next buffer's bin words and charge grid and the flags
+// zero the next buffer's bin words and charge grid, the flags, and *queue
+// (if not null: the counter of a following launch_particles_queue)
 hipError_t launch_prepare_next_b(const InstDev *insts, const int *parity, int ninst,
-                                 int64_t max_ng, int max_ntiles, hipStream_t st);
+                                 int64_t max_ng, int max_ntiles, unsigned int *queue,
+                                 hipStream_t st);
 // diag[kDiagKeOld..] := sums of ke_partial over the instance's items
 hipError_t launch_reduce_ke_b(const InstDev *insts, int ninst, hipStream_t st);
 // out[i * kFlagSlots + k] := flags[k] of instance i; parity[i] ^= 1 unless

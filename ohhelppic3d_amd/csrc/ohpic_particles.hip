@@ -101,12 +101,17 @@ __device__ __forceinline__ void flush_leavers(LeaveBuf &lb, int cnt, const SpecD
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-template <int TX, int TY, int TZ, int WO, int MODE>
+template <int TX, int TY, int TZ, int WO, int MODE_>
 __global__ void __launch_bounds__(kBlock)
     __attribute__((amdgpu_waves_per_eu(OHPIC_PUSH_MIN_WAVES, 8)))
     k_particles_fp64(const InstDev *__restrict__ insts, const WorkItem *__restrict__ items,
-                     const int *__restrict__ parity)
+                     const int *__restrict__ parity, unsigned int *__restrict__ queue,
+                     int nitems)
 {
+    // kModeFusedQueue: the fused step over a resident grid, each workgroup
+    // claiming work items in list order from `queue` until none is left
+    constexpr bool QUEUE = (MODE_ == kModeFusedQueue);
+    constexpr int MODE = QUEUE ? (int)kModeFused : MODE_;
     constexpr bool LIST = (MODE == kModeFusedList || MODE == kModeBoundaryList);
     constexpr bool PUSH = (MODE == kModeFused || MODE == kModeGatherPush || MODE == kModeFusedList);
     constexpr bool REBIN = (MODE == kModeFused || MODE == kModeBoundary || LIST);
@@ -120,292 +125,307 @@ __global__ void __launch_bounds__(kBlock)
     __shared__ double sRed[kWaves];
     __shared__ int sFront, sFrontLimit;
 
-    const WorkItem it = items[blockIdx.x];
-    const InstDev *__restrict__ I = insts + it.inst;
-    const int par = parity[it.inst];
-    const int nx = I->n[0], ny = I->n[1], nz = I->n[2];
-    const int ntx = I->nt[0], nty = I->nt[1];
-    const int tile = it.tile;
-    const int tx = tile % ntx, ty = (tile / ntx) % nty, tz = tile / (ntx * nty);
-    const int lox = tx * TX, loy = ty * TY, loz = tz * TZ;
-    const int tid = threadIdx.x;
-    const int wv = tid >> 6;
-    const bool ids = I->track_ids != 0;
-    const bool excl = it.nchunks == 1;  // this workgroup alone fills the tile's front runs
-    unsigned long long *__restrict__ flags = I->flags;
-
-    if (PUSH) {
-        const double *__restrict__ ex = I->e[0];
-        const double *__restrict__ ey = I->e[1];
-        const double *__restrict__ ez = I->e[2];
-        for (int l = tid; l < NE; l += kBlock) {
-            const int lx = l % EX, ly = (l / EX) % EY, lz = l / (EX * EY);
-            const int gx = (lox + lx) % nx, gy = (loy + ly) % ny, gz = (loz + lz) % nz;
-            const int64_t g = gx + (int64_t)nx * (gy + (int64_t)ny * gz);
-            sE[l] = gload(ex, g);
-            sE[NE + l] = gload(ey, g);
-            sE[2 * NE + l] = gload(ez, g);
-        }
-    }
-    if (DEP)
-        for (int l = tid; l < NR; l += kBlock) sR[l] = 0ull;
-    if (tid < kWaves) sLcnt[tid] = 0;
-    __syncthreads();
-
-    const double dt = I->dt;
-    const double dt2 = dt * 0.5;
-    const double bx = I->b0[0], by = I->b0[1], bz = I->b0[2];
-    const int nbuf = par ^ 1;
-    const unsigned long long lanemask_lt = (1ull << lane_id()) - 1ull;
-    unsigned long long *__restrict__ racc =
-        I->rho_acc[(MODE == kModeFused || MODE == kModeFusedList) ? nbuf : par];
-
-    for (int s = 0; s < I->nspecies; ++s) {
-        const SpecDev &S = I->sp[s];
-        const int cap = gload(S.bin_cap, tile);
-        const unsigned long long word = gload(S.count[par], tile);
-        const int F = min(bin_front(word), cap);
-        const int B = min(bin_back(word), cap - F);
-        const int cnt = F + B;
-        const int64_t base = gload(S.bin_base, tile);
-        int per = (cnt + it.nchunks - 1) / it.nchunks;
-        per = (per + 63) & ~63;
-        const int start = it.chunk * per;
-        const int end = min(cnt, start + per);
-        if (REBIN) {
-            if (tid == 0) {
-                int lim = cap;
-                if (!LIST && excl && cnt > 0) {
-                    // reserve the whole front run: arrivals from neighbours stay behind it
-                    const unsigned long long old =
-                        gatomic_add(&S.count[nbuf][tile], (unsigned long long)cnt);
-                    lim = cap - bin_back(old);
-                }
-                sFront = 0;
-                sFrontLimit = lim;
-            }
+    __shared__ int sItem;
+    for (int iter = 0;; ++iter) {
+        int item;
+        if (QUEUE) {
+            if (threadIdx.x == 0) sItem = (int)atomicAdd(queue, 1u);
             __syncthreads();
+            item = sItem;
+            if (item >= nitems) break;
+        } else {
+            if (iter > 0) break;
+            item = blockIdx.x;
         }
-        const double qm = S.qm;
-        const double qfix = S.qfix;
-        // Boris constants for the uniform B (t = qm B dt/2, s = 2t/(1+|t|^2))
-        const double t0 = qm * bx * dt2, t1 = qm * by * dt2, t2 = qm * bz * dt2;
-        const double tt = t0 * t0 + t1 * t1 + t2 * t2;
-        const double s0 = 2 * t0 / (1 + tt), s1 = 2 * t1 / (1 + tt), s2 = 2 * t2 / (1 + tt);
-        double keo = 0.0, ken = 0.0;
+        const WorkItem it = items[item];
+        const InstDev *__restrict__ I = insts + it.inst;
+        const int par = parity[it.inst];
+        const int nx = I->n[0], ny = I->n[1], nz = I->n[2];
+        const int ntx = I->nt[0], nty = I->nt[1];
+        const int tile = it.tile;
+        const int tx = tile % ntx, ty = (tile / ntx) % nty, tz = tile / (ntx * nty);
+        const int lox = tx * TX, loy = ty * TY, loz = tz * TZ;
+        const int tid = threadIdx.x;
+        const int wv = tid >> 6;
+        const bool ids = I->track_ids != 0;
+        const bool excl = it.nchunks == 1;  // this workgroup alone fills the tile's front runs
+        unsigned long long *__restrict__ flags = I->flags;
 
-        P64 nextq{0, 0, 0, 0, 0, 0, 0};
-        if (start + tid < end) nextq = load_p64(S, par, base + bin_slot(start + tid, F, B, cap), ids);
-        for (int p0 = start; p0 < end; p0 += kBlock) {
-            const int p = p0 + tid;
-            const bool active = p < end;
-            P64 q = nextq;
-            // prefetch the next iteration's particle while this one is processed
-            if (p + kBlock < end)
-                nextq = load_p64(S, par, base + bin_slot(p + kBlock, F, B, cap), ids);
-            if (PUSH && active) {
-                const int ix = (int)q.x, iy = (int)q.y, iz = (int)q.z;
-                double wx0, wx1, wy0, wy1, wz0, wz1;
-                cic_w<WO>(q.x - (double)ix, wx0, wx1);
-                cic_w<WO>(q.y - (double)iy, wy0, wy1);
-                cic_w<WO>(q.z - (double)iz, wz0, wz1);
-                // binned particles lie in the tile; the clamp only guards LDS
-                const int lx = min(max(ix - lox, 0), TX - 1);
-                const int ly = min(max(iy - loy, 0), TY - 1);
-                const int lz = min(max(iz - loz, 0), TZ - 1);
-                const int l0 = lx + EX * (ly + EY * lz);
-                double e0 = 0.0, e1 = 0.0, e2 = 0.0;
-                const double wx[2] = {wx0, wx1}, wy[2] = {wy0, wy1}, wz[2] = {wz0, wz1};
-#pragma unroll
-                for (int i = 0; i < 2; ++i)
-#pragma unroll
-                    for (int j = 0; j < 2; ++j)
-#pragma unroll
-                        for (int k = 0; k < 2; ++k) {
-                            const int l = l0 + i + EX * (j + EY * k);
-                            const double w = wx[i] * wy[j] * wz[k];
-                            e0 += sE[l] * w;
-                            e1 += sE[NE + l] * w;
-                            e2 += sE[2 * NE + l] * w;
-                        }
-                keo += q.vx * q.vx + q.vy * q.vy + q.vz * q.vz;
-                // Boris (particle_mover_boris.f90:42-62)
-                const double um0 = q.vx + qm * e0 * dt2;
-                const double um1 = q.vy + qm * e1 * dt2;
-                const double um2 = q.vz + qm * e2 * dt2;
-                const double ua0 = um0 + (um1 * t2 - um2 * t1);
-                const double ua1 = um1 + (um2 * t0 - um0 * t2);
-                const double ua2 = um2 + (um0 * t1 - um1 * t0);
-                const double up0 = um0 + (ua1 * s2 - ua2 * s1);
-                const double up1 = um1 + (ua2 * s0 - ua0 * s2);
-                const double up2 = um2 + (ua0 * s1 - ua1 * s0);
-                q.vx = up0 + qm * e0 * dt2;
-                q.vy = up1 + qm * e1 * dt2;
-                q.vz = up2 + qm * e2 * dt2;
-                ken += q.vx * q.vx + q.vy * q.vy + q.vz * q.vz;
-                q.x = q.x + q.vx * dt;
-                q.y = q.y + q.vy * dt;
-                q.z = q.z + q.vz * dt;
+        if (PUSH) {
+            const double *__restrict__ ex = I->e[0];
+            const double *__restrict__ ey = I->e[1];
+            const double *__restrict__ ez = I->e[2];
+            for (int l = tid; l < NE; l += kBlock) {
+                const int lx = l % EX, ly = (l / EX) % EY, lz = l / (EX * EY);
+                const int gx = (lox + lx) % nx, gy = (loy + ly) % ny, gz = (loz + lz) % nz;
+                const int64_t g = gx + (int64_t)nx * (gy + (int64_t)ny * gz);
+                sE[l] = gload(ex, g);
+                sE[NE + l] = gload(ey, g);
+                sE[2 * NE + l] = gload(ez, g);
             }
-            if (MODE == kModeGatherPush) {
-                if (active) store_p64(S, par, base + bin_slot(p, F, B, cap), q, ids);
-                continue;
-            }
-            int ntile = tile;
-            if (REBIN && active) {
-                q.x = apply_periodic(apply_periodic(q.x, nx), nx);
-                q.y = apply_periodic(apply_periodic(q.y, ny), ny);
-                q.z = apply_periodic(apply_periodic(q.z, nz), nz);
-                if (!(q.x >= 0.0 && q.x < nx && q.y >= 0.0 && q.y < ny && q.z >= 0.0 &&
-                      q.z < nz)) {
-                    gatomic_add(&flags[kFlagRange], 1ull);
-                    q.x = fmin(fmax(q.x - floor(q.x / nx) * nx, 0.0), nx - 1e-9);
-                    q.y = fmin(fmax(q.y - floor(q.y / ny) * ny, 0.0), ny - 1e-9);
-                    q.z = fmin(fmax(q.z - floor(q.z / nz) * nz, 0.0), nz - 1e-9);
-                }
-                const int cxi = (int)q.x, cyi = (int)q.y, czi = (int)q.z;
-                ntile = cxi / TX + ntx * (cyi / TY + nty * (czi / TZ));
-            }
-            if (DEP && active) {
-                const int ix = (int)q.x, iy = (int)q.y, iz = (int)q.z;
-                double wx0, wx1, wy0, wy1, wz0, wz1;
-                cic_w<WO>(q.x - (double)ix, wx0, wx1);
-                cic_w<WO>(q.y - (double)iy, wy0, wy1);
-                cic_w<WO>(q.z - (double)iz, wz0, wz1);
-                const double wx[2] = {wx0, wx1}, wy[2] = {wy0, wy1}, wz[2] = {wz0, wz1};
-                const int dx = wrap_index(ix - lox + 1, nx);
-                const int dy = wrap_index(iy - loy + 1, ny);
-                const int dz = wrap_index(iz - loz + 1, nz);
-                if (dx <= TX + 1 && dy <= TY + 1 && dz <= TZ + 1) {
-                    const int l0 = dx + RX * (dy + RY * dz);
-#pragma unroll
-                    for (int i = 0; i < 2; ++i)
-#pragma unroll
-                        for (int j = 0; j < 2; ++j)
-#pragma unroll
-                            for (int k = 0; k < 2; ++k) {
-                                const long long v = fix_round(wx[i] * wy[j] * wz[k], qfix);
-                                atomicAdd(&sR[l0 + i + RX * (j + RY * k)],
-                                          (unsigned long long)v);
-                            }
-                } else {
-                    gatomic_add(&flags[kFlagFast], 1ull);
-#pragma unroll
-                    for (int i = 0; i < 2; ++i)
-#pragma unroll
-                        for (int j = 0; j < 2; ++j)
-#pragma unroll
-                            for (int k = 0; k < 2; ++k) {
-                                const int gx = (ix + i) % nx, gy = (iy + j) % ny,
-                                          gz = (iz + k) % nz;
-                                const long long v = fix_round(wx[i] * wy[j] * wz[k], qfix);
-                                gatomic_add(&racc[gx + (int64_t)nx * (gy + (int64_t)ny * gz)],
-                                          (unsigned long long)v);
-                            }
-                }
-            }
-            if (LIST) {
-                // retry path: every particle to the species' contiguous list
-                const unsigned long long m = __ballot(active);
-                if (m) {
-                    const int leader = __ffsll((unsigned long long)m) - 1;
-                    unsigned long long b = 0;
-                    if (lane_id() == leader)
-                        b = gatomic_add(&S.count[nbuf][0], (unsigned long long)__popcll(m));
-                    b = __shfl(b, leader, 64);
-                    if (active) store_p64(S, nbuf, (int64_t)b + __popcll(m & lanemask_lt), q, ids);
-                }
-            } else if (REBIN) {
-                // particles that stayed: front run of this tile's bin
-                const bool stay = active && ntile == tile;
-                const unsigned long long m = __ballot(stay);
-                if (m) {
-                    const int leader = __ffsll((unsigned long long)m) - 1;
-                    const int nst = (int)__popcll(m);
-                    int b = 0, lim = 0;
-                    if (excl) {
-                        if (lane_id() == leader) b = atomicAdd(&sFront, nst);
-                        lim = sFrontLimit;
-                    } else if (lane_id() == leader) {
+        }
+        if (DEP)
+            for (int l = tid; l < NR; l += kBlock) sR[l] = 0ull;
+        if (tid < kWaves) sLcnt[tid] = 0;
+        __syncthreads();
+
+        const double dt = I->dt;
+        const double dt2 = dt * 0.5;
+        const double bx = I->b0[0], by = I->b0[1], bz = I->b0[2];
+        const int nbuf = par ^ 1;
+        const unsigned long long lanemask_lt = (1ull << lane_id()) - 1ull;
+        unsigned long long *__restrict__ racc =
+            I->rho_acc[(MODE == kModeFused || MODE == kModeFusedList) ? nbuf : par];
+
+        for (int s = 0; s < I->nspecies; ++s) {
+            const SpecDev &S = I->sp[s];
+            const int cap = gload(S.bin_cap, tile);
+            const unsigned long long word = gload(S.count[par], tile);
+            const int F = min(bin_front(word), cap);
+            const int B = min(bin_back(word), cap - F);
+            const int cnt = F + B;
+            const int64_t base = gload(S.bin_base, tile);
+            int per = (cnt + it.nchunks - 1) / it.nchunks;
+            per = (per + 63) & ~63;
+            const int start = it.chunk * per;
+            const int end = min(cnt, start + per);
+            if (REBIN) {
+                if (tid == 0) {
+                    int lim = cap;
+                    if (!LIST && excl && cnt > 0) {
+                        // reserve the whole front run: arrivals from neighbours stay behind it
                         const unsigned long long old =
-                            gatomic_add(&S.count[nbuf][tile], (unsigned long long)nst);
-                        b = bin_front(old);
+                            gatomic_add(&S.count[nbuf][tile], (unsigned long long)cnt);
                         lim = cap - bin_back(old);
                     }
-                    b = __shfl(b, leader, 64);
-                    lim = __shfl(lim, leader, 64);
-                    if (stay) {
-                        const int slot = b + (int)__popcll(m & lanemask_lt);
-                        if (slot < lim)
-                            store_p64(S, nbuf, base + slot, q, ids);
-                        else
-                            gatomic_add(&flags[kFlagOverflow], 1ull);
-                    }
+                    sFront = 0;
+                    sFrontLimit = lim;
                 }
-                // particles that left: per-wave LDS buffer, routed in batches
-                const bool leave = active && ntile != tile;
-                const unsigned long long lm = __ballot(leave);
-                if (lm) {
-                    const int nl = (int)__popcll(lm);
-                    int c = sLcnt[wv];
-                    if (c + nl > kLeaveCap) {
-                        flush_leavers(sLeave[wv], c, S, nbuf, ids, flags);
-                        c = 0;
-                    }
-                    if (nl <= kLeaveCap) {
-                        if (leave) {
-                            const int slot = c + (int)__popcll(lm & lanemask_lt);
-                            LeaveBuf &lb = sLeave[wv];
-                            lb.c[0][slot] = q.x;
-                            lb.c[1][slot] = q.y;
-                            lb.c[2][slot] = q.z;
-                            lb.c[3][slot] = q.vx;
-                            lb.c[4][slot] = q.vy;
-                            lb.c[5][slot] = q.vz;
-                            lb.id[slot] = q.id;
-                            lb.dest[slot] = ntile;
-                        }
-                        c += nl;
-                    } else if (leave) {
-                        route_back(S, nbuf, ntile, q, ids, flags);
-                    }
-                    if (lane_id() == 0) sLcnt[wv] = c;
-                }
+                __syncthreads();
             }
-        }
-        if (REBIN && !LIST) {
-            flush_leavers(sLeave[wv], sLcnt[wv], S, nbuf, ids, flags);
-            __syncthreads();
-            if (tid < kWaves) sLcnt[tid] = 0;
-            if (tid == 0 && excl && cnt > sFront) {
-                // release the reserved but unused part of the front run
-                gatomic_add(&S.count[nbuf][tile],
-                          (unsigned long long)(-(long long)(cnt - sFront)));
-            }
-            __syncthreads();
-        }
-        if (PUSH) {
-            const double so = block_sum(keo, sRed);
-            const double sn = block_sum(ken, sRed);
-            if (tid == 0) {
-                gstore(I->ke_partial, (int64_t)it.slot * kKeSlots + s, 0.5 * S.mass * so);
-                gstore(I->ke_partial, (int64_t)it.slot * kKeSlots + kMaxSpecies + s,
-                       0.5 * S.mass * sn);
-            }
-        }
-    }
+            const double qm = S.qm;
+            const double qfix = S.qfix;
+            // Boris constants for the uniform B (t = qm B dt/2, s = 2t/(1+|t|^2))
+            const double t0 = qm * bx * dt2, t1 = qm * by * dt2, t2 = qm * bz * dt2;
+            const double tt = t0 * t0 + t1 * t1 + t2 * t2;
+            const double s0 = 2 * t0 / (1 + tt), s1 = 2 * t1 / (1 + tt), s2 = 2 * t2 / (1 + tt);
+            double keo = 0.0, ken = 0.0;
 
-    if (DEP) {
-        __syncthreads();
-        for (int l = tid; l < NR; l += kBlock) {
-            const unsigned long long v = sR[l];
-            if (v == 0ull) continue;
-            const int lx = l % RX, ly = (l / RX) % RY, lz = l / (RX * RY);
-            const int gx = ((lox - 1 + lx) % nx + nx) % nx;
-            const int gy = ((loy - 1 + ly) % ny + ny) % ny;
-            const int gz = ((loz - 1 + lz) % nz + nz) % nz;
-            gatomic_add(&racc[gx + (int64_t)nx * (gy + (int64_t)ny * gz)], v);
+            P64 nextq{0, 0, 0, 0, 0, 0, 0};
+            if (start + tid < end) nextq = load_p64(S, par, base + bin_slot(start + tid, F, B, cap), ids);
+            for (int p0 = start; p0 < end; p0 += kBlock) {
+                const int p = p0 + tid;
+                const bool active = p < end;
+                P64 q = nextq;
+                // prefetch the next iteration's particle while this one is processed
+                if (p + kBlock < end)
+                    nextq = load_p64(S, par, base + bin_slot(p + kBlock, F, B, cap), ids);
+                if (PUSH && active) {
+                    const int ix = (int)q.x, iy = (int)q.y, iz = (int)q.z;
+                    double wx0, wx1, wy0, wy1, wz0, wz1;
+                    cic_w<WO>(q.x - (double)ix, wx0, wx1);
+                    cic_w<WO>(q.y - (double)iy, wy0, wy1);
+                    cic_w<WO>(q.z - (double)iz, wz0, wz1);
+                    // binned particles lie in the tile; the clamp only guards LDS
+                    const int lx = min(max(ix - lox, 0), TX - 1);
+                    const int ly = min(max(iy - loy, 0), TY - 1);
+                    const int lz = min(max(iz - loz, 0), TZ - 1);
+                    const int l0 = lx + EX * (ly + EY * lz);
+                    double e0 = 0.0, e1 = 0.0, e2 = 0.0;
+                    const double wx[2] = {wx0, wx1}, wy[2] = {wy0, wy1}, wz[2] = {wz0, wz1};
+    #pragma unroll
+                    for (int i = 0; i < 2; ++i)
+    #pragma unroll
+                        for (int j = 0; j < 2; ++j)
+    #pragma unroll
+                            for (int k = 0; k < 2; ++k) {
+                                const int l = l0 + i + EX * (j + EY * k);
+                                const double w = wx[i] * wy[j] * wz[k];
+                                e0 += sE[l] * w;
+                                e1 += sE[NE + l] * w;
+                                e2 += sE[2 * NE + l] * w;
+                            }
+                    keo += q.vx * q.vx + q.vy * q.vy + q.vz * q.vz;
+                    // Boris (particle_mover_boris.f90:42-62)
+                    const double um0 = q.vx + qm * e0 * dt2;
+                    const double um1 = q.vy + qm * e1 * dt2;
+                    const double um2 = q.vz + qm * e2 * dt2;
+                    const double ua0 = um0 + (um1 * t2 - um2 * t1);
+                    const double ua1 = um1 + (um2 * t0 - um0 * t2);
+                    const double ua2 = um2 + (um0 * t1 - um1 * t0);
+                    const double up0 = um0 + (ua1 * s2 - ua2 * s1);
+                    const double up1 = um1 + (ua2 * s0 - ua0 * s2);
+                    const double up2 = um2 + (ua0 * s1 - ua1 * s0);
+                    q.vx = up0 + qm * e0 * dt2;
+                    q.vy = up1 + qm * e1 * dt2;
+                    q.vz = up2 + qm * e2 * dt2;
+                    ken += q.vx * q.vx + q.vy * q.vy + q.vz * q.vz;
+                    q.x = q.x + q.vx * dt;
+                    q.y = q.y + q.vy * dt;
+                    q.z = q.z + q.vz * dt;
+                }
+                if (MODE == kModeGatherPush) {
+                    if (active) store_p64(S, par, base + bin_slot(p, F, B, cap), q, ids);
+                    continue;
+                }
+                int ntile = tile;
+                if (REBIN && active) {
+                    q.x = apply_periodic(apply_periodic(q.x, nx), nx);
+                    q.y = apply_periodic(apply_periodic(q.y, ny), ny);
+                    q.z = apply_periodic(apply_periodic(q.z, nz), nz);
+                    if (!(q.x >= 0.0 && q.x < nx && q.y >= 0.0 && q.y < ny && q.z >= 0.0 &&
+                          q.z < nz)) {
+                        gatomic_add(&flags[kFlagRange], 1ull);
+                        q.x = fmin(fmax(q.x - floor(q.x / nx) * nx, 0.0), nx - 1e-9);
+                        q.y = fmin(fmax(q.y - floor(q.y / ny) * ny, 0.0), ny - 1e-9);
+                        q.z = fmin(fmax(q.z - floor(q.z / nz) * nz, 0.0), nz - 1e-9);
+                    }
+                    const int cxi = (int)q.x, cyi = (int)q.y, czi = (int)q.z;
+                    ntile = cxi / TX + ntx * (cyi / TY + nty * (czi / TZ));
+                }
+                if (DEP && active) {
+                    const int ix = (int)q.x, iy = (int)q.y, iz = (int)q.z;
+                    double wx0, wx1, wy0, wy1, wz0, wz1;
+                    cic_w<WO>(q.x - (double)ix, wx0, wx1);
+                    cic_w<WO>(q.y - (double)iy, wy0, wy1);
+                    cic_w<WO>(q.z - (double)iz, wz0, wz1);
+                    const double wx[2] = {wx0, wx1}, wy[2] = {wy0, wy1}, wz[2] = {wz0, wz1};
+                    const int dx = wrap_index(ix - lox + 1, nx);
+                    const int dy = wrap_index(iy - loy + 1, ny);
+                    const int dz = wrap_index(iz - loz + 1, nz);
+                    if (dx <= TX + 1 && dy <= TY + 1 && dz <= TZ + 1) {
+                        const int l0 = dx + RX * (dy + RY * dz);
+    #pragma unroll
+                        for (int i = 0; i < 2; ++i)
+    #pragma unroll
+                            for (int j = 0; j < 2; ++j)
+    #pragma unroll
+                                for (int k = 0; k < 2; ++k) {
+                                    const long long v = fix_round(wx[i] * wy[j] * wz[k], qfix);
+                                    atomicAdd(&sR[l0 + i + RX * (j + RY * k)],
+                                              (unsigned long long)v);
+                                }
+                    } else {
+                        gatomic_add(&flags[kFlagFast], 1ull);
+    #pragma unroll
+                        for (int i = 0; i < 2; ++i)
+    #pragma unroll
+                            for (int j = 0; j < 2; ++j)
+    #pragma unroll
+                                for (int k = 0; k < 2; ++k) {
+                                    const int gx = (ix + i) % nx, gy = (iy + j) % ny,
+                                              gz = (iz + k) % nz;
+                                    const long long v = fix_round(wx[i] * wy[j] * wz[k], qfix);
+                                    gatomic_add(&racc[gx + (int64_t)nx * (gy + (int64_t)ny * gz)],
+                                              (unsigned long long)v);
+                                }
+                    }
+                }
+                if (LIST) {
+                    // retry path: every particle to the species' contiguous list
+                    const unsigned long long m = __ballot(active);
+                    if (m) {
+                        const int leader = __ffsll((unsigned long long)m) - 1;
+                        unsigned long long b = 0;
+                        if (lane_id() == leader)
+                            b = gatomic_add(&S.count[nbuf][0], (unsigned long long)__popcll(m));
+                        b = __shfl(b, leader, 64);
+                        if (active) store_p64(S, nbuf, (int64_t)b + __popcll(m & lanemask_lt), q, ids);
+                    }
+                } else if (REBIN) {
+                    // particles that stayed: front run of this tile's bin
+                    const bool stay = active && ntile == tile;
+                    const unsigned long long m = __ballot(stay);
+                    if (m) {
+                        const int leader = __ffsll((unsigned long long)m) - 1;
+                        const int nst = (int)__popcll(m);
+                        int b = 0, lim = 0;
+                        if (excl) {
+                            if (lane_id() == leader) b = atomicAdd(&sFront, nst);
+                            lim = sFrontLimit;
+                        } else if (lane_id() == leader) {
+                            const unsigned long long old =
+                                gatomic_add(&S.count[nbuf][tile], (unsigned long long)nst);
+                            b = bin_front(old);
+                            lim = cap - bin_back(old);
+                        }
+                        b = __shfl(b, leader, 64);
+                        lim = __shfl(lim, leader, 64);
+                        if (stay) {
+                            const int slot = b + (int)__popcll(m & lanemask_lt);
+                            if (slot < lim)
+                                store_p64(S, nbuf, base + slot, q, ids);
+                            else
+                                gatomic_add(&flags[kFlagOverflow], 1ull);
+                        }
+                    }
+                    // particles that left: per-wave LDS buffer, routed in batches
+                    const bool leave = active && ntile != tile;
+                    const unsigned long long lm = __ballot(leave);
+                    if (lm) {
+                        const int nl = (int)__popcll(lm);
+                        int c = sLcnt[wv];
+                        if (c + nl > kLeaveCap) {
+                            flush_leavers(sLeave[wv], c, S, nbuf, ids, flags);
+                            c = 0;
+                        }
+                        if (nl <= kLeaveCap) {
+                            if (leave) {
+                                const int slot = c + (int)__popcll(lm & lanemask_lt);
+                                LeaveBuf &lb = sLeave[wv];
+                                lb.c[0][slot] = q.x;
+                                lb.c[1][slot] = q.y;
+                                lb.c[2][slot] = q.z;
+                                lb.c[3][slot] = q.vx;
+                                lb.c[4][slot] = q.vy;
+                                lb.c[5][slot] = q.vz;
+                                lb.id[slot] = q.id;
+                                lb.dest[slot] = ntile;
+                            }
+                            c += nl;
+                        } else if (leave) {
+                            route_back(S, nbuf, ntile, q, ids, flags);
+                        }
+                        if (lane_id() == 0) sLcnt[wv] = c;
+                    }
+                }
+            }
+            if (REBIN && !LIST) {
+                flush_leavers(sLeave[wv], sLcnt[wv], S, nbuf, ids, flags);
+                __syncthreads();
+                if (tid < kWaves) sLcnt[tid] = 0;
+                if (tid == 0 && excl && cnt > sFront) {
+                    // release the reserved but unused part of the front run
+                    gatomic_add(&S.count[nbuf][tile],
+                              (unsigned long long)(-(long long)(cnt - sFront)));
+                }
+                __syncthreads();
+            }
+            if (PUSH) {
+                const double so = block_sum(keo, sRed);
+                const double sn = block_sum(ken, sRed);
+                if (tid == 0) {
+                    gstore(I->ke_partial, (int64_t)it.slot * kKeSlots + s, 0.5 * S.mass * so);
+                    gstore(I->ke_partial, (int64_t)it.slot * kKeSlots + kMaxSpecies + s,
+                           0.5 * S.mass * sn);
+                }
+            }
         }
+
+        if (DEP) {
+            __syncthreads();
+            for (int l = tid; l < NR; l += kBlock) {
+                const unsigned long long v = sR[l];
+                if (v == 0ull) continue;
+                const int lx = l % RX, ly = (l / RX) % RY, lz = l / (RX * RY);
+                const int gx = ((lox - 1 + lx) % nx + nx) % nx;
+                const int gy = ((loy - 1 + ly) % ny + ny) % ny;
+                const int gz = ((loz - 1 + lz) % nz + nz) % nz;
+                gatomic_add(&racc[gx + (int64_t)nx * (gy + (int64_t)ny * gz)], v);
+            }
+        }
+        // the next item reuses the LDS blocks and sItem
+        __syncthreads();
     }
 }
 
@@ -521,35 +541,46 @@ static hipError_t launch_tile(int mode, const InstDev *insts, const WorkItem *it
                               int nitems, const int *parity, hipStream_t st)
 {
     dim3 g(nitems), b(kBlock);
+#define OHPIC_FP_CASE(M)                                                                     \
+    case M:                                                                                  \
+        hipLaunchKernelGGL((k_particles_fp64<TX, TY, TZ, WO, M>), g, b, 0, st, insts, items, \
+                           parity, nullptr, nitems);                                         \
+        break;
     switch (mode) {
-    case kModeFused:
-        hipLaunchKernelGGL((k_particles_fp64<TX, TY, TZ, WO, kModeFused>), g, b, 0, st, insts,
-                           items, parity);
-        break;
-    case kModeDeposit:
-        hipLaunchKernelGGL((k_particles_fp64<TX, TY, TZ, WO, kModeDeposit>), g, b, 0, st,
-                           insts, items, parity);
-        break;
-    case kModeGatherPush:
-        hipLaunchKernelGGL((k_particles_fp64<TX, TY, TZ, WO, kModeGatherPush>), g, b, 0, st,
-                           insts, items, parity);
-        break;
-    case kModeBoundary:
-        hipLaunchKernelGGL((k_particles_fp64<TX, TY, TZ, WO, kModeBoundary>), g, b, 0, st,
-                           insts, items, parity);
-        break;
-    case kModeFusedList:
-        hipLaunchKernelGGL((k_particles_fp64<TX, TY, TZ, WO, kModeFusedList>), g, b, 0, st,
-                           insts, items, parity);
-        break;
-    case kModeBoundaryList:
-        hipLaunchKernelGGL((k_particles_fp64<TX, TY, TZ, WO, kModeBoundaryList>), g, b, 0, st,
-                           insts, items, parity);
-        break;
+        OHPIC_FP_CASE(kModeFused)
+        OHPIC_FP_CASE(kModeDeposit)
+        OHPIC_FP_CASE(kModeGatherPush)
+        OHPIC_FP_CASE(kModeBoundary)
+        OHPIC_FP_CASE(kModeFusedList)
+        OHPIC_FP_CASE(kModeBoundaryList)
     default:
         return hipErrorInvalidValue;
     }
+#undef OHPIC_FP_CASE
     return hipGetLastError();
+}
+
+template <int TX, int TY, int TZ, int WO>
+static hipError_t launch_tile_queue(const InstDev *insts, const WorkItem *items, int nitems,
+                                    const int *parity, unsigned int *queue, int blocks,
+                                    hipStream_t st)
+{
+    hipLaunchKernelGGL((k_particles_fp64<TX, TY, TZ, WO, kModeFusedQueue>), dim3(blocks),
+                       dim3(kBlock), 0, st, insts, items, parity, queue, nitems);
+    return hipGetLastError();
+}
+
+template <int TX, int TY, int TZ, int WO>
+static int queue_blocks_per_cu()
+{
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &nb, reinterpret_cast<const void *>(&k_particles_fp64<TX, TY, TZ, WO, kModeFusedQueue>),
+            kBlock, 0) != hipSuccess) {
+        (void)hipGetLastError();
+        nb = 1;
+    }
+    return nb > 0 ? nb : 1;
 }
 
 bool tile_supported(int tx, int ty, int tz)
@@ -577,6 +608,55 @@ hipError_t launch_particles(int mode, int precision, int weight_order, const int
     OHPIC_TILE(16, 8, 4)
 #undef OHPIC_TILE
     return hipErrorInvalidValue;
+}
+
+hipError_t launch_particles_queue(int precision, int weight_order, const int T[3],
+                                  const InstDev *insts, const WorkItem *items, int nitems,
+                                  const int *parity, unsigned int *queue, int blocks,
+                                  hipStream_t st)
+{
+    if (nitems <= 0) return hipSuccess;
+    if (blocks > nitems) blocks = nitems;
+    if (blocks < 1) blocks = 1;
+    if (precision == 1)
+        return launch_particles_mixed_queue(weight_order, T, insts, items, nitems, parity, queue,
+                                            blocks, st);
+    if (precision != 0) return hipErrorInvalidValue;
+#define OHPIC_TILE(A, B, C)                                                                    \
+    if (T[0] == A && T[1] == B && T[2] == C)                                                   \
+        return weight_order == 0                                                               \
+                   ? launch_tile_queue<A, B, C, 0>(insts, items, nitems, parity, queue, blocks, st) \
+                   : launch_tile_queue<A, B, C, 1>(insts, items, nitems, parity, queue, blocks, st);
+    OHPIC_TILE(4, 4, 4)
+    OHPIC_TILE(8, 8, 8)
+    OHPIC_TILE(8, 8, 4)
+    OHPIC_TILE(16, 8, 4)
+#undef OHPIC_TILE
+    return hipErrorInvalidValue;
+}
+
+int particles_queue_blocks(int precision, int weight_order, const int T[3], int device)
+{
+    int cus = 256;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess)
+        cus = prop.multiProcessorCount;
+    else
+        (void)hipGetLastError();
+    int per = 1;
+    if (precision == 1) {
+        per = particles_mixed_queue_blocks(weight_order, T);
+    } else {
+#define OHPIC_TILE(A, B, C)                                                              \
+    if (T[0] == A && T[1] == B && T[2] == C)                                             \
+        per = weight_order == 0 ? queue_blocks_per_cu<A, B, C, 0>() : queue_blocks_per_cu<A, B, C, 1>();
+        OHPIC_TILE(4, 4, 4)
+        OHPIC_TILE(8, 8, 8)
+        OHPIC_TILE(8, 8, 4)
+        OHPIC_TILE(16, 8, 4)
+#undef OHPIC_TILE
+    }
+    return per * cus;
 }
 
 hipError_t launch_bin_count(const InstDev *insts, int inst, int s, int lbuf, int64_t n,

@@ -197,12 +197,17 @@ __device__ __forceinline__ void flush_leavers_mx(LeaveBufMx &lb, int cnt, const 
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-template <int TX, int TY, int TZ, int WO, int MODE>
+template <int TX, int TY, int TZ, int WO, int MODE_>
 __global__ void __launch_bounds__(kBlock)
     __attribute__((amdgpu_waves_per_eu(OHPIC_MX_MIN_WAVES, 8)))
     k_particles_mixed(const InstDev *__restrict__ insts, const WorkItem *__restrict__ items,
-                      const int *__restrict__ parity)
+                      const int *__restrict__ parity, unsigned int *__restrict__ queue,
+                      int nitems)
 {
+    // kModeFusedQueue: the fused step over a resident grid, each workgroup
+    // claiming work items in list order from `queue` until none is left
+    constexpr bool QUEUE = (MODE_ == kModeFusedQueue);
+    constexpr int MODE = QUEUE ? (int)kModeFused : MODE_;
     constexpr bool LIST = (MODE == kModeFusedList || MODE == kModeBoundaryList);
     constexpr bool PUSH = (MODE == kModeFused || MODE == kModeGatherPush || MODE == kModeFusedList);
     constexpr bool REBIN = (MODE == kModeFused || MODE == kModeBoundary || LIST);
@@ -216,314 +221,329 @@ __global__ void __launch_bounds__(kBlock)
     __shared__ double sRed[kWaves];
     __shared__ int sFront, sFrontLimit;
 
-    const WorkItem it = items[blockIdx.x];
-    const InstDev *__restrict__ I = insts + it.inst;
-    const int par = parity[it.inst];
-    const int nx = I->n[0], ny = I->n[1], nz = I->n[2];
-    const int ntx = I->nt[0], nty = I->nt[1];
-    const int tile = it.tile;
-    const int tx = tile % ntx, ty = (tile / ntx) % nty, tz = tile / (ntx * nty);
-    const int lox = tx * TX, loy = ty * TY, loz = tz * TZ;
-    const int tid = threadIdx.x;
-    const int wv = tid >> 6;
-    const bool ids = I->track_ids != 0;
-    const bool excl = it.nchunks == 1;
-    unsigned long long *__restrict__ flags = I->flags;
-
-    if (PUSH) {
-        const double *__restrict__ ex = I->e[0];
-        const double *__restrict__ ey = I->e[1];
-        const double *__restrict__ ez = I->e[2];
-        for (int l = tid; l < NE; l += kBlock) {
-            const int lx = l % EX, ly = (l / EX) % EY, lz = l / (EX * EY);
-            const int gx = (lox + lx) % nx, gy = (loy + ly) % ny, gz = (loz + lz) % nz;
-            const int64_t g = gx + (int64_t)nx * (gy + (int64_t)ny * gz);
-            sE[l] = (float)gload(ex, g);
-            sE[NE + l] = (float)gload(ey, g);
-            sE[2 * NE + l] = (float)gload(ez, g);
-        }
-    }
-    if (DEP)
-        for (int l = tid; l < NR; l += kBlock) sR[l] = 0ull;
-    if (tid < kWaves) sLcnt[tid] = 0;
-    __syncthreads();
-
-    const float dt = (float)I->dt;
-    const float dt2 = 0.5f * dt;
-    const float dts = dt * kFracScale;  // cells per unit velocity, in 2^-24 units
-    const float bx = (float)I->b0[0], by = (float)I->b0[1], bz = (float)I->b0[2];
-    const int nbuf = par ^ 1;
-    const unsigned long long lanemask_lt = (1ull << lane_id()) - 1ull;
-    unsigned long long *__restrict__ racc =
-        I->rho_acc[(MODE == kModeFused || MODE == kModeFusedList) ? nbuf : par];
-
-    for (int s = 0; s < I->nspecies; ++s) {
-        const SpecDev &S = I->sp[s];
-        const int cap = gload(S.bin_cap, tile);
-        const unsigned long long word = gload(S.count[par], tile);
-        const int F = min(bin_front(word), cap);
-        const int B = min(bin_back(word), cap - F);
-        const int cnt = F + B;
-        const int64_t base = gload(S.bin_base, tile);
-        int per = (cnt + it.nchunks - 1) / it.nchunks;
-        per = (per + 63) & ~63;
-        const int start = it.chunk * per;
-        const int end = min(cnt, start + per);
-        if (REBIN) {
-            if (tid == 0) {
-                int lim = cap;
-                if (!LIST && excl && cnt > 0) {
-                    const unsigned long long old =
-                        gatomic_add(&S.count[nbuf][tile], (unsigned long long)cnt);
-                    lim = cap - bin_back(old);
-                }
-                sFront = 0;
-                sFrontLimit = lim;
-            }
+    __shared__ int sItem;
+    for (int iter = 0;; ++iter) {
+        int item;
+        if (QUEUE) {
+            if (threadIdx.x == 0) sItem = (int)atomicAdd(queue, 1u);
             __syncthreads();
+            item = sItem;
+            if (item >= nitems) break;
+        } else {
+            if (iter > 0) break;
+            item = blockIdx.x;
         }
-        const float qm = (float)S.qm;
-        const double qfix = S.qfix;
-        const float t0 = qm * bx * dt2, t1 = qm * by * dt2, t2 = qm * bz * dt2;
-        const float tt = t0 * t0 + t1 * t1 + t2 * t2;
-        const float s0 = 2.0f * t0 / (1.0f + tt), s1 = 2.0f * t1 / (1.0f + tt),
-                    s2 = 2.0f * t2 / (1.0f + tt);
-        double keo = 0.0, ken = 0.0;
+        const WorkItem it = items[item];
+        const InstDev *__restrict__ I = insts + it.inst;
+        const int par = parity[it.inst];
+        const int nx = I->n[0], ny = I->n[1], nz = I->n[2];
+        const int ntx = I->nt[0], nty = I->nt[1];
+        const int tile = it.tile;
+        const int tx = tile % ntx, ty = (tile / ntx) % nty, tz = tile / (ntx * nty);
+        const int lox = tx * TX, loy = ty * TY, loz = tz * TZ;
+        const int tid = threadIdx.x;
+        const int wv = tid >> 6;
+        const bool ids = I->track_ids != 0;
+        const bool excl = it.nchunks == 1;
+        unsigned long long *__restrict__ flags = I->flags;
 
-        // the next kMxPrefetch particles of this lane are loaded ahead, so each
-        // lane keeps that many particle reads in flight
-        const BinView cur = bin_view(S, par, base, ids);
-        const BinView nxt = bin_view(S, nbuf, base, ids);
-        PMx ahead[kMxPrefetch];
-#pragma unroll
-        for (int k = 0; k < kMxPrefetch; ++k) {
-            ahead[k] = PMx{{0u, 0u, 0u}, {0.0f, 0.0f, 0.0f}, 0};
-            if (start + tid + k * kBlock < end)
-                ahead[k] = load_bin(cur, (uint32_t)bin_slot(start + tid + k * kBlock, F, B, cap),
-                                    ids);
+        if (PUSH) {
+            const double *__restrict__ ex = I->e[0];
+            const double *__restrict__ ey = I->e[1];
+            const double *__restrict__ ez = I->e[2];
+            for (int l = tid; l < NE; l += kBlock) {
+                const int lx = l % EX, ly = (l / EX) % EY, lz = l / (EX * EY);
+                const int gx = (lox + lx) % nx, gy = (loy + ly) % ny, gz = (loz + lz) % nz;
+                const int64_t g = gx + (int64_t)nx * (gy + (int64_t)ny * gz);
+                sE[l] = (float)gload(ex, g);
+                sE[NE + l] = (float)gload(ey, g);
+                sE[2 * NE + l] = (float)gload(ez, g);
+            }
         }
-        for (int p0 = start; p0 < end; p0 += kBlock) {
-            const int p = p0 + tid;
-            const bool active = p < end;
-            PMx q = ahead[0];
-#pragma unroll
-            for (int k = 0; k + 1 < kMxPrefetch; ++k) ahead[k] = ahead[k + 1];
-            if (p + kMxPrefetch * kBlock < end)
-                ahead[kMxPrefetch - 1] =
-                    load_bin(cur, (uint32_t)bin_slot(p + kMxPrefetch * kBlock, F, B, cap), ids);
-            // cell relative to this tile's origin and in-cell offset per axis
-            int rx = pos_rel(q.w[0]), ry = pos_rel(q.w[1]), rz = pos_rel(q.w[2]);
-            float ox = pos_frac(q.w[0]), oy = pos_frac(q.w[1]), oz = pos_frac(q.w[2]);
-            if (PUSH && active) {
-                float wx0, wx1, wy0, wy1, wz0, wz1;
-                cic_wf<WO>(ox, wx0, wx1);
-                cic_wf<WO>(oy, wy0, wy1);
-                cic_wf<WO>(oz, wz0, wz1);
-                const int lx = min(max(rx, 0), TX - 1);
-                const int ly = min(max(ry, 0), TY - 1);
-                const int lz = min(max(rz, 0), TZ - 1);
-                const int l0 = lx + EX * (ly + EY * lz);
-                float e0 = 0.0f, e1 = 0.0f, e2 = 0.0f;
-                const float wx[2] = {wx0, wx1}, wy[2] = {wy0, wy1}, wz[2] = {wz0, wz1};
-#pragma unroll
-                for (int i = 0; i < 2; ++i)
-#pragma unroll
-                    for (int j = 0; j < 2; ++j)
-#pragma unroll
-                        for (int k = 0; k < 2; ++k) {
-                            const int l = l0 + i + EX * (j + EY * k);
-                            const float w = wx[i] * wy[j] * wz[k];
-                            e0 += sE[l] * w;
-                            e1 += sE[NE + l] * w;
-                            e2 += sE[2 * NE + l] * w;
-                        }
-                const float vx = q.v[0], vy = q.v[1], vz = q.v[2];
-                keo += (double)(vx * vx + vy * vy + vz * vz);
-                const float um0 = vx + qm * e0 * dt2;
-                const float um1 = vy + qm * e1 * dt2;
-                const float um2 = vz + qm * e2 * dt2;
-                const float ua0 = um0 + (um1 * t2 - um2 * t1);
-                const float ua1 = um1 + (um2 * t0 - um0 * t2);
-                const float ua2 = um2 + (um0 * t1 - um1 * t0);
-                const float up0 = um0 + (ua1 * s2 - ua2 * s1);
-                const float up1 = um1 + (ua2 * s0 - ua0 * s2);
-                const float up2 = um2 + (ua0 * s1 - ua1 * s0);
-                q.v[0] = up0 + qm * e0 * dt2;
-                q.v[1] = up1 + qm * e1 * dt2;
-                q.v[2] = up2 + qm * e2 * dt2;
-                ken += (double)(q.v[0] * q.v[0] + q.v[1] * q.v[1] + q.v[2] * q.v[2]);
-                // drift in fixed point: the tile-relative position word (8.24)
-                // plus the displacement rounded to 2^-24 of a cell, exact in
-                // int32 while |v dt| < 64 cells (farther is reported)
-                const float sx = q.v[0] * dts, sy = q.v[1] * dts, sz = q.v[2] * dts;
-                const bool ok =
-                    fabsf(sx) < kMaxShift && fabsf(sy) < kMaxShift && fabsf(sz) < kMaxShift;
-                if (!ok) {
-                    gatomic_add(&flags[kFlagRange], 1ull);
-                } else {
-                    q.w[0] = (uint32_t)((int32_t)q.w[0] + (int32_t)rintf(sx));
-                    q.w[1] = (uint32_t)((int32_t)q.w[1] + (int32_t)rintf(sy));
-                    q.w[2] = (uint32_t)((int32_t)q.w[2] + (int32_t)rintf(sz));
-                }
-                rx = pos_rel(q.w[0]);
-                ry = pos_rel(q.w[1]);
-                rz = pos_rel(q.w[2]);
-                ox = pos_frac(q.w[0]);
-                oy = pos_frac(q.w[1]);
-                oz = pos_frac(q.w[2]);
-                if (MODE == kModeGatherPush) {
-                    // not re-binned: the cell stays relative to this tile
-                    store_bin(cur, (uint32_t)bin_slot(p, F, B, cap), q, ids);
-                }
-            }
-            if (MODE == kModeGatherPush) continue;
-            // global cell, periodic; new tile and the cell relative to it
-            const int gx = wrap_cell(lox + rx, nx), gy = wrap_cell(loy + ry, ny),
-                      gz = wrap_cell(loz + rz, nz);
-            const int ctx_t = gx / TX, cty = gy / TY, ctz = gz / TZ;
-            const int ntile = ctx_t + ntx * (cty + nty * ctz);
+        if (DEP)
+            for (int l = tid; l < NR; l += kBlock) sR[l] = 0ull;
+        if (tid < kWaves) sLcnt[tid] = 0;
+        __syncthreads();
+
+        const float dt = (float)I->dt;
+        const float dt2 = 0.5f * dt;
+        const float dts = dt * kFracScale;  // cells per unit velocity, in 2^-24 units
+        const float bx = (float)I->b0[0], by = (float)I->b0[1], bz = (float)I->b0[2];
+        const int nbuf = par ^ 1;
+        const unsigned long long lanemask_lt = (1ull << lane_id()) - 1ull;
+        unsigned long long *__restrict__ racc =
+            I->rho_acc[(MODE == kModeFused || MODE == kModeFusedList) ? nbuf : par];
+
+        for (int s = 0; s < I->nspecies; ++s) {
+            const SpecDev &S = I->sp[s];
+            const int cap = gload(S.bin_cap, tile);
+            const unsigned long long word = gload(S.count[par], tile);
+            const int F = min(bin_front(word), cap);
+            const int B = min(bin_back(word), cap - F);
+            const int cnt = F + B;
+            const int64_t base = gload(S.bin_base, tile);
+            int per = (cnt + it.nchunks - 1) / it.nchunks;
+            per = (per + 63) & ~63;
+            const int start = it.chunk * per;
+            const int end = min(cnt, start + per);
             if (REBIN) {
-                q.w[0] = pos_word(gx - ctx_t * TX, q.w[0] & kFracMask);
-                q.w[1] = pos_word(gy - cty * TY, q.w[1] & kFracMask);
-                q.w[2] = pos_word(gz - ctz * TZ, q.w[2] & kFracMask);
-            }
-            if (DEP && active) {
-                float wx0, wx1, wy0, wy1, wz0, wz1;
-                cic_wf<WO>(ox, wx0, wx1);
-                cic_wf<WO>(oy, wy0, wy1);
-                cic_wf<WO>(oz, wz0, wz1);
-                const float wx[2] = {wx0, wx1}, wy[2] = {wy0, wy1}, wz[2] = {wz0, wz1};
-                const int dx = wrap_index(gx - lox + 1, nx);
-                const int dy = wrap_index(gy - loy + 1, ny);
-                const int dz = wrap_index(gz - loz + 1, nz);
-                if (dx <= TX + 1 && dy <= TY + 1 && dz <= TZ + 1) {
-                    const int l0 = dx + RX * (dy + RY * dz);
-#pragma unroll
-                    for (int i = 0; i < 2; ++i)
-#pragma unroll
-                        for (int j = 0; j < 2; ++j)
-#pragma unroll
-                            for (int k = 0; k < 2; ++k) {
-                                const long long v =
-                                    fix_round((double)(wx[i] * wy[j] * wz[k]), qfix);
-                                atomicAdd(&sR[l0 + i + RX * (j + RY * k)],
-                                          (unsigned long long)v);
-                            }
-                } else {
-                    gatomic_add(&flags[kFlagFast], 1ull);
-#pragma unroll
-                    for (int i = 0; i < 2; ++i)
-#pragma unroll
-                        for (int j = 0; j < 2; ++j)
-#pragma unroll
-                            for (int k = 0; k < 2; ++k) {
-                                const int hx = (gx + i) % nx, hy = (gy + j) % ny,
-                                          hz = (gz + k) % nz;
-                                const long long v =
-                                    fix_round((double)(wx[i] * wy[j] * wz[k]), qfix);
-                                gatomic_add(&racc[hx + (int64_t)nx * (hy + (int64_t)ny * hz)],
-                                            (unsigned long long)v);
-                            }
-                }
-            }
-            if (LIST) {
-                const unsigned long long m = __ballot(active);
-                if (m) {
-                    const int leader = __ffsll((unsigned long long)m) - 1;
-                    unsigned long long b = 0;
-                    if (lane_id() == leader)
-                        b = gatomic_add(&S.count[nbuf][0], (unsigned long long)__popcll(m));
-                    b = __shfl(b, leader, 64);
-                    if (active)
-                        store_list_mx(S, nbuf, (int64_t)b + __popcll(m & lanemask_lt), q, ntile,
-                                      ids);
-                }
-            } else if (REBIN) {
-                const bool stay = active && ntile == tile;
-                const unsigned long long m = __ballot(stay);
-                if (m) {
-                    const int leader = __ffsll((unsigned long long)m) - 1;
-                    const int nst = (int)__popcll(m);
-                    int b = 0, lim = 0;
-                    if (excl) {
-                        if (lane_id() == leader) b = atomicAdd(&sFront, nst);
-                        lim = sFrontLimit;
-                    } else if (lane_id() == leader) {
+                if (tid == 0) {
+                    int lim = cap;
+                    if (!LIST && excl && cnt > 0) {
                         const unsigned long long old =
-                            gatomic_add(&S.count[nbuf][tile], (unsigned long long)nst);
-                        b = bin_front(old);
+                            gatomic_add(&S.count[nbuf][tile], (unsigned long long)cnt);
                         lim = cap - bin_back(old);
                     }
-                    b = __shfl(b, leader, 64);
-                    lim = __shfl(lim, leader, 64);
-                    if (stay) {
-                        const int slot = b + (int)__popcll(m & lanemask_lt);
-                        if (slot < lim)
-                            store_bin(nxt, (uint32_t)slot, q, ids);
-                        else
-                            gatomic_add(&flags[kFlagOverflow], 1ull);
-                    }
+                    sFront = 0;
+                    sFrontLimit = lim;
                 }
-                const bool leave = active && ntile != tile;
-                const unsigned long long lm = __ballot(leave);
-                if (lm) {
-                    const int nl = (int)__popcll(lm);
-                    int c = sLcnt[wv];
-                    if (c + nl > kLeaveCap) {
-                        flush_leavers_mx(sLeave[wv], c, S, nbuf, ids, flags);
-                        c = 0;
-                    }
-                    if (nl <= kLeaveCap) {
-                        if (leave) {
-                            const int slot = c + (int)__popcll(lm & lanemask_lt);
-                            LeaveBufMx &lb = sLeave[wv];
-#pragma unroll
-                            for (int a = 0; a < 3; ++a) {
-                                lb.w[a][slot] = q.w[a];
-                                lb.v[a][slot] = q.v[a];
-                            }
-                            lb.id[slot] = q.id;
-                            lb.dest[slot] = ntile;
-                        }
-                        c += nl;
-                    } else if (leave) {
-                        route_back_mx(S, nbuf, ntile, q, ids, flags);
-                    }
-                    if (lane_id() == 0) sLcnt[wv] = c;
-                }
+                __syncthreads();
             }
-        }
-        if (REBIN && !LIST) {
-            flush_leavers_mx(sLeave[wv], sLcnt[wv], S, nbuf, ids, flags);
-            __syncthreads();
-            if (tid < kWaves) sLcnt[tid] = 0;
-            if (tid == 0 && excl && cnt > sFront)
-                gatomic_add(&S.count[nbuf][tile],
-                          (unsigned long long)(-(long long)(cnt - sFront)));
-            __syncthreads();
-        }
-        if (PUSH) {
-            const double so = block_sum(keo, sRed);
-            const double sn = block_sum(ken, sRed);
-            if (tid == 0) {
-                gstore(I->ke_partial, (int64_t)it.slot * kKeSlots + s, 0.5 * S.mass * so);
-                gstore(I->ke_partial, (int64_t)it.slot * kKeSlots + kMaxSpecies + s,
-                       0.5 * S.mass * sn);
-            }
-        }
-    }
+            const float qm = (float)S.qm;
+            const double qfix = S.qfix;
+            const float t0 = qm * bx * dt2, t1 = qm * by * dt2, t2 = qm * bz * dt2;
+            const float tt = t0 * t0 + t1 * t1 + t2 * t2;
+            const float s0 = 2.0f * t0 / (1.0f + tt), s1 = 2.0f * t1 / (1.0f + tt),
+                        s2 = 2.0f * t2 / (1.0f + tt);
+            double keo = 0.0, ken = 0.0;
 
-    if (DEP) {
-        __syncthreads();
-        for (int l = tid; l < NR; l += kBlock) {
-            const unsigned long long v = sR[l];
-            if (v == 0ull) continue;
-            const int lx = l % RX, ly = (l / RX) % RY, lz = l / (RX * RY);
-            const int gx = ((lox - 1 + lx) % nx + nx) % nx;
-            const int gy = ((loy - 1 + ly) % ny + ny) % ny;
-            const int gz = ((loz - 1 + lz) % nz + nz) % nz;
-            gatomic_add(&racc[gx + (int64_t)nx * (gy + (int64_t)ny * gz)], v);
+            // the next kMxPrefetch particles of this lane are loaded ahead, so each
+            // lane keeps that many particle reads in flight
+            const BinView cur = bin_view(S, par, base, ids);
+            const BinView nxt = bin_view(S, nbuf, base, ids);
+            PMx ahead[kMxPrefetch];
+    #pragma unroll
+            for (int k = 0; k < kMxPrefetch; ++k) {
+                ahead[k] = PMx{{0u, 0u, 0u}, {0.0f, 0.0f, 0.0f}, 0};
+                if (start + tid + k * kBlock < end)
+                    ahead[k] = load_bin(cur, (uint32_t)bin_slot(start + tid + k * kBlock, F, B, cap),
+                                        ids);
+            }
+            for (int p0 = start; p0 < end; p0 += kBlock) {
+                const int p = p0 + tid;
+                const bool active = p < end;
+                PMx q = ahead[0];
+    #pragma unroll
+                for (int k = 0; k + 1 < kMxPrefetch; ++k) ahead[k] = ahead[k + 1];
+                if (p + kMxPrefetch * kBlock < end)
+                    ahead[kMxPrefetch - 1] =
+                        load_bin(cur, (uint32_t)bin_slot(p + kMxPrefetch * kBlock, F, B, cap), ids);
+                // cell relative to this tile's origin and in-cell offset per axis
+                int rx = pos_rel(q.w[0]), ry = pos_rel(q.w[1]), rz = pos_rel(q.w[2]);
+                float ox = pos_frac(q.w[0]), oy = pos_frac(q.w[1]), oz = pos_frac(q.w[2]);
+                if (PUSH && active) {
+                    float wx0, wx1, wy0, wy1, wz0, wz1;
+                    cic_wf<WO>(ox, wx0, wx1);
+                    cic_wf<WO>(oy, wy0, wy1);
+                    cic_wf<WO>(oz, wz0, wz1);
+                    const int lx = min(max(rx, 0), TX - 1);
+                    const int ly = min(max(ry, 0), TY - 1);
+                    const int lz = min(max(rz, 0), TZ - 1);
+                    const int l0 = lx + EX * (ly + EY * lz);
+                    float e0 = 0.0f, e1 = 0.0f, e2 = 0.0f;
+                    const float wx[2] = {wx0, wx1}, wy[2] = {wy0, wy1}, wz[2] = {wz0, wz1};
+    #pragma unroll
+                    for (int i = 0; i < 2; ++i)
+    #pragma unroll
+                        for (int j = 0; j < 2; ++j)
+    #pragma unroll
+                            for (int k = 0; k < 2; ++k) {
+                                const int l = l0 + i + EX * (j + EY * k);
+                                const float w = wx[i] * wy[j] * wz[k];
+                                e0 += sE[l] * w;
+                                e1 += sE[NE + l] * w;
+                                e2 += sE[2 * NE + l] * w;
+                            }
+                    const float vx = q.v[0], vy = q.v[1], vz = q.v[2];
+                    keo += (double)(vx * vx + vy * vy + vz * vz);
+                    const float um0 = vx + qm * e0 * dt2;
+                    const float um1 = vy + qm * e1 * dt2;
+                    const float um2 = vz + qm * e2 * dt2;
+                    const float ua0 = um0 + (um1 * t2 - um2 * t1);
+                    const float ua1 = um1 + (um2 * t0 - um0 * t2);
+                    const float ua2 = um2 + (um0 * t1 - um1 * t0);
+                    const float up0 = um0 + (ua1 * s2 - ua2 * s1);
+                    const float up1 = um1 + (ua2 * s0 - ua0 * s2);
+                    const float up2 = um2 + (ua0 * s1 - ua1 * s0);
+                    q.v[0] = up0 + qm * e0 * dt2;
+                    q.v[1] = up1 + qm * e1 * dt2;
+                    q.v[2] = up2 + qm * e2 * dt2;
+                    ken += (double)(q.v[0] * q.v[0] + q.v[1] * q.v[1] + q.v[2] * q.v[2]);
+                    // drift in fixed point: the tile-relative position word (8.24)
+                    // plus the displacement rounded to 2^-24 of a cell, exact in
+                    // int32 while |v dt| < 64 cells (farther is reported)
+                    const float sx = q.v[0] * dts, sy = q.v[1] * dts, sz = q.v[2] * dts;
+                    const bool ok =
+                        fabsf(sx) < kMaxShift && fabsf(sy) < kMaxShift && fabsf(sz) < kMaxShift;
+                    if (!ok) {
+                        gatomic_add(&flags[kFlagRange], 1ull);
+                    } else {
+                        q.w[0] = (uint32_t)((int32_t)q.w[0] + (int32_t)rintf(sx));
+                        q.w[1] = (uint32_t)((int32_t)q.w[1] + (int32_t)rintf(sy));
+                        q.w[2] = (uint32_t)((int32_t)q.w[2] + (int32_t)rintf(sz));
+                    }
+                    rx = pos_rel(q.w[0]);
+                    ry = pos_rel(q.w[1]);
+                    rz = pos_rel(q.w[2]);
+                    ox = pos_frac(q.w[0]);
+                    oy = pos_frac(q.w[1]);
+                    oz = pos_frac(q.w[2]);
+                    if (MODE == kModeGatherPush) {
+                        // not re-binned: the cell stays relative to this tile
+                        store_bin(cur, (uint32_t)bin_slot(p, F, B, cap), q, ids);
+                    }
+                }
+                if (MODE == kModeGatherPush) continue;
+                // global cell, periodic; new tile and the cell relative to it
+                const int gx = wrap_cell(lox + rx, nx), gy = wrap_cell(loy + ry, ny),
+                          gz = wrap_cell(loz + rz, nz);
+                const int ctx_t = gx / TX, cty = gy / TY, ctz = gz / TZ;
+                const int ntile = ctx_t + ntx * (cty + nty * ctz);
+                if (REBIN) {
+                    q.w[0] = pos_word(gx - ctx_t * TX, q.w[0] & kFracMask);
+                    q.w[1] = pos_word(gy - cty * TY, q.w[1] & kFracMask);
+                    q.w[2] = pos_word(gz - ctz * TZ, q.w[2] & kFracMask);
+                }
+                if (DEP && active) {
+                    float wx0, wx1, wy0, wy1, wz0, wz1;
+                    cic_wf<WO>(ox, wx0, wx1);
+                    cic_wf<WO>(oy, wy0, wy1);
+                    cic_wf<WO>(oz, wz0, wz1);
+                    const float wx[2] = {wx0, wx1}, wy[2] = {wy0, wy1}, wz[2] = {wz0, wz1};
+                    const int dx = wrap_index(gx - lox + 1, nx);
+                    const int dy = wrap_index(gy - loy + 1, ny);
+                    const int dz = wrap_index(gz - loz + 1, nz);
+                    if (dx <= TX + 1 && dy <= TY + 1 && dz <= TZ + 1) {
+                        const int l0 = dx + RX * (dy + RY * dz);
+    #pragma unroll
+                        for (int i = 0; i < 2; ++i)
+    #pragma unroll
+                            for (int j = 0; j < 2; ++j)
+    #pragma unroll
+                                for (int k = 0; k < 2; ++k) {
+                                    const long long v =
+                                        fix_round((double)(wx[i] * wy[j] * wz[k]), qfix);
+                                    atomicAdd(&sR[l0 + i + RX * (j + RY * k)],
+                                              (unsigned long long)v);
+                                }
+                    } else {
+                        gatomic_add(&flags[kFlagFast], 1ull);
+    #pragma unroll
+                        for (int i = 0; i < 2; ++i)
+    #pragma unroll
+                            for (int j = 0; j < 2; ++j)
+    #pragma unroll
+                                for (int k = 0; k < 2; ++k) {
+                                    const int hx = (gx + i) % nx, hy = (gy + j) % ny,
+                                              hz = (gz + k) % nz;
+                                    const long long v =
+                                        fix_round((double)(wx[i] * wy[j] * wz[k]), qfix);
+                                    gatomic_add(&racc[hx + (int64_t)nx * (hy + (int64_t)ny * hz)],
+                                                (unsigned long long)v);
+                                }
+                    }
+                }
+                if (LIST) {
+                    const unsigned long long m = __ballot(active);
+                    if (m) {
+                        const int leader = __ffsll((unsigned long long)m) - 1;
+                        unsigned long long b = 0;
+                        if (lane_id() == leader)
+                            b = gatomic_add(&S.count[nbuf][0], (unsigned long long)__popcll(m));
+                        b = __shfl(b, leader, 64);
+                        if (active)
+                            store_list_mx(S, nbuf, (int64_t)b + __popcll(m & lanemask_lt), q, ntile,
+                                          ids);
+                    }
+                } else if (REBIN) {
+                    const bool stay = active && ntile == tile;
+                    const unsigned long long m = __ballot(stay);
+                    if (m) {
+                        const int leader = __ffsll((unsigned long long)m) - 1;
+                        const int nst = (int)__popcll(m);
+                        int b = 0, lim = 0;
+                        if (excl) {
+                            if (lane_id() == leader) b = atomicAdd(&sFront, nst);
+                            lim = sFrontLimit;
+                        } else if (lane_id() == leader) {
+                            const unsigned long long old =
+                                gatomic_add(&S.count[nbuf][tile], (unsigned long long)nst);
+                            b = bin_front(old);
+                            lim = cap - bin_back(old);
+                        }
+                        b = __shfl(b, leader, 64);
+                        lim = __shfl(lim, leader, 64);
+                        if (stay) {
+                            const int slot = b + (int)__popcll(m & lanemask_lt);
+                            if (slot < lim)
+                                store_bin(nxt, (uint32_t)slot, q, ids);
+                            else
+                                gatomic_add(&flags[kFlagOverflow], 1ull);
+                        }
+                    }
+                    const bool leave = active && ntile != tile;
+                    const unsigned long long lm = __ballot(leave);
+                    if (lm) {
+                        const int nl = (int)__popcll(lm);
+                        int c = sLcnt[wv];
+                        if (c + nl > kLeaveCap) {
+                            flush_leavers_mx(sLeave[wv], c, S, nbuf, ids, flags);
+                            c = 0;
+                        }
+                        if (nl <= kLeaveCap) {
+                            if (leave) {
+                                const int slot = c + (int)__popcll(lm & lanemask_lt);
+                                LeaveBufMx &lb = sLeave[wv];
+    #pragma unroll
+                                for (int a = 0; a < 3; ++a) {
+                                    lb.w[a][slot] = q.w[a];
+                                    lb.v[a][slot] = q.v[a];
+                                }
+                                lb.id[slot] = q.id;
+                                lb.dest[slot] = ntile;
+                            }
+                            c += nl;
+                        } else if (leave) {
+                            route_back_mx(S, nbuf, ntile, q, ids, flags);
+                        }
+                        if (lane_id() == 0) sLcnt[wv] = c;
+                    }
+                }
+            }
+            if (REBIN && !LIST) {
+                flush_leavers_mx(sLeave[wv], sLcnt[wv], S, nbuf, ids, flags);
+                __syncthreads();
+                if (tid < kWaves) sLcnt[tid] = 0;
+                if (tid == 0 && excl && cnt > sFront)
+                    gatomic_add(&S.count[nbuf][tile],
+                              (unsigned long long)(-(long long)(cnt - sFront)));
+                __syncthreads();
+            }
+            if (PUSH) {
+                const double so = block_sum(keo, sRed);
+                const double sn = block_sum(ken, sRed);
+                if (tid == 0) {
+                    gstore(I->ke_partial, (int64_t)it.slot * kKeSlots + s, 0.5 * S.mass * so);
+                    gstore(I->ke_partial, (int64_t)it.slot * kKeSlots + kMaxSpecies + s,
+                           0.5 * S.mass * sn);
+                }
+            }
         }
+
+        if (DEP) {
+            __syncthreads();
+            for (int l = tid; l < NR; l += kBlock) {
+                const unsigned long long v = sR[l];
+                if (v == 0ull) continue;
+                const int lx = l % RX, ly = (l / RX) % RY, lz = l / (RX * RY);
+                const int gx = ((lox - 1 + lx) % nx + nx) % nx;
+                const int gy = ((loy - 1 + ly) % ny + ny) % ny;
+                const int gz = ((loz - 1 + lz) % nz + nz) % nz;
+                gatomic_add(&racc[gx + (int64_t)nx * (gy + (int64_t)ny * gz)], v);
+            }
+        }
+        // the next item reuses the LDS blocks and sItem
+        __syncthreads();
     }
 }
 
@@ -716,7 +736,7 @@ static hipError_t launch_tile_mx(int mode, const InstDev *insts, const WorkItem 
 #define OHPIC_MX_CASE(M)                                                                 \
     case M:                                                                              \
         hipLaunchKernelGGL((k_particles_mixed<TX, TY, TZ, WO, M>), g, b, 0, st, insts, items, \
-                           parity);                                                      \
+                           parity, nullptr, nitems);                                     \
         break;
     switch (mode) {
         OHPIC_MX_CASE(kModeFused)
@@ -747,6 +767,60 @@ hipError_t launch_particles_mixed(int mode, int weight_order, const int T[3],
     OHPIC_TILE(16, 8, 4)
 #undef OHPIC_TILE
     return hipErrorInvalidValue;
+}
+
+template <int TX, int TY, int TZ, int WO>
+static hipError_t launch_tile_mx_queue(const InstDev *insts, const WorkItem *items, int nitems,
+                                       const int *parity, unsigned int *queue, int blocks,
+                                       hipStream_t st)
+{
+    hipLaunchKernelGGL((k_particles_mixed<TX, TY, TZ, WO, kModeFusedQueue>), dim3(blocks),
+                       dim3(kBlock), 0, st, insts, items, parity, queue, nitems);
+    return hipGetLastError();
+}
+
+template <int TX, int TY, int TZ, int WO>
+static int queue_blocks_per_cu_mx()
+{
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &nb, reinterpret_cast<const void *>(&k_particles_mixed<TX, TY, TZ, WO, kModeFusedQueue>),
+            kBlock, 0) != hipSuccess) {
+        (void)hipGetLastError();
+        nb = 1;
+    }
+    return nb > 0 ? nb : 1;
+}
+
+hipError_t launch_particles_mixed_queue(int weight_order, const int T[3], const InstDev *insts,
+                                        const WorkItem *items, int nitems, const int *parity,
+                                        unsigned int *queue, int blocks, hipStream_t st)
+{
+#define OHPIC_TILE(A, B, C)                                                                   \
+    if (T[0] == A && T[1] == B && T[2] == C)                                                  \
+        return weight_order == 0                                                              \
+                   ? launch_tile_mx_queue<A, B, C, 0>(insts, items, nitems, parity, queue, blocks, st) \
+                   : launch_tile_mx_queue<A, B, C, 1>(insts, items, nitems, parity, queue, blocks, st);
+    OHPIC_TILE(4, 4, 4)
+    OHPIC_TILE(8, 8, 8)
+    OHPIC_TILE(8, 8, 4)
+    OHPIC_TILE(16, 8, 4)
+#undef OHPIC_TILE
+    return hipErrorInvalidValue;
+}
+
+int particles_mixed_queue_blocks(int weight_order, const int T[3])
+{
+#define OHPIC_TILE(A, B, C)                                                            \
+    if (T[0] == A && T[1] == B && T[2] == C)                                           \
+        return weight_order == 0 ? queue_blocks_per_cu_mx<A, B, C, 0>()                \
+                                 : queue_blocks_per_cu_mx<A, B, C, 1>();
+    OHPIC_TILE(4, 4, 4)
+    OHPIC_TILE(8, 8, 8)
+    OHPIC_TILE(8, 8, 4)
+    OHPIC_TILE(16, 8, 4)
+#undef OHPIC_TILE
+    return 1;
 }
 
 hipError_t launch_bin_count_mx(const InstDev *insts, int inst, int s, int lbuf, int64_t n,

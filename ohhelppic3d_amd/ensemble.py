@@ -45,12 +45,15 @@ class InstanceSpec:
         return sum(self.counts)
 
     def config(self, tile=(0, 0, 0), chunks: int = 0, precision: int = L.MIXED,
-               track_ids: bool = False, device: int = 0) -> SimConfig:
+               track_ids: bool = False, device: int = 0, slack: float = 0.3) -> SimConfig:
+        # bins 30% above their count: small instances have few particles per
+        # tile and larger relative fluctuations, and a bin that overflows
+        # costs that step a list-mode retry
         return SimConfig(n=self.n, qm=(-1.0, 0.01),
                          q_macro=(-1.0 / self.ppc[0], 1.0 / self.ppc[1]),
                          capacity=self.counts, dt=self.dt, b0=(0.0, 0.0, self.omega_ce),
                          precision=precision, tile=tuple(tile), chunks=chunks,
-                         track_ids=track_ids, device=device)
+                         track_ids=track_ids, device=device, slack=slack)
 
     def create(self, tile=(0, 0, 0), chunks: int = 0, precision: int = L.MIXED,
                track_ids: bool = False, device: int = 0) -> Simulation:

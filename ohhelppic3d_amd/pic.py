@@ -255,13 +255,15 @@ class Group:
     hipGraph.  Members must share device, precision, weight order and tile;
     each member's result equals stepping it alone with ``Simulation.step``."""
 
-    def __init__(self, device: int = 0, use_graph: bool = True):
+    def __init__(self, device: int = 0, use_graph: bool = True, persistent: bool = True):
         self._lib = L.lib()
         self._g = C.c_void_p()
         check(self._lib.ohpic_group_create(int(device), C.byref(self._g)))
         self.members: list = []
         if not use_graph:
             self.use_graph(False)
+        if not persistent:
+            self.use_persistent(False)
 
     def _chk(self, rc):
         L.check_group(rc, self._g)
@@ -315,6 +317,9 @@ class Group:
 
     def use_graph(self, on: bool) -> None:
         self._chk(self._lib.ohpic_group_use_graph(self._g, 1 if on else 0))
+
+    def use_persistent(self, on: bool) -> None:
+        self._chk(self._lib.ohpic_group_use_persistent(self._g, 1 if on else 0))
 
     def stats(self) -> dict:
         s, c, i = C.c_int64(), C.c_int64(), C.c_int64()

@@ -72,10 +72,10 @@ def run_alone(ohp, precision, steps, **kw):
     return snaps
 
 
-def run_group(ohp, precision, steps, use_graph=True, **kw):
+def run_group(ohp, precision, steps, use_graph=True, persistent=True, **kw):
     sims = [make(ohp, n, ppc, seed, precision, **kw) for n, ppc, seed in CASES]
     try:
-        with ohp.Group(use_graph=use_graph) as grp:
+        with ohp.Group(use_graph=use_graph, persistent=persistent) as grp:
             for s in sims:
                 grp.attach(s)
             assert len(grp) == len(CASES)
@@ -104,6 +104,17 @@ def test_group_graph_equals_direct_launches(ohp):
     assert st_g["captures"] >= 1 and st_d["captures"] == 0
     for i, (a, b) in enumerate(zip(with_graph, direct)):
         assert_same(a, b, f"member {i}")
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_group_persistent_launch_equals_per_item_launch(ohp, precision):
+    # one resident grid claiming work items from a counter vs one workgroup
+    # per work item: same particles, fields and counts
+    prec = ohp.FP64 if precision == "fp64" else ohp.MIXED
+    queued, _ = run_group(ohp, prec, 3, persistent=True, tile=(4, 4, 4), chunks=2)
+    per_item, _ = run_group(ohp, prec, 3, persistent=False, tile=(4, 4, 4), chunks=2)
+    for i, (a, b) in enumerate(zip(queued, per_item)):
+        assert_same(a, b, f"{precision} member {i}")
 
 
 def drifting_pair(ohp):
