@@ -76,6 +76,36 @@ __device__ __forceinline__ unsigned long long gatomic_add(unsigned long long *p,
     return __hip_atomic_fetch_add(as_global(p), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// counter[t] += 1 for every active lane with `valid`; returns the lane's
+// value of the counter before its own increment.  Lanes of the wavefront
+// that share t are served by one atomic (their leader adds the group size and
+// each lane takes old + its rank in the group), for the first kAggRounds
+// distinct values of t; the remaining lanes add individually.  Particle lists
+// written by the step kernels keep each source tile's particles together, so
+// a wavefront usually targets one to a few bins.
+constexpr int kAggRounds = 4;
+
+__device__ __forceinline__ unsigned long long wave_counter_add(unsigned long long *counter, int t,
+                                                               bool valid)
+{
+    const int lane = __lane_id();
+    const unsigned long long lt = (1ull << lane) - 1ull;
+    unsigned long long todo = __ballot(valid);
+    unsigned long long result = 0ull;
+    for (int r = 0; r < kAggRounds && todo; ++r) {
+        const int leader = __ffsll((unsigned long long)todo) - 1;
+        const int tl = __shfl(t, leader, 64);
+        const unsigned long long m = __ballot(valid && t == tl) & todo;
+        unsigned long long old = 0ull;
+        if (lane == leader) old = gatomic_add(&counter[tl], (unsigned long long)__popcll(m));
+        old = __shfl(old, leader, 64);
+        if ((m >> lane) & 1ull) result = old + (unsigned long long)__popcll(m & lt);
+        todo &= ~m;
+    }
+    if ((todo >> lane) & 1ull) result = gatomic_add(&counter[t], 1ull);
+    return result;
+}
+
 __device__ __forceinline__ double block_sum(double v, double *red)
 {
     for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);

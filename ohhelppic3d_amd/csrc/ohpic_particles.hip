@@ -443,21 +443,23 @@ __global__ void k_bin_count(const InstDev *__restrict__ insts, int inst, int s, 
     const InstDev *I = insts + inst;
     const SpecDev &S = I->sp[s];
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    double x = ((const double *)S.buf[lbuf][0])[i];
-    double y = ((const double *)S.buf[lbuf][1])[i];
-    double z = ((const double *)S.buf[lbuf][2])[i];
-    if (wrap) {
-        x = apply_periodic(apply_periodic(x, I->n[0]), I->n[0]);
-        y = apply_periodic(apply_periodic(y, I->n[1]), I->n[1]);
-        z = apply_periodic(apply_periodic(z, I->n[2]), I->n[2]);
+    const bool live = i < n;
+    double x = 0.0, y = 0.0, z = 0.0;
+    if (live) {
+        x = gload((const double *)S.buf[lbuf][0], i);
+        y = gload((const double *)S.buf[lbuf][1], i);
+        z = gload((const double *)S.buf[lbuf][2], i);
+        if (wrap) {
+            x = apply_periodic(apply_periodic(x, I->n[0]), I->n[0]);
+            y = apply_periodic(apply_periodic(y, I->n[1]), I->n[1]);
+            z = apply_periodic(apply_periodic(z, I->n[2]), I->n[2]);
+        }
     }
-    if (!in_box(I, x, y, z)) {
-        gatomic_add(&I->flags[kFlagRange], 1ull);
-        return;
-    }
-    const int t = ((int)x) / tx_ + I->nt[0] * (((int)y) / ty_ + I->nt[1] * (((int)z) / tz_));
-    atomicAdd(&S.count[lbuf ^ 1][t], 1ull);
+    const bool inb = live && in_box(I, x, y, z);
+    if (live && !inb) gatomic_add(&I->flags[kFlagRange], 1ull);
+    const int t =
+        inb ? ((int)x) / tx_ + I->nt[0] * (((int)y) / ty_ + I->nt[1] * (((int)z) / tz_)) : 0;
+    (void)wave_counter_add(S.count[lbuf ^ 1], t, inb);
 }
 
 // Scatter the list into the front runs of buffer lbuf^1 (words zeroed on
@@ -468,21 +470,25 @@ __global__ void k_bin_scatter(const InstDev *__restrict__ insts, int inst, int s
     const InstDev *I = insts + inst;
     const SpecDev &S = I->sp[s];
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    const bool live = i < n;
     const bool ids = I->track_ids != 0;
-    P64 q = load_p64(S, lbuf, i, ids);
-    if (wrap) {
-        q.x = apply_periodic(apply_periodic(q.x, I->n[0]), I->n[0]);
-        q.y = apply_periodic(apply_periodic(q.y, I->n[1]), I->n[1]);
-        q.z = apply_periodic(apply_periodic(q.z, I->n[2]), I->n[2]);
+    P64 q{0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0};
+    if (live) {
+        q = load_p64(S, lbuf, i, ids);
+        if (wrap) {
+            q.x = apply_periodic(apply_periodic(q.x, I->n[0]), I->n[0]);
+            q.y = apply_periodic(apply_periodic(q.y, I->n[1]), I->n[1]);
+            q.z = apply_periodic(apply_periodic(q.z, I->n[2]), I->n[2]);
+        }
     }
-    if (!in_box(I, q.x, q.y, q.z)) return;
-    const int t =
-        ((int)q.x) / tx_ + I->nt[0] * (((int)q.y) / ty_ + I->nt[1] * (((int)q.z) / tz_));
-    const unsigned long long old = atomicAdd(&S.count[lbuf ^ 1][t], 1ull);
+    const bool inb = live && in_box(I, q.x, q.y, q.z);
+    const int t = inb ? ((int)q.x) / tx_ + I->nt[0] * (((int)q.y) / ty_ + I->nt[1] * (((int)q.z) / tz_))
+                      : 0;
+    const unsigned long long old = wave_counter_add(S.count[lbuf ^ 1], t, inb);
+    if (!inb) return;
     const int slot = bin_front(old);
-    if (slot < S.bin_cap[t])
-        store_p64(S, lbuf ^ 1, S.bin_base[t] + slot, q, ids);
+    if (slot < gload(S.bin_cap, t))
+        store_p64(S, lbuf ^ 1, gload(S.bin_base, t) + slot, q, ids);
     else
         gatomic_add(&I->flags[kFlagOverflow], 1ull);
 }

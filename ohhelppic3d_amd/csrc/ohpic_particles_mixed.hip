@@ -570,14 +570,16 @@ __global__ void k_bin_count_mx(const InstDev *__restrict__ insts, int inst, int 
     const InstDev *I = insts + inst;
     const SpecDev &S = I->sp[s];
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const PMx q = load_pmx(S, lbuf, i, false);
-    const int t = ((const int32_t *)S.buf[lbuf][0])[i];
-    if (!list_valid(I, t, q, tx_, ty_, tz_)) {
-        gatomic_add(&I->flags[kFlagRange], 1ull);
-        return;
+    const bool live = i < n;
+    PMx q{{0u, 0u, 0u}, {0.0f, 0.0f, 0.0f}, 0};
+    int t = 0;
+    if (live) {
+        q = load_pmx(S, lbuf, i, false);
+        t = gload((const int32_t *)S.buf[lbuf][0], i);
     }
-    atomicAdd(&S.count[lbuf ^ 1][t], 1ull);
+    const bool ok = live && list_valid(I, t, q, tx_, ty_, tz_);
+    if (live && !ok) gatomic_add(&I->flags[kFlagRange], 1ull);
+    (void)wave_counter_add(S.count[lbuf ^ 1], ok ? t : 0, ok);
 }
 
 __global__ void k_bin_scatter_mx(const InstDev *__restrict__ insts, int inst, int s, int lbuf,
@@ -586,15 +588,20 @@ __global__ void k_bin_scatter_mx(const InstDev *__restrict__ insts, int inst, in
     const InstDev *I = insts + inst;
     const SpecDev &S = I->sp[s];
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    const bool live = i < n;
     const bool ids = I->track_ids != 0;
-    const PMx q = load_pmx(S, lbuf, i, ids);
-    const int t = ((const int32_t *)S.buf[lbuf][0])[i];
-    if (!list_valid(I, t, q, tx_, ty_, tz_)) return;
-    const unsigned long long old = atomicAdd(&S.count[lbuf ^ 1][t], 1ull);
+    PMx q{{0u, 0u, 0u}, {0.0f, 0.0f, 0.0f}, 0};
+    int t = 0;
+    if (live) {
+        q = load_pmx(S, lbuf, i, ids);
+        t = gload((const int32_t *)S.buf[lbuf][0], i);
+    }
+    const bool ok = live && list_valid(I, t, q, tx_, ty_, tz_);
+    const unsigned long long old = wave_counter_add(S.count[lbuf ^ 1], ok ? t : 0, ok);
+    if (!ok) return;
     const int slot = bin_front(old);
-    if (slot < S.bin_cap[t])
-        store_pmx(S, lbuf ^ 1, S.bin_base[t] + slot, q, ids);
+    if (slot < gload(S.bin_cap, t))
+        store_pmx(S, lbuf ^ 1, gload(S.bin_base, t) + slot, q, ids);
     else
         gatomic_add(&I->flags[kFlagOverflow], 1ull);
 }

@@ -37,3 +37,21 @@ def sim_for(sp, n, dt=0.1, b0=(0.0, 0.0, 0.5), weight_order=0, track_ids=True, *
 def by_pid(st):
     o = np.argsort(st["pid"], kind="stable")
     return {k: v[o] for k, v in st.items()}
+
+
+def energy_histories(ohp, orc, precision, steps, n=(16, 16, 16), ppc=16):
+    """Total and field energy per step of the GPU run and of the FP64 oracle
+    from the same seeded state (config-0 physics)."""
+    sp = two_species(orc, n, ppc)
+    osim = orc.OracleSim(n, sp, 0.1, (0, 0, 0.5), 0)
+    tot_g, tot_o, fe_g, fe_o = [], [], [], []
+    with sim_for(sp, n, track_ids=False, precision=precision) as sim:
+        for _ in range(steps):
+            sim.step(1)
+            osim.step(1)
+            d = sim.diagnostics()
+            tot_g.append(d["total_energy"])
+            fe_g.append(d["field_energy"])
+            tot_o.append(0.5 * (osim.ke_old.sum() + osim.ke_new.sum()) + osim.field_energy)
+            fe_o.append(osim.field_energy)
+    return (np.array(tot_g), np.array(tot_o), np.array(fe_g), np.array(fe_o))

@@ -1,11 +1,13 @@
-"""GPU checks of the mixed-precision form (int32 cell + FP32 offset, FP32 v,
-FP32 push/gather/deposit, FP64 charge grid, FFT, E and diagnostics) against
+"""GPU checks of the mixed-precision form (tile-relative cell + 24-bit fixed-point
+in-cell offset per axis, FP32 v, FP32 push/gather/deposit, FP64 charge grid,
+FFT, E and diagnostics) against
 the FP64 oracle.  Bars: rho and E within 1e-5 relative L2 of FP64 after a step
-from the same state; energy drift under 1% over a few hundred steps."""
+from the same state; over 1000 steps the total energy within 0.5% of the
+FP64 oracle's and the drift the mixed form adds below 0.25%."""
 import numpy as np
 import pytest
 
-from helpers import by_pid, rel_l2, sim_for, two_species
+from helpers import by_pid, energy_histories, rel_l2, sim_for, two_species
 
 pytestmark = pytest.mark.gpu
 
@@ -47,22 +49,17 @@ def test_mixed_one_step_vs_fp64_oracle(ohp, orc):
 
 
 def test_mixed_energy_drift_and_history(ohp, orc):
-    n = (16, 16, 16)
-    sp = two_species(orc, n, 16)
-    osim = orc.OracleSim(n, sp, 0.1, (0, 0, 0.5), 0)
-    hm, ho = [], []
-    with sim_for(sp, n, precision=ohp.MIXED, track_ids=False) as sim:
-        for _ in range(300):
-            sim.step(1)
-            osim.step(1)
-            hm.append(sim.diagnostics()["total_energy"])
-            ho.append(0.5 * (osim.ke_old.sum() + osim.ke_new.sum()) + osim.field_energy)
-    hm, ho = np.array(hm), np.array(ho)
-    drift = np.max(np.abs(hm - hm[0])) / hm[0]
-    print(f"mixed energy drift over 300 steps: {drift:.2e}; "
-          f"max |mixed - FP64| / FP64 = {np.max(np.abs(hm - ho) / ho):.2e}")
-    assert drift <= 0.01
-    assert np.max(np.abs(hm - ho) / ho) <= 0.01
+    # 1000 steps of the mixed form against the FP64 oracle: total energy
+    # within 0.5% at every step, and the drift the mixed form adds to the
+    # algorithm's own (FP64) drift below 0.25%
+    tot_m, tot_o, fe_m, fe_o = energy_histories(ohp, orc, ohp.MIXED, 1000)
+    drift_m = np.max(np.abs(tot_m - tot_m[0])) / tot_m[0]
+    drift_o = np.max(np.abs(tot_o - tot_o[0])) / tot_o[0]
+    print(f"energy drift over 1000 steps: mixed {drift_m:.3e}, FP64 oracle {drift_o:.3e}; "
+          f"max |mixed - FP64| / FP64 = {np.max(np.abs(tot_m - tot_o) / tot_o):.2e}")
+    assert np.max(np.abs(tot_m - tot_o) / tot_o) <= 0.005
+    assert abs(drift_m - drift_o) <= 0.0025
+    assert abs(fe_m[500:].mean() - fe_o[500:].mean()) <= 0.01 * fe_o[500:].mean()
 
 
 def test_mixed_separate_operators_equal_fused(ohp, orc):

@@ -7,7 +7,7 @@ the arithmetic is exact (wrap edge cases, integer work).
 import numpy as np
 import pytest
 
-from helpers import by_pid, rel_l2, sim_for, two_species
+from helpers import by_pid, energy_histories, rel_l2, sim_for, two_species
 
 pytestmark = pytest.mark.gpu
 
@@ -197,25 +197,19 @@ def test_step_is_deterministic(ohp, orc):
 
 
 def test_energy_history_matches_oracle(ohp, orc):
-    n = (16, 16, 16)
-    sp = two_species(orc, n, 16)
-    osim = orc.OracleSim(n, sp, 0.1, (0, 0, 0.5), 0)
-    hist_g, hist_o, fe_g, fe_o = [], [], [], []
-    with sim_for(sp, n, track_ids=False) as sim:
-        for _ in range(200):
-            sim.step(1)
-            osim.step(1)
-            d = sim.diagnostics()
-            hist_g.append(d["total_energy"])
-            fe_g.append(d["field_energy"])
-            hist_o.append(0.5 * (osim.ke_old.sum() + osim.ke_new.sum()) + osim.field_energy)
-            fe_o.append(osim.field_energy)
-    hist_g, hist_o = np.array(hist_g), np.array(hist_o)
-    fe_g, fe_o = np.array(fe_g), np.array(fe_o)
-    assert np.max(np.abs(hist_g - hist_o) / np.abs(hist_o)) <= 0.005
-    assert np.max(np.abs(fe_g - fe_o)) <= 0.005 * np.max(np.abs(fe_o))
-    # total energy drift of the GPU run itself stays small
-    assert np.max(np.abs(hist_g - hist_g[0])) / hist_g[0] <= 0.02
+    # 1000 steps: total energy within 0.5% of the oracle at every step; the
+    # field energy (a sum over fluctuating modes of a chaotic system) within
+    # 0.5% at every step until the two FP64 runs decorrelate (measured: first
+    # 0.5% deviation near step 340-370), and its time mean over the last 500
+    # steps within 1% afterwards
+    tot_g, tot_o, fe_g, fe_o = energy_histories(ohp, orc, ohp.FP64, 1000)
+    assert np.max(np.abs(tot_g - tot_o) / np.abs(tot_o)) <= 0.005
+    assert np.max(np.abs(fe_g[:250] - fe_o[:250]) / np.abs(fe_o[:250])) <= 0.005
+    assert abs(fe_g[500:].mean() - fe_o[500:].mean()) <= 0.01 * fe_o[500:].mean()
+    # the GPU run's own drift equals the algorithm's (the oracle's) to 0.25%
+    drift_g = np.max(np.abs(tot_g - tot_g[0])) / tot_g[0]
+    drift_o = np.max(np.abs(tot_o - tot_o[0])) / tot_o[0]
+    assert abs(drift_g - drift_o) <= 0.0025
 
 
 # ------------------------------------------------------------------ boundaries
