@@ -77,7 +77,9 @@ def run_gpu(args, workload=None):
         sim.step(1)
         log(f"[bench] warmup step {i + 1}/{args.warmup} done")
     sim.synchronize()
-    e_start = sim.diagnostics()["total_energy"]
+    # energy drift is measured from the first diagnostics available: after
+    # the warm-up, or after the first timed step without warm-up
+    e_start = sim.diagnostics()["total_energy"] if args.warmup > 0 else None
     sim.enable_timing(True)
     per_step = []
     t_start = time.perf_counter()
@@ -85,6 +87,8 @@ def run_gpu(args, workload=None):
         ts = time.perf_counter()
         sim.step(1)
         per_step.append(time.perf_counter() - ts)
+        if e_start is None:
+            e_start = sim.diagnostics()["total_energy"]
         if (i + 1) % max(1, args.steps // 5) == 0:
             log(f"[bench] timed step {i + 1}/{args.steps}: {per_step[-1] * 1e3:.2f} ms")
     sim.synchronize()
@@ -132,7 +136,8 @@ def run_gpu(args, workload=None):
         "field_solve_avg_ms": solve_ms / max(solve_n, 1),
         "energy": {"total": diag["total_energy"], "field": diag["field_energy"],
                    "repacks": diag["repacks"],
-                   "drift_over_timed_steps": abs(diag["total_energy"] - e_start) / abs(e_start)},
+                   "drift_over_timed_steps": (abs(diag["total_energy"] - e_start) / abs(e_start)
+                                              if e_start else None)},
     }
     return res
 
