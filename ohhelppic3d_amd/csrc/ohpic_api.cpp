@@ -1084,9 +1084,9 @@ int ohpic_create(const ohpic_config *cfg, ohpic_ctx **out)
         return fail(nullptr, OHPIC_EINVAL, "nspecies must be in [1, %d]", OHPIC_MAX_SPECIES);
     if (c.precision != OHPIC_FP64 && c.precision != OHPIC_MIXED)
         return fail(nullptr, OHPIC_EINVAL, "unknown precision mode %d", c.precision);
-    if (c.precision == OHPIC_MIXED && (c.nx > 1024 || c.ny > 1024 || c.nz > 1024))
-        return fail(nullptr, OHPIC_EINVAL, "mixed precision packs 10-bit cell coordinates: "
-                    "axis lengths must be <= 1024");
+    if (c.nx > kMaxFftLen || c.ny > kMaxFftLen || c.nz > kMaxFftLen)
+        return fail(nullptr, OHPIC_EINVAL, "axis lengths must be <= %d (one FFT line per LDS block)",
+                    kMaxFftLen);
     if (c.weight_order != 0 && c.weight_order != 1)
         return fail(nullptr, OHPIC_EINVAL, "weight_order must be 0 or 1");
     for (int s = 0; s < c.nspecies; ++s) {

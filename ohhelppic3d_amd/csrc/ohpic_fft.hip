@@ -322,8 +322,14 @@ __device__ void build_packed_inplace(cplx *buf, int n, int nlines, uint32_t mg_n
 // line group; groups beyond a job's count exit).  FROM_SRC reads job.src
 // (rho) instead of transforming job.data in place; SCALE multiplies the
 // output by job.inv_n (the final HC2R pass).
+#ifndef OHPIC_FFT_MIN_WAVES
+#define OHPIC_FFT_MIN_WAVES 8
+#endif
+
 template <int KIND, int AXIS, int RS, bool FROM_SRC, bool SCALE>
-__global__ void __launch_bounds__(kFftBlock) k_fft_pass(const FftJob *__restrict__ jobs)
+__global__ void __launch_bounds__(kFftBlock)
+    __attribute__((amdgpu_waves_per_eu(OHPIC_FFT_MIN_WAVES, 8)))
+    k_fft_pass(const FftJob *__restrict__ jobs)
 {
     __shared__ cplx buf[kFftLdsCplx];
     __shared__ double sLamXY[KIND == kFftPoisson ? 64 : 1];
