@@ -508,7 +508,7 @@ int do_deposit(ohpic_ctx *ctx)
 // rho from the int64 grid (from_acc), five axis passes, node E + energies.
 hipError_t solve_passes(const InstDev *insts, const int *parity, const FftJob *jobs, int n,
                         bool from_acc, int64_t max_ng, const int64_t grp[3], const int rs[3],
-                        int max_rows, hipStream_t st)
+                        int max_eblocks, hipStream_t st)
 {
     hipError_t e = hipSuccess;
     if (from_acc) e = launch_rho_from_acc_b(insts, parity, n, max_ng, st);
@@ -517,7 +517,7 @@ hipError_t solve_passes(const InstDev *insts, const int *parity, const FftJob *j
     if (e == hipSuccess) e = fft_pass(kPassPoissonZ, jobs, n, grp[2], rs[2], st);
     if (e == hipSuccess) e = fft_pass(kPassHc2rY, jobs, n, grp[1], rs[1], st);
     if (e == hipSuccess) e = fft_pass(kPassHc2rXScaled, jobs, n, grp[0], rs[0], st);
-    if (e == hipSuccess) e = launch_efield_b(insts, n, max_rows, st);
+    if (e == hipSuccess) e = launch_efield_b(insts, n, max_eblocks, st);
     return e;
 }
 
@@ -525,7 +525,8 @@ int do_solve(ohpic_ctx *ctx, bool from_acc)
 {
     timer_begin(ctx, 1);
     HIPCHK(ctx, solve_passes(ctx->d_inst, parity_of(ctx), ctx->d_job, 1, from_acc, ctx->ng,
-                             ctx->fft_grp, ctx->fft_rs, ctx->n[1] * ctx->n[2], ctx->st));
+                             ctx->fft_grp, ctx->fft_rs,
+                             efield_blocks_of(ctx->n[0], ctx->n[1], ctx->n[2]), ctx->st));
     timer_end(ctx, 1);
     return OHPIC_OK;
 }
@@ -733,7 +734,7 @@ struct ohpic_group {
     int nitems = 0;
     bool dirty = true;
     int64_t max_ng = 0;
-    int max_ntiles = 0, max_rows = 0;
+    int max_ntiles = 0, max_rows = 0;  // max_rows: largest E-kernel block count
     int64_t grp[3]{};
     int rs[3]{};
     int T[3]{};
@@ -854,7 +855,7 @@ int group_refresh(ohpic_group *g)
             }
             g->max_ng = std::max(g->max_ng, m->ng);
             g->max_ntiles = std::max(g->max_ntiles, m->ntiles);
-            g->max_rows = std::max(g->max_rows, m->n[1] * m->n[2]);
+            g->max_rows = std::max(g->max_rows, efield_blocks_of(m->n[0], m->n[1], m->n[2]));
             for (int a = 0; a < 3; ++a) {
                 g->grp[a] = std::max(g->grp[a], m->fft_grp[a]);
                 g->rs[a] = std::max(g->rs[a], m->fft_rs[a]);

@@ -2,6 +2,8 @@
 // with energy diagnostics) and the seeded particle loaders for gfx950.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include <cstdint>
 
 #include "ohpic_internal.hpp"
@@ -42,13 +44,17 @@ __global__ void __launch_bounds__(256) k_efield_b(const InstDev *__restrict__ in
     double *__restrict__ ez = I->e[2];
     const int nx = I->n[0], ny = I->n[1], nz = I->n[2];
     const int rows = ny * nz;
-    // the block count depends on this instance only, so the partial sums
-    // (and the diagnostics) do not depend on the other instances
-    const int nblk = rows < kEfieldMaxBlocks ? rows : kEfieldMaxBlocks;
+    const int rpb = efield_rows_per_block(nx);
+    const int nblk = efield_blocks_of(nx, ny, nz);
     if ((int)blockIdx.x >= nblk) return;
+    // thread -> (row within the block's rows, first x); x steps by nx when
+    // rows share the block, by the block size when one row spans it
+    const int tsub = rpb > 1 ? (int)threadIdx.x / nx : 0;
+    const int tx0 = rpb > 1 ? (int)threadIdx.x - tsub * nx : (int)threadIdx.x;
+    const int xstep = rpb > 1 ? nx : (int)blockDim.x;
     const int64_t sy = nx, sz = (int64_t)nx * ny;
     double we = 0.0, we2 = 0.0, qn = 0.0;
-    for (int r = blockIdx.x; r < rows; r += nblk) {
+    for (int r = blockIdx.x * rpb + tsub; tsub < rpb && r < rows; r += nblk * rpb) {
         const int iy = r % ny;
         const int iz = r / ny;
         const int64_t row = (int64_t)r * nx;
@@ -56,7 +62,7 @@ __global__ void __launch_bounds__(256) k_efield_b(const InstDev *__restrict__ in
         const int64_t dym = iy > 0 ? -sy : (int64_t)(ny - 1) * sy;
         const int64_t dzp = iz + 1 < nz ? sz : -(int64_t)(nz - 1) * sz;
         const int64_t dzm = iz > 0 ? -sz : (int64_t)(nz - 1) * sz;
-        for (int ix = threadIdx.x; ix < nx; ix += blockDim.x) {
+        for (int ix = tx0; ix < nx; ix += xstep) {
             const int64_t o = row + ix;
             const int64_t oxp = ix + 1 < nx ? o + 1 : row;
             const int64_t oxm = ix > 0 ? o - 1 : row + nx - 1;
@@ -118,8 +124,7 @@ __global__ void __launch_bounds__(256) k_reduce_e_b(const InstDev *__restrict__ 
 {
     __shared__ double red[256];
     const InstDev *I = insts + blockIdx.x;
-    const int rows = I->n[1] * I->n[2];
-    const int count = rows < kEfieldMaxBlocks ? rows : kEfieldMaxBlocks;
+    const int count = efield_blocks_of(I->n[0], I->n[1], I->n[2]);
     for (int k = 0; k < 3; ++k) {
         const double v = block_ordered_sum(I->e_partial, count, 3, k, red);
         if (threadIdx.x == 0) I->diag[kDiagWe + k] = v;
@@ -281,20 +286,28 @@ static unsigned grid_cap(int64_t n, int64_t cap)
     return (unsigned)b;
 }
 
+// Blocks per instance of the grid-stride grid kernels: enough for n
+// elements, at most 4096, and within a budget of about 16K blocks per launch
+// shared by the instances (small instances need few).
+static unsigned grid_blocks(int64_t n, int ninst)
+{
+    const int64_t budget = std::max<int64_t>(32, 16384 / std::max(ninst, 1));
+    return grid_cap(n, std::min<int64_t>(4096, budget));
+}
+
 hipError_t launch_rho_from_acc_b(const InstDev *insts, const int *parity, int ninst,
                                  int64_t max_ng, hipStream_t st)
 {
     if (ninst <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_rho_from_acc_b, dim3(grid_cap(max_ng, 4096), ninst), dim3(256), 0, st,
-                       insts, parity);
+    hipLaunchKernelGGL(k_rho_from_acc_b, dim3(grid_blocks(max_ng, ninst), ninst), dim3(256), 0,
+                       st, insts, parity);
     return hipGetLastError();
 }
 
-hipError_t launch_efield_b(const InstDev *insts, int ninst, int max_rows, hipStream_t st)
+hipError_t launch_efield_b(const InstDev *insts, int ninst, int max_blocks, hipStream_t st)
 {
     if (ninst <= 0) return hipSuccess;
-    const int blocks = max_rows < kEfieldMaxBlocks ? (max_rows > 0 ? max_rows : 1)
-                                                   : kEfieldMaxBlocks;
+    const int blocks = max_blocks > 0 ? max_blocks : 1;
     hipLaunchKernelGGL(k_efield_b, dim3(blocks, ninst), dim3(256), 0, st, insts);
     hipLaunchKernelGGL(k_reduce_e_b, dim3(ninst), dim3(256), 0, st, insts);
     return hipGetLastError();
@@ -306,8 +319,8 @@ hipError_t launch_prepare_next_b(const InstDev *insts, const int *parity, int ni
 {
     if (ninst <= 0) return hipSuccess;
     const int64_t n = max_ng > max_ntiles ? max_ng : max_ntiles;
-    hipLaunchKernelGGL(k_prepare_next_b, dim3(grid_cap(n, 4096), ninst), dim3(256), 0, st, insts,
-                       parity, queue);
+    hipLaunchKernelGGL(k_prepare_next_b, dim3(grid_blocks(n, ninst), ninst), dim3(256), 0, st,
+                       insts, parity, queue);
     return hipGetLastError();
 }
 

@@ -91,6 +91,17 @@ struct WorkItem {
 constexpr int kKeSlots = 2 * kMaxSpecies;  // per item: ke_old[8], ke_new[8]
 constexpr int kEfieldMaxBlocks = 4096;     // rows of e_partial
 
+// k_efield_b: x rows shorter than 256 nodes share a 256-thread block
+// (256 / nx of them); the instance's block count depends on its grid only,
+// so its energy partial sums do not depend on the other instances.
+__host__ __device__ inline int efield_rows_per_block(int nx) { return nx >= 256 ? 1 : 256 / nx; }
+__host__ __device__ inline int efield_blocks_of(int nx, int ny, int nz)
+{
+    const int rpb = efield_rows_per_block(nx);
+    const int64_t need = ((int64_t)ny * nz + rpb - 1) / rpb;
+    return (int)(need < kEfieldMaxBlocks ? need : kEfieldMaxBlocks);
+}
+
 // Records the message ohpic_last_error(NULL) returns (host side).
 void set_global_error(const char *msg);
 

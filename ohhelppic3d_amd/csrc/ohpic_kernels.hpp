@@ -85,7 +85,8 @@ hipError_t launch_rho_from_acc_b(const InstDev *insts, const int *parity, int ni
                                  int64_t max_ng, hipStream_t st);
 // node E from phi + block partials of sum rho*phi, sum E^2, sum rho; then the
 // partials are reduced into diag[kDiagWe..kDiagQnet]
-hipError_t launch_efield_b(const InstDev *insts, int ninst, int max_rows, hipStream_t st);
+// max_blocks: the largest efield_blocks_of() over the instances
+hipError_t launch_efield_b(const InstDev *insts, int ninst, int max_blocks, hipStream_t st);
 // zero the next buffer's bin words and charge grid, the flags, and *queue
 // (if not null: the counter of a following launch_particles_queue)
 hipError_t launch_prepare_next_b(const InstDev *insts, const int *parity, int ninst,

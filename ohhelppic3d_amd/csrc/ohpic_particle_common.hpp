@@ -69,6 +69,29 @@ __device__ __forceinline__ void gstore(T *p, int64_t i, T v)
     as_global(p)[i] = v;
 }
 
+// Particle state streams through each step once: with OHPIC_NT set, its
+// loads and stores carry the non-temporal hint (measured 3-7% slower on the
+// 256^3 x 64 ppc step, so off by default).
+#ifndef OHPIC_NT
+#define OHPIC_NT 0
+#endif
+
+template <typename T>
+__device__ __forceinline__ T sload(const T *p, int64_t i)
+{
+    if (OHPIC_NT) return __builtin_nontemporal_load(&as_global(p)[i]);
+    return as_global(p)[i];
+}
+
+template <typename T>
+__device__ __forceinline__ void sstore(T *p, int64_t i, T v)
+{
+    if (OHPIC_NT)
+        __builtin_nontemporal_store(v, &as_global(p)[i]);
+    else
+        as_global(p)[i] = v;
+}
+
 // relaxed device-scope atomic add on global memory (as atomicAdd)
 __device__ __forceinline__ unsigned long long gatomic_add(unsigned long long *p,
                                                           unsigned long long v)

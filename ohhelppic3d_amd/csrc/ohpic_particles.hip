@@ -36,24 +36,24 @@ struct P64 {
 __device__ __forceinline__ void store_p64(const SpecDev &S, int nbuf, int64_t o, const P64 &p,
                                           bool ids)
 {
-    gstore((double *)S.buf[nbuf][0], o, p.x);
-    gstore((double *)S.buf[nbuf][1], o, p.y);
-    gstore((double *)S.buf[nbuf][2], o, p.z);
-    gstore((double *)S.buf[nbuf][3], o, p.vx);
-    gstore((double *)S.buf[nbuf][4], o, p.vy);
-    gstore((double *)S.buf[nbuf][5], o, p.vz);
+    sstore((double *)S.buf[nbuf][0], o, p.x);
+    sstore((double *)S.buf[nbuf][1], o, p.y);
+    sstore((double *)S.buf[nbuf][2], o, p.z);
+    sstore((double *)S.buf[nbuf][3], o, p.vx);
+    sstore((double *)S.buf[nbuf][4], o, p.vy);
+    sstore((double *)S.buf[nbuf][5], o, p.vz);
     if (ids) gstore(S.pid[nbuf], o, p.id);
 }
 
 __device__ __forceinline__ P64 load_p64(const SpecDev &S, int b, int64_t o, bool ids)
 {
     P64 q;
-    q.x = gload((const double *)S.buf[b][0], o);
-    q.y = gload((const double *)S.buf[b][1], o);
-    q.z = gload((const double *)S.buf[b][2], o);
-    q.vx = gload((const double *)S.buf[b][3], o);
-    q.vy = gload((const double *)S.buf[b][4], o);
-    q.vz = gload((const double *)S.buf[b][5], o);
+    q.x = sload((const double *)S.buf[b][0], o);
+    q.y = sload((const double *)S.buf[b][1], o);
+    q.z = sload((const double *)S.buf[b][2], o);
+    q.vx = sload((const double *)S.buf[b][3], o);
+    q.vy = sload((const double *)S.buf[b][4], o);
+    q.vz = sload((const double *)S.buf[b][5], o);
     q.id = ids ? gload(S.pid[b], o) : 0;
     return q;
 }

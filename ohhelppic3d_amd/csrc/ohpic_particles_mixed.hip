@@ -112,8 +112,8 @@ __device__ __forceinline__ PMx load_bin(const BinView &bv, uint32_t slot, bool i
     PMx q;
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-        q.w[a] = *as_global((const uint32_t *)(bv.w[a] + off));
-        q.v[a] = *as_global((const float *)(bv.v[a] + off));
+        q.w[a] = sload((const uint32_t *)(bv.w[a] + off), 0);
+        q.v[a] = sload((const float *)(bv.v[a] + off), 0);
     }
     q.id = ids ? gload(bv.id, (int64_t)slot) : 0;
     return q;
@@ -124,8 +124,8 @@ __device__ __forceinline__ void store_bin(const BinView &bv, uint32_t slot, cons
     const uint32_t off = slot * 4u;
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-        *as_global((uint32_t *)(bv.w[a] + off)) = q.w[a];
-        *as_global((float *)(bv.v[a] + off)) = q.v[a];
+        sstore((uint32_t *)(bv.w[a] + off), 0, q.w[a]);
+        sstore((float *)(bv.v[a] + off), 0, q.v[a]);
     }
     if (ids) gstore(const_cast<int64_t *>(bv.id), (int64_t)slot, q.id);
 }
