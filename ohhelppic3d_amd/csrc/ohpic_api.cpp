@@ -335,7 +335,10 @@ int set_bins(ohpic_ctx *ctx, int s, const std::vector<int64_t> &counts)
     for (int t = 0; t < nt; ++t) {
         const int64_t c = counts[t];
         int64_t cap = round_up((int64_t)std::ceil((double)c * (1.0 + slack)) + 16, 64);
-        if (cap > INT32_MAX) return fail(ctx, OHPIC_ENOMEM, "bin capacity exceeds int32");
+        // the particle kernels address a bin with 32-bit byte offsets
+        if (cap >= (int64_t)1 << 29)
+            return fail(ctx, OHPIC_ENOMEM, "bin %d needs %lld slots (limit 2^29): use smaller tiles",
+                        t, (long long)cap);
         S.base[t] = off;
         S.cap[t] = (int32_t)cap;
         off += cap;

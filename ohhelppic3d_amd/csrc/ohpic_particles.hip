@@ -58,6 +58,51 @@ __device__ __forceinline__ P64 load_p64(const SpecDev &S, int b, int64_t o, bool
     return q;
 }
 
+// The bin of one tile in one buffer: array bases offset to the bin's first
+// element (uniform per workgroup: scalar registers) and particles addressed
+// by a 32-bit byte offset (bins hold < 2^29 particles), so per-particle loads
+// and stores need no 64-bit address arithmetic.
+struct BinView64 {
+    const char *c[6];
+    const int64_t *id;
+};
+
+__device__ __forceinline__ BinView64 bin_view64(const SpecDev &S, int b, int64_t base, bool ids)
+{
+    BinView64 r;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) r.c[k] = (const char *)((const double *)S.buf[b][k] + base);
+    r.id = ids ? S.pid[b] + base : nullptr;
+    return r;
+}
+
+__device__ __forceinline__ P64 load_bin64(const BinView64 &bv, uint32_t slot, bool ids)
+{
+    const uint32_t off = slot * 8u;
+    P64 q;
+    q.x = sload((const double *)(bv.c[0] + off), 0);
+    q.y = sload((const double *)(bv.c[1] + off), 0);
+    q.z = sload((const double *)(bv.c[2] + off), 0);
+    q.vx = sload((const double *)(bv.c[3] + off), 0);
+    q.vy = sload((const double *)(bv.c[4] + off), 0);
+    q.vz = sload((const double *)(bv.c[5] + off), 0);
+    q.id = ids ? gload(bv.id, (int64_t)slot) : 0;
+    return q;
+}
+
+__device__ __forceinline__ void store_bin64(const BinView64 &bv, uint32_t slot, const P64 &q,
+                                            bool ids)
+{
+    const uint32_t off = slot * 8u;
+    sstore((double *)(bv.c[0] + off), 0, q.x);
+    sstore((double *)(bv.c[1] + off), 0, q.y);
+    sstore((double *)(bv.c[2] + off), 0, q.z);
+    sstore((double *)(bv.c[3] + off), 0, q.vx);
+    sstore((double *)(bv.c[4] + off), 0, q.vy);
+    sstore((double *)(bv.c[5] + off), 0, q.vz);
+    if (ids) gstore(const_cast<int64_t *>(bv.id), (int64_t)slot, q.id);
+}
+
 // Append one particle to the back run of bin t (slot from the packed word).
 __device__ __forceinline__ void route_back(const SpecDev &S, int nbuf, int t, const P64 &q,
                                            bool ids, unsigned long long *flags)
@@ -211,15 +256,18 @@ __global__ void __launch_bounds__(kBlock)
             const double s0 = 2 * t0 / (1 + tt), s1 = 2 * t1 / (1 + tt), s2 = 2 * t2 / (1 + tt);
             double keo = 0.0, ken = 0.0;
 
+            const BinView64 cur = bin_view64(S, par, base, ids);
+            const BinView64 nxt = bin_view64(S, nbuf, base, ids);
             P64 nextq{0, 0, 0, 0, 0, 0, 0};
-            if (start + tid < end) nextq = load_p64(S, par, base + bin_slot(start + tid, F, B, cap), ids);
+            if (start + tid < end)
+                nextq = load_bin64(cur, (uint32_t)bin_slot(start + tid, F, B, cap), ids);
             for (int p0 = start; p0 < end; p0 += kBlock) {
                 const int p = p0 + tid;
                 const bool active = p < end;
                 P64 q = nextq;
                 // prefetch the next iteration's particle while this one is processed
                 if (p + kBlock < end)
-                    nextq = load_p64(S, par, base + bin_slot(p + kBlock, F, B, cap), ids);
+                    nextq = load_bin64(cur, (uint32_t)bin_slot(p + kBlock, F, B, cap), ids);
                 if (PUSH && active) {
                     const int ix = (int)q.x, iy = (int)q.y, iz = (int)q.z;
                     double wx0, wx1, wy0, wy1, wz0, wz1;
@@ -265,7 +313,7 @@ __global__ void __launch_bounds__(kBlock)
                     q.z = q.z + q.vz * dt;
                 }
                 if (MODE == kModeGatherPush) {
-                    if (active) store_p64(S, par, base + bin_slot(p, F, B, cap), q, ids);
+                    if (active) store_bin64(cur, (uint32_t)bin_slot(p, F, B, cap), q, ids);
                     continue;
                 }
                 int ntile = tile;
@@ -354,7 +402,7 @@ __global__ void __launch_bounds__(kBlock)
                         if (stay) {
                             const int slot = b + (int)__popcll(m & lanemask_lt);
                             if (slot < lim)
-                                store_p64(S, nbuf, base + slot, q, ids);
+                                store_bin64(nxt, (uint32_t)slot, q, ids);
                             else
                                 gatomic_add(&flags[kFlagOverflow], 1ull);
                         }
