@@ -336,3 +336,60 @@ def test_errors_are_reported(ohp, orc):
         assert e.value.code == -4
         sim.boundary()
         sim.step(1)
+
+
+# ------------------------------------------------------------------ edge cases
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_partial_tiles_match_oracle(ohp, orc, precision):
+    # grid lengths that 8-cell tiles do not divide: the last tile of every
+    # axis is partly outside the box
+    n = (20, 12, 10)
+    sp = two_species(orc, n, 4)
+    osim = orc.OracleSim(n, sp, 0.1, (0, 0, 0.5), 0)
+    prec = ohp.FP64 if precision == "fp64" else ohp.MIXED
+    with sim_for(sp, n, tile=(8, 8, 8), precision=prec) as sim:
+        sim.step(2)
+        osim.step(2)
+        rho = sim.grid(ohp.FIELD_RHO)
+        counts = sim.cell_counts()
+        g = [by_pid(sim.store_particles(s, True)) for s in range(2)]
+    tol = TOL if precision == "fp64" else 1e-3
+    assert rel_l2(rho, osim.rho) <= tol
+    ref_counts = np.zeros(n[0] * n[1] * n[2], np.int32)
+    for s in range(2):
+        assert len(g[s]["x"]) == len(osim.sp[s]["x"])
+        assert np.all((g[s]["x"] >= 0) & (g[s]["x"] < n[0]) & (g[s]["z"] < n[2]))
+        ref_counts += orc.cell_counts(n, osim.sp[s]["x"], osim.sp[s]["y"], osim.sp[s]["z"])
+    if precision == "fp64":
+        assert np.array_equal(counts, ref_counts)
+    else:
+        assert counts.sum() == ref_counts.sum()
+
+
+def test_empty_species_and_empty_instance(ohp, orc):
+    n = (16, 16, 16)
+    sp = two_species(orc, n, 2)
+    empty = [dict(sp[0]), dict(sp[1])]
+    for k in ("x", "y", "z", "vx", "vy", "vz"):
+        empty[1][k] = empty[1][k][:0]
+    osim = orc.OracleSim(n, empty, 0.1, (0, 0, 0.5), 0)
+    with sim_for(empty, n) as sim:
+        assert sim.particle_count(1) == 0
+        sim.step(3)
+        osim.step(3)
+        assert rel_l2(sim.grid(ohp.FIELD_PHI), osim.phi) <= 1e-11
+        assert len(sim.store_particles(1)["x"]) == 0
+        d = sim.diagnostics()
+        assert d["ke_new"][1] == 0.0 and d["ke_old"][1] == 0.0
+    # an instance without particles inside a group steps as well
+    cfg = ohp.SimConfig(n=(16, 16, 16), qm=(-1.0, 0.01), q_macro=(-0.5, 0.5),
+                        capacity=(0, 0), track_ids=True)
+    with ohp.Simulation(cfg) as nothing, sim_for(sp, n) as full, sim_for(sp, n) as ref:
+        ref.step(2)
+        with ohp.Group() as grp:
+            grp.attach(nothing)
+            grp.attach(full)
+            grp.step(2)
+        assert np.array_equal(full.grid(ohp.FIELD_PHI), ref.grid(ohp.FIELD_PHI))
+        assert np.all(nothing.grid(ohp.FIELD_RHO) == 0.0)
+        assert nothing.diagnostics()["step"] == 2
