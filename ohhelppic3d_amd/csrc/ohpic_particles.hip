@@ -169,6 +169,7 @@ __global__ void __launch_bounds__(kBlock)
     __shared__ int sLcnt[kWaves];
     __shared__ double sRed[kWaves];
     __shared__ int sFront, sFrontLimit;
+    __shared__ int64_t sListBase;
 
     __shared__ int sItem;
     for (int iter = 0;; ++iter) {
@@ -235,8 +236,14 @@ __global__ void __launch_bounds__(kBlock)
             const int start = it.chunk * per;
             const int end = min(cnt, start + per);
             if (REBIN) {
+                // the previous species' particles are all stored before its
+                // list base and front counters are replaced
+                __syncthreads();
                 if (tid == 0) {
                     int lim = cap;
+                    if (LIST && end > start)  // one reservation per item and species
+                        sListBase = (int64_t)gatomic_add(&S.count[nbuf][0],
+                                                         (unsigned long long)(end - start));
                     if (!LIST && excl && cnt > 0) {
                         // reserve the whole front run: arrivals from neighbours stay behind it
                         const unsigned long long old =
@@ -370,16 +377,9 @@ __global__ void __launch_bounds__(kBlock)
                     }
                 }
                 if (LIST) {
-                    // retry path: every particle to the species' contiguous list
-                    const unsigned long long m = __ballot(active);
-                    if (m) {
-                        const int leader = __ffsll((unsigned long long)m) - 1;
-                        unsigned long long b = 0;
-                        if (lane_id() == leader)
-                            b = gatomic_add(&S.count[nbuf][0], (unsigned long long)__popcll(m));
-                        b = __shfl(b, leader, 64);
-                        if (active) store_p64(S, nbuf, (int64_t)b + __popcll(m & lanemask_lt), q, ids);
-                    }
+                    // retry path: every particle to the species' contiguous
+                    // list, in bin order inside this item's reserved range
+                    if (active) store_p64(S, nbuf, sListBase + (p - start), q, ids);
                 } else if (REBIN) {
                     // particles that stayed: front run of this tile's bin
                     const bool stay = active && ntile == tile;
