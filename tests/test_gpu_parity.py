@@ -393,3 +393,19 @@ def test_empty_species_and_empty_instance(ohp, orc):
         assert np.array_equal(full.grid(ohp.FIELD_PHI), ref.grid(ohp.FIELD_PHI))
         assert np.all(nothing.grid(ohp.FIELD_RHO) == 0.0)
         assert nothing.diagnostics()["step"] == 2
+
+
+def test_run_from_reference_parameters_matches_oracle(ohp, orc):
+    # the config-0 run described in the reference's parameters.toml layout:
+    # the device-seeded state and one step equal the oracle's
+    import os
+    from ohhelppic3d_amd import params
+    spec = params.load_parameters(os.path.join(os.path.dirname(__file__), "golden",
+                                               "parameters_config0.toml"))
+    sp = two_species(orc, spec.cfg.n, 16)
+    osim = orc.OracleSim(spec.cfg.n, sp, spec.cfg.dt, spec.cfg.b0, 0)
+    with spec.create() as sim:
+        sim.step(1)
+        osim.step(1)
+        assert rel_l2(sim.grid(ohp.FIELD_PHI), osim.phi) <= TOL
+        assert rel_l2(sim.grid(ohp.FIELD_RHO), osim.rho) <= TOL
