@@ -288,11 +288,11 @@ __global__ void __launch_bounds__(kBlock)
                     const int l0 = lx + EX * (ly + EY * lz);
                     double e0 = 0.0, e1 = 0.0, e2 = 0.0;
                     const double wx[2] = {wx0, wx1}, wy[2] = {wy0, wy1}, wz[2] = {wz0, wz1};
-    #pragma unroll
+#pragma unroll
                     for (int i = 0; i < 2; ++i)
-    #pragma unroll
+#pragma unroll
                         for (int j = 0; j < 2; ++j)
-    #pragma unroll
+#pragma unroll
                             for (int k = 0; k < 2; ++k) {
                                 const int l = l0 + i + EX * (j + EY * k);
                                 const double w = wx[i] * wy[j] * wz[k];
@@ -350,11 +350,11 @@ __global__ void __launch_bounds__(kBlock)
                     const int dz = wrap_index(iz - loz + 1, nz);
                     if (dx <= TX + 1 && dy <= TY + 1 && dz <= TZ + 1) {
                         const int l0 = dx + RX * (dy + RY * dz);
-    #pragma unroll
+#pragma unroll
                         for (int i = 0; i < 2; ++i)
-    #pragma unroll
+#pragma unroll
                             for (int j = 0; j < 2; ++j)
-    #pragma unroll
+#pragma unroll
                                 for (int k = 0; k < 2; ++k) {
                                     const long long v = fix_round(wx[i] * wy[j] * wz[k], qfix);
                                     atomicAdd(&sR[l0 + i + RX * (j + RY * k)],
@@ -362,11 +362,11 @@ __global__ void __launch_bounds__(kBlock)
                                 }
                     } else {
                         gatomic_add(&flags[kFlagFast], 1ull);
-    #pragma unroll
+#pragma unroll
                         for (int i = 0; i < 2; ++i)
-    #pragma unroll
+#pragma unroll
                             for (int j = 0; j < 2; ++j)
-    #pragma unroll
+#pragma unroll
                                 for (int k = 0; k < 2; ++k) {
                                     const int gx = (ix + i) % nx, gy = (iy + j) % ny,
                                               gz = (iz + k) % nz;

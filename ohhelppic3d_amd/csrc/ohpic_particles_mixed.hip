@@ -319,7 +319,7 @@ __global__ void __launch_bounds__(kBlock)
             const BinView cur = bin_view(S, par, base, ids);
             const BinView nxt = bin_view(S, nbuf, base, ids);
             PMx ahead[kMxPrefetch];
-    #pragma unroll
+#pragma unroll
             for (int k = 0; k < kMxPrefetch; ++k) {
                 ahead[k] = PMx{{0u, 0u, 0u}, {0.0f, 0.0f, 0.0f}, 0};
                 if (start + tid + k * kBlock < end)
@@ -330,7 +330,7 @@ __global__ void __launch_bounds__(kBlock)
                 const int p = p0 + tid;
                 const bool active = p < end;
                 PMx q = ahead[0];
-    #pragma unroll
+#pragma unroll
                 for (int k = 0; k + 1 < kMxPrefetch; ++k) ahead[k] = ahead[k + 1];
                 if (p + kMxPrefetch * kBlock < end)
                     ahead[kMxPrefetch - 1] =
@@ -349,11 +349,11 @@ __global__ void __launch_bounds__(kBlock)
                     const int l0 = lx + EX * (ly + EY * lz);
                     float e0 = 0.0f, e1 = 0.0f, e2 = 0.0f;
                     const float wx[2] = {wx0, wx1}, wy[2] = {wy0, wy1}, wz[2] = {wz0, wz1};
-    #pragma unroll
+#pragma unroll
                     for (int i = 0; i < 2; ++i)
-    #pragma unroll
+#pragma unroll
                         for (int j = 0; j < 2; ++j)
-    #pragma unroll
+#pragma unroll
                             for (int k = 0; k < 2; ++k) {
                                 const int l = l0 + i + EX * (j + EY * k);
                                 const float w = wx[i] * wy[j] * wz[k];
@@ -422,11 +422,11 @@ __global__ void __launch_bounds__(kBlock)
                     const int dz = wrap_index(gz - loz + 1, nz);
                     if (dx <= TX + 1 && dy <= TY + 1 && dz <= TZ + 1) {
                         const int l0 = dx + RX * (dy + RY * dz);
-    #pragma unroll
+#pragma unroll
                         for (int i = 0; i < 2; ++i)
-    #pragma unroll
+#pragma unroll
                             for (int j = 0; j < 2; ++j)
-    #pragma unroll
+#pragma unroll
                                 for (int k = 0; k < 2; ++k) {
                                     const long long v =
                                         fix_round((double)(wx[i] * wy[j] * wz[k]), qfix);
@@ -435,11 +435,11 @@ __global__ void __launch_bounds__(kBlock)
                                 }
                     } else {
                         gatomic_add(&flags[kFlagFast], 1ull);
-    #pragma unroll
+#pragma unroll
                         for (int i = 0; i < 2; ++i)
-    #pragma unroll
+#pragma unroll
                             for (int j = 0; j < 2; ++j)
-    #pragma unroll
+#pragma unroll
                                 for (int k = 0; k < 2; ++k) {
                                     const int hx = (gx + i) % nx, hy = (gy + j) % ny,
                                               hz = (gz + k) % nz;
@@ -493,7 +493,7 @@ __global__ void __launch_bounds__(kBlock)
                             if (leave) {
                                 const int slot = c + (int)__popcll(lm & lanemask_lt);
                                 LeaveBufMx &lb = sLeave[wv];
-    #pragma unroll
+#pragma unroll
                                 for (int a = 0; a < 3; ++a) {
                                     lb.w[a][slot] = q.w[a];
                                     lb.v[a][slot] = q.v[a];
