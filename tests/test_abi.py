@@ -83,3 +83,61 @@ def test_host_mirror_validation(L):
     assert q[0] == pytest.approx(-1 / 64) and q[1] == pytest.approx(1 / 64)
     with pytest.raises(ohp.OhpicError):
         ohp.SimConfig(qm=[-1.0], q_macro=[1.0, 2.0], capacity=[1]).to_c()
+
+
+def test_group_and_autotune_errors_without_gpu(L):
+    # argument checks come first; with no GPU in this process the group and
+    # context constructors report ENODEV instead of touching a device
+    lib = L.lib()
+    g = C.c_void_p()
+    assert lib.ohpic_group_create(0, None) == L.EINVAL
+    rc = lib.ohpic_group_create(0, C.byref(g))
+    assert rc in (L.ENODEV, L.OK)
+    if rc == L.OK:  # a GPU is visible: nothing else to check here
+        lib.ohpic_group_destroy(g)
+        return
+    assert lib.ohpic_group_step(None, 1) == L.EINVAL
+    assert lib.ohpic_group_attach(None, None) == L.EINVAL
+    assert lib.ohpic_group_size(None, None) == L.EINVAL
+    assert lib.ohpic_group_destroy(None) == L.OK
+    r = L.TuneResult()
+    assert lib.ohpic_autotune(None, 1, 1, 1, C.byref(r)) == L.EINVAL
+    c = L.Config()
+    lib.ohpic_config_default(C.byref(c))
+    assert lib.ohpic_autotune(C.byref(c), 0, 1, 1, C.byref(r)) == L.EINVAL
+    ctx = C.c_void_p()
+    assert lib.ohpic_create(C.byref(c), C.byref(ctx)) == L.ENODEV
+    assert b"device" in lib.ohpic_last_error(None).lower()
+
+
+def test_create_rejects_oversized_axes_and_bad_modes(L):
+    lib = L.lib()
+    c = L.Config()
+    ctx = C.c_void_p()
+    for field, value, msg in (("nx", 2048, b"axis lengths"), ("precision", 7, b"precision"),
+                              ("weight_order", 2, b"weight_order")):
+        lib.ohpic_config_default(C.byref(c))
+        setattr(c, field, value)
+        assert lib.ohpic_create(C.byref(c), C.byref(ctx)) == L.EINVAL, field
+        assert msg in lib.ohpic_last_error(None), field
+    lib.ohpic_config_default(C.byref(c))
+    c.qm[1] = 0.0
+    assert lib.ohpic_create(C.byref(c), C.byref(ctx)) == L.EINVAL
+    assert b"qm == 0" in lib.ohpic_last_error(None)
+
+
+def test_footprint_estimate_grows_with_particles_and_slack(L):
+    lib = L.lib()
+    c = L.Config()
+    lib.ohpic_config_default(C.byref(c))
+    b1, b2, b3 = C.c_int64(), C.c_int64(), C.c_int64()
+    assert lib.ohpic_estimate_footprint(C.byref(c), C.byref(b1)) == 0
+    c.capacity[0] *= 2
+    assert lib.ohpic_estimate_footprint(C.byref(c), C.byref(b2)) == 0
+    c.slack = 0.5
+    assert lib.ohpic_estimate_footprint(C.byref(c), C.byref(b3)) == 0
+    assert b1.value < b2.value < b3.value
+    c.precision = L.MIXED
+    b4 = C.c_int64()
+    assert lib.ohpic_estimate_footprint(C.byref(c), C.byref(b4)) == 0
+    assert b4.value < b3.value   # 28 B per particle against FP64's 48 B
