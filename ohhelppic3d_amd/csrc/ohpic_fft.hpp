@@ -11,7 +11,10 @@ namespace ohpic {
 
 constexpr int kMaxRadix = 13;   // largest prime factor an axis length may have
 constexpr int kMaxFactors = 24;
-constexpr int kFftBlock = 256;     // threads per FFT workgroup
+#ifndef OHPIC_FFT_BLOCK
+#define OHPIC_FFT_BLOCK 256
+#endif
+constexpr int kFftBlock = OHPIC_FFT_BLOCK;  // threads per FFT workgroup
 // 1024 complex values (16 KiB) per workgroup: ten workgroups fit a CU's LDS
 // and the passes run at 8 waves per SIMD (34-46 VGPRs); measured 28% faster
 // per pass at 256^3 than 32 KiB at 4 waves.  Axis lengths up to 1024.
