@@ -179,9 +179,23 @@ def mixed_error_vs_fp64(args):
     return out
 
 
+def _cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo", "r") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def run_cpu_baseline(seconds: float):
-    """The FP64 oracle (a CPU restatement of the reference step) on the host:
-    32^3 grid, 16 ppc per species (2^20 particles), single thread."""
+    """The FP64 oracle (a CPU restatement of the reference step, gcc -O2
+    -fopenmp) on the host: 32^3 grid, 16 ppc per species (2^20 particles), one
+    MPI-free rank, timed on one thread and on the OpenMP default thread count
+    (per-thread charge arrays for the deposit).  ``value`` is the threaded
+    rate; the one-thread rate is reported beside it."""
     import oracle as orc
 
     n = (32, 32, 32)
@@ -193,18 +207,31 @@ def run_cpu_baseline(seconds: float):
         sp.append(dict(qm=qm, q=(1 if qm > 0 else -1) / ppc, x=x, y=y, z=z, vx=vx, vy=vy,
                        vz=vz))
     sim = orc.OracleSim(n, sp, 0.1, (0, 0, 0.5), 0)
-    sim.step(1)  # warm-up
-    times = []
-    t_end = time.perf_counter() + seconds
-    while time.perf_counter() < t_end or len(times) < 3:
-        t = time.perf_counter()
-        sim.step(1)
-        times.append(time.perf_counter() - t)
     npart = 2 * ppc * 32 ** 3
-    med = float(np.median(times))
-    return {"value": npart / med, "unit": "particle-steps/s", "cores": 1, "kind": "port",
-            "sample": f"FP64 oracle step, 32^3 grid x 16 ppc x 2 species ({npart} particles), "
-                      f"{len(times)} steps, median step {med * 1e3:.1f} ms, 1 thread"}
+
+    def timed(step, budget):
+        step()  # warm-up
+        times = []
+        t_end = time.perf_counter() + budget
+        while time.perf_counter() < t_end or len(times) < 3:
+            t = time.perf_counter()
+            step()
+            times.append(time.perf_counter() - t)
+        return float(np.median(times)), len(times)
+
+    med1, k1 = timed(lambda: sim.step(1), 0.6 * seconds)
+    used = [1]
+
+    def threaded():
+        used[0] = sim.step_threads(1, 0)
+
+    medt, kt = timed(threaded, 0.4 * seconds)
+    return {"value": npart / medt, "unit": "particle-steps/s", "cores": used[0],
+            "kind": "port", "single_thread_value": npart / med1,
+            "cpu_model": _cpu_model(), "ranks": 1, "compiler": "gcc -O2 -fopenmp",
+            "sample": f"FP64 oracle step, 32^3 grid x 16 ppc x 2 species ({npart} particles): "
+                      f"{used[0]} threads, {kt} steps, median {medt * 1e3:.1f} ms; "
+                      f"1 thread, {k1} steps, median {med1 * 1e3:.1f} ms"}
 
 
 def run_ensemble_bench(args):

@@ -99,6 +99,8 @@ def lib():
         L.orc_poisson_periodic.argtypes = [C.c_int, C.c_int, C.c_int, _dp, _dp]
         L.orc_efield.argtypes = [C.c_int, C.c_int, C.c_int, _dp, _dp, _dp, _dp]
         L.orc_step.argtypes = [C.POINTER(OrcSim)]
+        L.orc_step_threads.argtypes = [C.POINTER(OrcSim), C.c_int]
+        L.orc_step_threads.restype = C.c_int
         L.orc_kinetic_energy.argtypes = [C.c_int64, _dp, _dp, _dp, C.c_double]
         L.orc_kinetic_energy.restype = C.c_double
         L.orc_cell_counts.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int64, _dp, _dp, _dp,
@@ -283,6 +285,16 @@ class OracleSim:
     def step(self, nsteps: int = 1):
         for _ in range(nsteps):
             lib().orc_step(C.byref(self.s))
+
+    def step_threads(self, nsteps: int = 1, threads: int = 0) -> int:
+        """orc_step_threads: the same step over OpenMP threads (0: default
+        count); returns the thread count used."""
+        used = 1
+        for _ in range(nsteps):
+            used = lib().orc_step_threads(C.byref(self.s), threads)
+            if used < 0:
+                raise MemoryError("orc_step_threads: no memory for per-thread charge arrays")
+        return used
 
     @property
     def ke_old(self):

@@ -457,26 +457,38 @@ void orc_hc2r_1d(int n, const double *in, double *out)
     free(a);
 }
 
-void orc_r2r_axis(int nx, int ny, int nz, int axis, int kind, double *arr)
+static int64_t orc_line_count(int nx, int ny, int nz, int axis)
+{
+    return axis == 0 ? (int64_t)ny * nz : (axis == 1 ? (int64_t)nx * nz : (int64_t)nx * ny);
+}
+
+/* Transforms the lines [l0, l1) of one axis; a line's first element is
+ * nx*l (x lines), (l % nx) + nx*ny*(l / nx) (y lines) or l (z lines). */
+static void orc_r2r_lines(int nx, int ny, int nz, int axis, int kind, double *arr,
+                          int64_t l0, int64_t l1)
 {
     int n = axis == 0 ? nx : (axis == 1 ? ny : nz);
     int64_t stride = axis == 0 ? 1 : (axis == 1 ? nx : (int64_t)nx * ny);
     double *line = (double *)malloc(sizeof(double) * 2 * (size_t)n);
     double *res = line + n;
-    for (int iz = 0; iz < (axis == 2 ? 1 : nz); ++iz)
-        for (int iy = 0; iy < (axis == 1 ? 1 : ny); ++iy)
-            for (int ix = 0; ix < (axis == 0 ? 1 : nx); ++ix) {
-                int64_t base = ix + (int64_t)nx * (iy + (int64_t)ny * iz);
-                for (int j = 0; j < n; ++j)
-                    line[j] = arr[base + j * stride];
-                if (kind == 0)
-                    orc_r2hc_1d(n, line, res);
-                else
-                    orc_hc2r_1d(n, line, res);
-                for (int j = 0; j < n; ++j)
-                    arr[base + j * stride] = res[j];
-            }
+    for (int64_t l = l0; l < l1; ++l) {
+        int64_t base = axis == 0 ? (int64_t)nx * l
+                     : (axis == 1 ? l % nx + (int64_t)nx * ny * (l / nx) : l);
+        for (int j = 0; j < n; ++j)
+            line[j] = arr[base + j * stride];
+        if (kind == 0)
+            orc_r2hc_1d(n, line, res);
+        else
+            orc_hc2r_1d(n, line, res);
+        for (int j = 0; j < n; ++j)
+            arr[base + j * stride] = res[j];
+    }
     free(line);
+}
+
+void orc_r2r_axis(int nx, int ny, int nz, int axis, int kind, double *arr)
+{
+    orc_r2r_lines(nx, ny, nz, axis, kind, arr, 0, orc_line_count(nx, ny, nz, axis));
 }
 
 double orc_wave_number_sq(int k, int n)
@@ -522,12 +534,12 @@ void orc_poisson_periodic(int nx, int ny, int nz, const double *rho, double *phi
         phi[i] = phi[i] / norm;
 }
 
-void orc_efield(int nx, int ny, int nz, const double *phi, double *ex, double *ey,
-                double *ez)
+static void orc_efield_planes(int nx, int ny, int nz, const double *phi, double *ex,
+                              double *ey, double *ez, int z0, int z1)
 {
     /* poisson_field_solver.f90:139-144 edge difference E_a(i) = phi(i)-phi(i+e_a)
      * and :165-170 node average E_a(i) = 0.5*(E_a(i-e_a) + E_a(i)), periodic */
-    for (int iz = 0; iz < nz; ++iz)
+    for (int iz = z0; iz < z1; ++iz)
         for (int iy = 0; iy < ny; ++iy)
             for (int ix = 0; ix < nx; ++ix) {
                 int xp = (ix + 1) % nx, xm = (ix + nx - 1) % nx;
@@ -549,6 +561,12 @@ void orc_efield(int nx, int ny, int nz, const double *phi, double *ex, double *e
             }
 }
 
+void orc_efield(int nx, int ny, int nz, const double *phi, double *ex, double *ey,
+                double *ez)
+{
+    orc_efield_planes(nx, ny, nz, phi, ex, ey, ez, 0, nz);
+}
+
 /* ======================================================================== */
 /* One timestep                                                              */
 /* ======================================================================== */
@@ -559,6 +577,28 @@ double orc_kinetic_energy(int64_t np, const double *vx, const double *vy, const 
     for (int64_t i = 0; i < np; ++i)
         s += vx[i] * vx[i] + vy[i] * vy[i] + vz[i] * vz[i];
     return 0.5 * mass * s;
+}
+
+static void orc_push_range(orc_sim *s, int sp, int64_t p0, int64_t p1)
+{
+    /* gather -> boris_move -> boundary applied twice (ohhelppic3d.f90:335-358) */
+    const double qm = s->qm[sp];
+    double *x = s->x[sp], *y = s->y[sp], *z = s->z[sp];
+    double *vx = s->vx[sp], *vy = s->vy[sp], *vz = s->vz[sp];
+    for (int64_t p = p0; p < p1; ++p) {
+        double eb[6];
+        orc_gather_periodic(s->nx, s->ny, s->nz, s->ex, s->ey, s->ez, x[p], y[p], z[p],
+                            s->weight_order, eb);
+        eb[3] = s->b0[0];
+        eb[4] = s->b0[1];
+        eb[5] = s->b0[2];
+        orc_boris_move(&x[p], &y[p], &z[p], &vx[p], &vy[p], &vz[p], qm, eb, s->dt);
+        for (int pass = 0; pass < 2; ++pass) {
+            x[p] = orc_apply_periodic(s->nx, x[p]);
+            y[p] = orc_apply_periodic(s->ny, y[p]);
+            z[p] = orc_apply_periodic(s->nz, z[p]);
+        }
+    }
 }
 
 void orc_step(orc_sim *s)
@@ -581,27 +621,136 @@ void orc_step(orc_sim *s)
     s->field_energy = 0.5 * we;
     s->net_charge = qnet;
     for (int sp = 0; sp < s->nspecies; ++sp) {
-        const double qm = s->qm[sp];
-        const double mass = s->q[sp] / qm;
-        double *x = s->x[sp], *y = s->y[sp], *z = s->z[sp];
-        double *vx = s->vx[sp], *vy = s->vy[sp], *vz = s->vz[sp];
-        s->ke_old[sp] = orc_kinetic_energy(s->np[sp], vx, vy, vz, mass);
-        for (int64_t p = 0; p < s->np[sp]; ++p) {
-            double eb[6];
-            orc_gather_periodic(nx, ny, nz, s->ex, s->ey, s->ez, x[p], y[p], z[p],
-                                s->weight_order, eb);
-            eb[3] = s->b0[0];
-            eb[4] = s->b0[1];
-            eb[5] = s->b0[2];
-            orc_boris_move(&x[p], &y[p], &z[p], &vx[p], &vy[p], &vz[p], qm, eb, s->dt);
-            for (int pass = 0; pass < 2; ++pass) {
-                x[p] = orc_apply_periodic(nx, x[p]);
-                y[p] = orc_apply_periodic(ny, y[p]);
-                z[p] = orc_apply_periodic(nz, z[p]);
+        const double mass = s->q[sp] / s->qm[sp];
+        s->ke_old[sp] = orc_kinetic_energy(s->np[sp], s->vx[sp], s->vy[sp], s->vz[sp], mass);
+        orc_push_range(s, sp, 0, s->np[sp]);
+        s->ke_new[sp] = orc_kinetic_energy(s->np[sp], s->vx[sp], s->vy[sp], s->vz[sp], mass);
+    }
+}
+
+/* ---- the same step on several host threads -------------------------------- */
+/* The CPU baseline's multi-threaded form: every particle and every FFT line
+ * goes through the same functions as in orc_step, split over threads; the
+ * deposit goes to one charge array per thread, summed in thread order.  Only
+ * rho (and the sums) can differ from orc_step, by summation order. */
+#ifdef _OPENMP
+#include <omp.h>
+#define ORC_TID() omp_get_thread_num()
+#define ORC_NT() omp_get_num_threads()
+#else
+#define ORC_TID() 0
+#define ORC_NT() 1
+#endif
+
+static void orc_r2r_axis_mt(int nx, int ny, int nz, int axis, int kind, double *arr, int nt)
+{
+    const int64_t nl = orc_line_count(nx, ny, nz, axis);
+#pragma omp parallel num_threads(nt)
+    {
+        const int64_t t = ORC_TID(), T = ORC_NT();
+        orc_r2r_lines(nx, ny, nz, axis, kind, arr, nl * t / T, nl * (t + 1) / T);
+    }
+}
+
+int orc_step_threads(orc_sim *s, int nthreads)
+{
+#ifdef _OPENMP
+    const int nt = nthreads > 0 ? nthreads : omp_get_max_threads();
+#else
+    const int nt = 1;
+    (void)nthreads;
+#endif
+    const int nx = s->nx, ny = s->ny, nz = s->nz;
+    const int64_t ng = (int64_t)nx * ny * nz;
+    double *part = (double *)malloc(sizeof(double) * (size_t)ng * (size_t)nt);
+    if (!part)
+        return -1;
+    int used = 1;
+#pragma omp parallel num_threads(nt)
+    {
+        const int t = ORC_TID(), T = ORC_NT();
+        if (t == 0)
+            used = T;
+        double *mine = part + (size_t)ng * (size_t)t;
+        memset(mine, 0, sizeof(double) * (size_t)ng);
+        for (int sp = 0; sp < s->nspecies; ++sp) {
+            const int64_t n = s->np[sp], p0 = n * t / T, p1 = n * (t + 1) / T;
+            orc_deposit_periodic(nx, ny, nz, p1 - p0, s->x[sp] + p0, s->y[sp] + p0,
+                                 s->z[sp] + p0, s->q[sp], s->weight_order, mine);
+        }
+#pragma omp barrier
+#pragma omp for schedule(static)
+        for (int64_t i = 0; i < ng; ++i) {
+            double r = 0.0;
+            for (int k = 0; k < T; ++k)
+                r += part[(size_t)ng * (size_t)k + (size_t)i];
+            s->rho[i] = r;
+        }
+    }
+    free(part);
+
+    /* field solve: orc_poisson_periodic with its loops split over threads */
+    double *phi = s->phi;
+    memcpy(phi, s->rho, sizeof(double) * (size_t)ng);
+    for (int a = 0; a < 3; ++a)
+        orc_r2r_axis_mt(nx, ny, nz, a, 0, phi, used);
+#pragma omp parallel for schedule(static) num_threads(used)
+    for (int kz = 0; kz < nz; ++kz) {
+        double wz = orc_wave_number_sq(kz, nz);
+        for (int ky = 0; ky < ny; ++ky) {
+            double wy = orc_wave_number_sq(ky, ny);
+            for (int kx = 0; kx < nx; ++kx) {
+                double wx = orc_wave_number_sq(kx, nx);
+                int64_t idx = kx + (int64_t)nx * (ky + (int64_t)ny * kz);
+                if (kx == 0 && ky == 0 && kz == 0)
+                    phi[idx] = 0.0;
+                else
+                    phi[idx] = phi[idx] / (wx + wy + wz);
             }
         }
-        s->ke_new[sp] = orc_kinetic_energy(s->np[sp], vx, vy, vz, mass);
     }
+    for (int a = 0; a < 3; ++a)
+        orc_r2r_axis_mt(nx, ny, nz, a, 1, phi, used);
+    const double norm = (double)nx * (double)ny * (double)nz;
+    double we = 0.0, qnet = 0.0;
+#pragma omp parallel num_threads(used)
+    {
+#pragma omp for schedule(static)
+        for (int64_t i = 0; i < ng; ++i)
+            phi[i] = phi[i] / norm;
+        const int t = ORC_TID(), T = ORC_NT();
+        orc_efield_planes(nx, ny, nz, phi, s->ex, s->ey, s->ez, (int)((int64_t)nz * t / T),
+                          (int)((int64_t)nz * (t + 1) / T));
+#pragma omp for schedule(static) reduction(+ : we, qnet)
+        for (int64_t i = 0; i < ng; ++i) {
+            we += s->rho[i] * phi[i];
+            qnet += s->rho[i];
+        }
+    }
+    s->field_energy = 0.5 * we;
+    s->net_charge = qnet;
+
+    for (int sp = 0; sp < s->nspecies; ++sp) {
+        const double mass = s->q[sp] / s->qm[sp];
+        const int64_t n = s->np[sp];
+        const double *vx = s->vx[sp], *vy = s->vy[sp], *vz = s->vz[sp];
+        double ko = 0.0, kn = 0.0;
+#pragma omp parallel num_threads(used)
+        {
+#pragma omp for schedule(static) reduction(+ : ko)
+            for (int64_t p = 0; p < n; ++p)
+                ko += vx[p] * vx[p] + vy[p] * vy[p] + vz[p] * vz[p];
+            const int64_t t = ORC_TID(), T = ORC_NT();
+            orc_push_range(s, sp, n * t / T, n * (t + 1) / T);
+#pragma omp barrier
+#pragma omp for schedule(static) reduction(+ : kn)
+            for (int64_t p = 0; p < n; ++p)
+                kn += vx[p] * vx[p] + vy[p] * vy[p] + vz[p] * vz[p];
+        }
+        s->ke_old[sp] = 0.5 * mass * ko;
+        s->ke_new[sp] = 0.5 * mass * kn;
+    }
+    return used;
 }
 
 void orc_cell_counts(int nx, int ny, int nz, int64_t np, const double *x, const double *y,

@@ -110,6 +110,12 @@ typedef struct {
 
 void orc_step(orc_sim *s);
 
+/* orc_step with its particle and FFT-line loops split over nthreads OpenMP
+ * threads (0: the OpenMP default); the deposit sums one charge array per
+ * thread in thread order.  Returns the thread count used, -1 if out of
+ * memory.  One thread when built without OpenMP. */
+int orc_step_threads(orc_sim *s, int nthreads);
+
 /* Energy helpers. */
 double orc_kinetic_energy(int64_t np, const double *vx, const double *vy, const double *vz,
                           double mass);

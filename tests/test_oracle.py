@@ -390,3 +390,35 @@ def test_cell_counts(orc):
     c = orc.cell_counts(n, x, y, z)
     assert c.sum() == 4
     assert c[0] == 1 and c[3 + 4 * (2 + 3 * 1)] == 1 and c[1 + 4 * (0 + 3 * 1)] == 2
+
+
+def _state(sim):
+    return ([sim.rho, sim.phi, sim.ex, sim.ey, sim.ez] +
+            [s[k] for s in sim.sp for k in ("x", "y", "z", "vx", "vy", "vz")])
+
+
+def test_threaded_step_on_one_thread_is_the_serial_step(orc):
+    # the threaded form runs the same per-particle and per-line functions; on
+    # one thread even its sums come out in the serial order
+    n = (12, 8, 10)
+    a = orc.OracleSim(n, _two_species(orc, n, 5), 0.1, (0, 0, 0.5), 0)
+    b = orc.OracleSim(n, _two_species(orc, n, 5), 0.1, (0, 0, 0.5), 0)
+    a.step(3)
+    assert b.step_threads(3, 1) == 1
+    for u, v in zip(_state(a), _state(b)):
+        assert np.array_equal(u, v)
+    assert np.array_equal(a.ke_new, b.ke_new) and a.field_energy == b.field_energy
+
+
+@pytest.mark.parametrize("threads", [2, 3, 8])
+def test_threaded_step_matches_serial(orc, threads):
+    # several threads: only the order of the charge sums changes
+    n = (16, 12, 8)
+    a = orc.OracleSim(n, _two_species(orc, n, 7), 0.1, (0, 0, 0.5), 0)
+    b = orc.OracleSim(n, _two_species(orc, n, 7), 0.1, (0, 0, 0.5), 0)
+    a.step(3)
+    assert b.step_threads(3, threads) == threads
+    for u, v in zip(_state(a), _state(b)):
+        assert np.linalg.norm(u - v) <= 1e-12 * np.linalg.norm(u)
+    assert np.allclose(a.ke_new, b.ke_new, rtol=1e-12, atol=0)
+    assert abs(a.field_energy - b.field_energy) <= 1e-12 * abs(a.field_energy)
