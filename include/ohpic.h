@@ -38,6 +38,8 @@ extern "C" {
 #define OHPIC_ENOMEM -3   /* device memory / bin capacity exhausted */
 #define OHPIC_ESTATE -4   /* call not valid in the current state */
 #define OHPIC_ENODEV -6   /* no usable GPU */
+#define OHPIC_EIO -7      /* file or HDF5 error */
+#define OHPIC_ENOLIB -8   /* a library loaded at run time (HDF5) is missing */
 
 /* precision modes */
 #define OHPIC_FP64 0      /* FP64 positions/velocities and arithmetic */
@@ -247,6 +249,41 @@ typedef struct {
 } ohpic_tune_result;
 int ohpic_autotune(const ohpic_config *cfgs, int n, int sample, int steps,
                    ohpic_tune_result *out);
+
+/* --- snapshots (t_Hdf5ForOhfield, src/hdf5/hdf5_for_ohfield.f90:11-89) -------
+ * One HDF5 file holding one group, with one 3-D FP64 dataset per write, as
+ * the reference's phi output 'phisp00_0000.h5' / group 'phisp' / datasets '1',
+ * '2', ... (ohhelppic3d.f90:79-96).  Shapes and offsets are in Fortran order
+ * (x first) as the reference passes them to the Fortran HDF5 API; through the
+ * C API the dataset dimensions read (z, y, x) with x fastest.  HDF5 >= 1.10 is
+ * loaded at run time (OHPIC_HDF5_LIB, else libhdf5.so.103 / libhdf5.so, then
+ * /opt/conda/lib): without it these return OHPIC_ENOLIB.  Calls are serialised
+ * process-wide. */
+typedef struct ohpic_snapshot ohpic_snapshot;
+/* new_Hdf5ForOhfield(filename, group_name, global_shape, global_offset, comm)
+ * (hdf5_for_ohfield.f90:27-43; new_HDF5File 'w' + create_group, hdf5.f90:46-89):
+ * creates (truncates) the file and the group.  global_offset may be NULL. */
+int ohpic_snapshot_open(const char *filename, const char *group, const int64_t global_shape[3],
+                        const int64_t global_offset[3], ohpic_snapshot **out);
+/* hdf5ForOhfield_write(dataset_name, ohfield, ps) (hdf5_for_ohfield.f90:45-76):
+ * field `which` (RHO, PHI, EX, EY or EZ) of ctx as the block of nodes 0..n per
+ * axis at offset 0, node n holding node 0 (periodic); the global shape must be
+ * at least (nx+1, ny+1, nz+1). */
+int ohpic_snapshot_write(ohpic_snapshot *s, const char *dataset, ohpic_ctx *ctx, int which);
+/* write_dataset3d_double (hdf5.f90:140-185): a host block values(local_shape)
+ * in Fortran order written at local_offset (NULL: 0) of a new dataset of the
+ * global shape. */
+int ohpic_snapshot_write_host(ohpic_snapshot *s, const char *dataset, const double *values,
+                              const int64_t local_shape[3], const int64_t local_offset[3]);
+/* hdf5ForOhfield_close (hdf5_for_ohfield.f90:78-86); s may be NULL. */
+int ohpic_snapshot_close(ohpic_snapshot *s);
+const char *ohpic_snapshot_last_error(const ohpic_snapshot *s);
+/* Reads group/dataset of a snapshot file: shape[3] (Fortran order) always,
+ * and the values (x fastest) when values is not NULL and capacity (in
+ * doubles) holds them.  No counterpart in the reference's output path (its
+ * reader, hdf5.f90:91-138, reads 1-D integer datasets). */
+int ohpic_snapshot_read(const char *filename, const char *group, const char *dataset,
+                        double *values, int64_t capacity, int64_t shape[3]);
 
 #ifdef __cplusplus
 }

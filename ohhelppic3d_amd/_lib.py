@@ -16,7 +16,7 @@ LIB_PATH = os.environ.get("OHPIC_LIB", os.path.join(PKG, "libohpic.so"))
 
 MAX_SPECIES = 8
 
-OK, EINVAL, EHIP, ENOMEM, ESTATE, ENODEV = 0, -1, -2, -3, -4, -6
+OK, EINVAL, EHIP, ENOMEM, ESTATE, ENODEV, EIO, ENOLIB = 0, -1, -2, -3, -4, -6, -7, -8
 FP64, MIXED = 0, 1
 WEIGHTS_REFERENCE, WEIGHTS_STANDARD = 0, 1
 FIELD_RHO, FIELD_PHI, FIELD_EX, FIELD_EY, FIELD_EZ, FIELD_EB = range(6)
@@ -36,6 +36,8 @@ EXPORTS = (
     "ohpic_group_size", "ohpic_group_contains", "ohpic_group_step", "ohpic_group_use_graph",
     "ohpic_group_use_persistent", "ohpic_group_stats",
     "ohpic_group_last_error", "ohpic_autotune",
+    "ohpic_snapshot_open", "ohpic_snapshot_write", "ohpic_snapshot_write_host",
+    "ohpic_snapshot_close", "ohpic_snapshot_last_error", "ohpic_snapshot_read",
 )
 
 
@@ -163,10 +165,15 @@ def lib():
         "ohpic_group_use_persistent": [vp, C.c_int],
         "ohpic_group_stats": [vp, i64p, i64p, i64p],
         "ohpic_autotune": [C.POINTER(Config), C.c_int, C.c_int, C.c_int, C.POINTER(TuneResult)],
+        "ohpic_snapshot_open": [C.c_char_p, C.c_char_p, i64p, i64p, C.POINTER(vp)],
+        "ohpic_snapshot_write": [vp, C.c_char_p, vp, C.c_int],
+        "ohpic_snapshot_write_host": [vp, C.c_char_p, dp, i64p, i64p],
+        "ohpic_snapshot_close": [vp],
+        "ohpic_snapshot_read": [C.c_char_p, C.c_char_p, C.c_char_p, dp, i64, i64p],
     }
     for name in EXPORTS:
         f = getattr(L, name)
-        if name in ("ohpic_last_error", "ohpic_group_last_error"):
+        if name in ("ohpic_last_error", "ohpic_group_last_error", "ohpic_snapshot_last_error"):
             f.argtypes = [vp]
             f.restype = C.c_char_p
             continue
@@ -185,6 +192,12 @@ def check(rc: int, ctx=None) -> None:
 def check_group(rc: int, group) -> None:
     if rc != OK:
         msg = lib().ohpic_group_last_error(group)
+        raise OhpicError(rc, msg.decode() if msg else "")
+
+
+def check_snapshot(rc: int, snap) -> None:
+    if rc != OK:
+        msg = lib().ohpic_snapshot_last_error(snap)
         raise OhpicError(rc, msg.decode() if msg else "")
 
 

@@ -17,7 +17,7 @@ OBJ = os.path.join(PKG, "_build")
 LIB = os.path.join(PKG, "libohpic.so")
 
 SOURCES = ["ohpic_particles.hip", "ohpic_particles_mixed.hip", "ohpic_fields.hip",
-           "ohpic_fft.hip", "ohpic_ops.hip", "ohpic_api.cpp"]
+           "ohpic_fft.hip", "ohpic_ops.hip", "ohpic_api.cpp", "ohpic_snapshot.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("OHPIC_ARCH", "gfx950")
@@ -46,7 +46,7 @@ def build(verbose: bool = False) -> str:
     with ThreadPoolExecutor(max_workers=min(6, len(SOURCES))) as ex:
         objs = list(ex.map(_compile, SOURCES))
     tmp = LIB + ".tmp"
-    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs]
+    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs, "-ldl"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr[-6000:]}")

@@ -14,6 +14,7 @@
 !>   t_ParticleBoundaries   (particle_boundaries.f90)    t_ParticleBoundariesHip
 !>   t_PoissonFieldSolver   (poisson_field_solver.f90)   t_PoissonFieldSolverHip
 !>   t_MPIFFTW3Solver3d     (mpi_fftw3_solver.f90)       t_FFTSolverHip
+!>   t_Hdf5ForOhfield       (hdf5_for_ohfield.f90)       t_Hdf5ForOhfieldHip
 !>
 !> Non-zero statuses end the program with `error stop` and the library's
 !> message, as the reference's own failures do.
@@ -37,6 +38,7 @@ module m_ohpic
     public :: t_InterpolatorHip, t_ScatterHip
     public :: t_ParticleBoundariesHip, new_ParticleBoundariesHip
     public :: t_PoissonFieldSolverHip, t_FFTSolverHip, new_FFTSolverHip
+    public :: t_Hdf5ForOhfieldHip, new_Hdf5ForOhfieldHip
 
     !> One device-resident simulation (an ohpic_ctx).
     type :: t_OhpicSimulation
@@ -98,6 +100,20 @@ module m_ohpic
     contains
         procedure :: forward => hip_fft_forward
         procedure :: backward => hip_fft_backward
+    end type
+
+    !> t_Hdf5ForOhfield (hdf5_for_ohfield.f90:11-22) over ohpic_snapshot_*: one
+    !> HDF5 file and group, one dataset of global_shape per write.
+    type :: t_Hdf5ForOhfieldHip
+        type(c_ptr) :: snap = c_null_ptr
+        character(:), allocatable :: filename
+        character(:), allocatable :: group_name
+        integer(kind=8) :: global_shape(3) = 0
+        integer(kind=8) :: global_offset(3) = 0
+    contains
+        procedure :: write => h5_write
+        procedure :: write_device => h5_write_device
+        procedure :: close => h5_close
     end type
 
 contains
@@ -462,4 +478,66 @@ contains
                                          int(self%local_block%sizes(2), c_int), &
                                          int(self%local_block%sizes(3), c_int), 1_c_int, in, out))
     end subroutine
+
+    ! ----------------------------------------------------------------- snapshots
+    !> new_Hdf5ForOhfield (hdf5_for_ohfield.f90:27-43); comm is accepted for
+    !> the same call shape and unused (one rank writes the whole box).
+    function new_Hdf5ForOhfieldHip(filename, group_name, global_shape, global_offset, comm) &
+        result(obj)
+        character(*), intent(in) :: filename
+        character(*), intent(in) :: group_name
+        integer(kind=8), intent(in) :: global_shape(3)
+        integer(kind=8), intent(in) :: global_offset(3)
+        integer, intent(in), optional :: comm
+        type(t_Hdf5ForOhfieldHip) :: obj
+        obj%filename = filename
+        obj%group_name = group_name
+        obj%global_shape = global_shape
+        obj%global_offset = global_offset
+        call ohpic_check(ohpic_snapshot_open(filename//c_null_char, group_name//c_null_char, &
+                                             int(global_shape, c_int64_t), &
+                                             int(global_offset, c_int64_t), obj%snap))
+    end function
+
+    !> hdf5ForOhfield_write (hdf5_for_ohfield.f90:45-76): the host block
+    !> values(1, 0:xu-xl, 0:yu-yl, 0:zu-zl, ps) at offset (xl, yl, zl).
+    subroutine h5_write(self, dataset_name, ohfield, ps)
+        class(t_Hdf5ForOhfieldHip), intent(in) :: self
+        character(*), intent(in) :: dataset_name
+        class(t_OhField), intent(in) :: ohfield
+        integer, intent(in) :: ps
+        integer :: xl, xu, yl, yu, zl, zu
+        integer(c_int64_t) :: shp(3), off(3)
+        double precision, allocatable :: block(:, :, :)
+        xl = ohfield%subdomain_range(1, 1, ps)
+        xu = ohfield%subdomain_range(2, 1, ps)
+        yl = ohfield%subdomain_range(1, 2, ps)
+        yu = ohfield%subdomain_range(2, 2, ps)
+        zl = ohfield%subdomain_range(1, 3, ps)
+        zu = ohfield%subdomain_range(2, 3, ps)
+        shp = [xu - xl + 1, yu - yl + 1, zu - zl + 1]
+        off = [xl, yl, zl]
+        block = ohfield%values(1, 0:xu - xl, 0:yu - yl, 0:zu - zl, ps)
+        call ohpic_check(ohpic_snapshot_write_host(self%snap, dataset_name//c_null_char, block, &
+                                                   shp, off))
+    end subroutine
+
+    !> The same dataset written from the device field `which` of a simulation
+    !> (nodes 0..n per axis, node n = node 0), without a host t_OhField.
+    subroutine h5_write_device(self, dataset_name, sim, which)
+        class(t_Hdf5ForOhfieldHip), intent(in) :: self
+        character(*), intent(in) :: dataset_name
+        class(t_OhpicSimulation), intent(in) :: sim
+        integer, intent(in) :: which
+        call ohpic_check(ohpic_snapshot_write(self%snap, dataset_name//c_null_char, sim%ctx, &
+                                              int(which, c_int)))
+    end subroutine
+
+    !> hdf5ForOhfield_close (hdf5_for_ohfield.f90:78-86).
+    subroutine h5_close(self)
+        class(t_Hdf5ForOhfieldHip), intent(inout) :: self
+        call ohpic_check(ohpic_snapshot_close(self%snap))
+        self%snap = c_null_ptr
+    end subroutine
+
 end module

@@ -12,6 +12,8 @@ module m_ohpic_c
     public :: ohpic_get_field, ohpic_put_field, ohpic_fft3d_r2r, ohpic_diagnostics
     public :: ohpic_cell_counts, ohpic_boris_batch, ohpic_interp_batch
     public :: ohpic_scatter_batch, ohpic_wrap_batch
+    public :: ohpic_snapshot_open, ohpic_snapshot_write, ohpic_snapshot_write_host
+    public :: ohpic_snapshot_close
 
     integer, parameter, public :: OHPIC_MAX_SPECIES = 8
     integer, parameter, public :: OHPIC_FP64 = 0, OHPIC_MIXED = 1
@@ -218,6 +220,38 @@ module m_ohpic_c
             real(c_double), intent(inout) :: p(*)
             integer(c_int32_t), value :: nn
             integer(c_int), value :: applications
+        end function
+
+        integer(c_int) function ohpic_snapshot_open(filename, group, global_shape, &
+                                                    global_offset, out) &
+            bind(c, name='ohpic_snapshot_open')
+            import :: c_int, c_char, c_int64_t, c_ptr
+            character(kind=c_char), intent(in) :: filename(*), group(*)
+            integer(c_int64_t), intent(in) :: global_shape(3), global_offset(3)
+            type(c_ptr), intent(out) :: out
+        end function
+
+        integer(c_int) function ohpic_snapshot_write(snap, dataset, ctx, which) &
+            bind(c, name='ohpic_snapshot_write')
+            import :: c_int, c_char, c_ptr
+            type(c_ptr), value :: snap, ctx
+            character(kind=c_char), intent(in) :: dataset(*)
+            integer(c_int), value :: which
+        end function
+
+        integer(c_int) function ohpic_snapshot_write_host(snap, dataset, values, local_shape, &
+                                                          local_offset) &
+            bind(c, name='ohpic_snapshot_write_host')
+            import :: c_int, c_char, c_ptr, c_double, c_int64_t
+            type(c_ptr), value :: snap
+            character(kind=c_char), intent(in) :: dataset(*)
+            real(c_double), intent(in) :: values(*)
+            integer(c_int64_t), intent(in) :: local_shape(3), local_offset(3)
+        end function
+
+        integer(c_int) function ohpic_snapshot_close(snap) bind(c, name='ohpic_snapshot_close')
+            import :: c_int, c_ptr
+            type(c_ptr), value :: snap
         end function
     end interface
 end module

@@ -1,6 +1,6 @@
 !> Drives the GPU implementation through the reference's Fortran interfaces.
 !>
-!> usage: ohpic_fortran_check <input.bin> <output.bin>
+!> usage: ohpic_fortran_check <input.bin> <output.bin> [<snapshot.h5>]
 !> input (stream): int32 n(3), nspecies, nsteps, kcheck; int64 np(nspecies);
 !>                 float64 qm(nspecies), q(nspecies), dt, b0(3); oh_particle records
 !> output (stream): see the write statements below, in order.
@@ -10,7 +10,9 @@
 !> objects on host t_OhField arrays; part 2 calls the per-particle bindings
 !> (interp, move, apply twice, scatter) on the first kcheck input particles;
 !> part 3 transforms the reference's FFT impulse (1 x 1 x 63); part 4 runs the
-!> fused step on a second simulation from the same input.
+!> fused step on a second simulation from the same input; part 5 (with a third
+!> argument) writes phi of part 1 (host t_OhField) and of part 4 (device) as
+!> datasets '1' and '2' of group 'phisp' (ohhelppic3d.f90:79-99).
 program ohpic_fortran_check
     use iso_c_binding
     use oh_type, only: oh_particle
@@ -23,9 +25,11 @@ program ohpic_fortran_check
     use m_field_solver, only: t_FieldSolver
     use m_mpi_fft_solver, only: t_MPIFFTSolver3d
     use m_ohpic
+    use m_ohpic_c, only: OHPIC_FIELD_PHI
     implicit none
 
-    character(len=1024) :: inpath, outpath
+    character(len=1024) :: inpath, outpath, h5path
+    type(t_Hdf5ForOhfieldHip) :: h5
     integer(c_int32_t) :: hdr(6)
     integer :: n(3), nsp, nsteps, kcheck, u, i, s, istep
     integer(c_int64_t), allocatable :: np(:)
@@ -47,6 +51,8 @@ program ohpic_fortran_check
 
     call get_command_argument(1, inpath)
     call get_command_argument(2, outpath)
+    h5path = ''
+    if (command_argument_count() >= 3) call get_command_argument(3, h5path)
     open (newunit=u, file=trim(inpath), access='stream', form='unformatted', status='old')
     read (u) hdr
     n = hdr(1:3)
@@ -119,6 +125,15 @@ program ohpic_fortran_check
     call sim2%init(n, qm, q, np, dt, b0, track_ids=.true.)
     call sim2%load(parts)
     call sim2%step(nsteps)
+
+    ! ---- part 5: snapshots in the reference's phisp layout --------------------
+    if (len_trim(h5path) > 0) then
+        h5 = new_Hdf5ForOhfieldHip(trim(h5path), 'phisp', int(n + 1, kind=8), &
+                                   int([0, 0, 0], kind=8))
+        call h5%write('1', phi, 1)
+        call h5%write_device('2', sim2, OHPIC_FIELD_PHI)
+        call h5%close()
+    end if
 
     open (newunit=u, file=trim(outpath), access='stream', form='unformatted', status='replace')
     write (u) nout

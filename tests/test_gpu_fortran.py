@@ -13,6 +13,13 @@ from helpers import rel_l2, two_species
 pytestmark = pytest.mark.gpu
 
 
+def _have_hdf5():
+    from ohhelppic3d_amd import _lib
+    probe = np.zeros(3, np.int64)
+    return _lib.lib().ohpic_snapshot_read(b"/nonexistent.h5", b"g", b"d", None, 0,
+                                          _lib.i64ptr(probe)) != _lib.ENOLIB
+
+
 def test_fortran_mainstep_and_bindings_match_oracle(orc, tmp_path):
     if not os.path.exists(fio.EXE):
         pytest.fail("ohpic_fortran_check not built (make -C ohhelppic3d_amd/fortran)")
@@ -21,8 +28,9 @@ def test_fortran_mainstep_and_bindings_match_oracle(orc, tmp_path):
     dt, b0 = 0.1, (0.0, 0.0, 0.5)
     sp = two_species(orc, n, 4)
     recs = fio.write_input(str(tmp_path / "in.bin"), n, sp, dt, b0, nsteps, k)
-    r = subprocess.run([fio.EXE, str(tmp_path / "in.bin"), str(tmp_path / "out.bin")],
-                       capture_output=True, text=True, timeout=300)
+    h5 = str(tmp_path / "phisp00_0000.h5") if _have_hdf5() else None
+    r = subprocess.run([fio.EXE, str(tmp_path / "in.bin"), str(tmp_path / "out.bin")] +
+                       ([h5] if h5 else []), capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, r.stdout + r.stderr
     out = fio.read_output(str(tmp_path / "out.bin"), n, k)
 
@@ -83,3 +91,11 @@ def test_fortran_mainstep_and_bindings_match_oracle(orc, tmp_path):
     kk = np.arange(63)
     exp = np.where(kk <= 31, np.cos(kk / 63 * 2 * np.pi), np.sin(kk / 63 * 2 * np.pi))
     assert np.sum(np.abs(out["spectrum"] - exp)) <= 1e-8
+
+    # part 5: t_Hdf5ForOhfieldHip%write of the host phi (nodes 0..n) and
+    # %write_device of the fused run's phi, in the reference's phisp layout
+    if h5:
+        from ohhelppic3d_amd import snapshot
+        want = np.transpose(out["phi"][1:n[2] + 2, 1:n[1] + 2, 1:n[0] + 2], (2, 1, 0))
+        assert np.array_equal(snapshot.read_dataset(h5, "phisp", "1"), want)
+        assert rel_l2(snapshot.read_dataset(h5, "phisp", "2"), want) <= 1e-12
