@@ -20,6 +20,7 @@
 #include <type_traits>
 
 #include "ohpic_fft.hpp"
+#include "ohpic_internal.hpp"
 
 namespace ohpic {
 
@@ -338,8 +339,11 @@ __global__ void __launch_bounds__(kFftBlock)
     const FftAxisDev &ax = J.ax[AXIS];
     const int n = ax.n;
     const int G = fft_group_lines(n);
-    const int grp = blockIdx.x;
-    if ((int64_t)grp >= fft_groups(AXIS, G, nx, ny, nz)) return;
+    const int64_t ngroups = fft_groups(AXIS, G, nx, ny, nz);
+    if ((int64_t)blockIdx.x >= ngroups) return;
+    // neighbouring line groups read the two halves of the same 128-byte
+    // lines (y and z passes): keep them on one XCD
+    const int grp = xcd_remap((int)blockIdx.x, (int)ngroups);
     const double *__restrict__ src = FROM_SRC ? J.src : J.data;
     double *__restrict__ dst = J.data;
     const double out_scale = SCALE ? J.inv_n : 1.0;
@@ -365,36 +369,50 @@ __global__ void __launch_bounds__(kFftBlock)
     if (KIND == kFftR2HC || KIND == kFftPoisson) {
         fft_lds<-1, RS>(buf, n, nlines, ax);
         if (KIND == kFftPoisson) {
-            // halfcomplex pairs divided by the Green's-function denominator
+            // Green's-function division on the packed spectra: with
+            // Z = FFT(a + i b), A_k = (Z_k + conj Z_{n-k}) / 2 and
+            // B_k = (Z_k - conj Z_{n-k}) / 2i; A and B are divided by their
+            // lines' denominators (DC := 0) and repacked as A' + i B', whose
+            // inverse transform is (n a', n b') as HC2R of the halfcomplex
+            // quotients gives.  One task per (pair, k <= n/2) owns slots k and
+            // n - k, so the update needs no barrier of its own.
             const double *__restrict__ lamz = J.lam[2];
-            const int total = n * nlines;
-            cplx h[kElemMax];
+            const int half = n / 2 + 1;
+            const int total = half * nlines;
+            constexpr int kTasks = (kFftLdsCplx / 2 + kFftLdsCplx / 16 + kFftBlock - 1) / kFftBlock + 1;
+            const uint32_t mg_half = ax.mg_half;
 #pragma unroll
-            for (int k = 0; k < kElemMax; ++k) {
-                const int t = threadIdx.x + k * kFftBlock;
-                if (t < total) {
-                    const int c = fastdiv(t, ax.mg_n);
-                    const int m = t - c * n;
-                    double va = hc_value(buf, n, c, 0, m);
-                    double vb = hc_value(buf, n, c, 1, m);
-                    const double la = sLamXY[2 * c];
-                    const double lbv = sLamXY[2 * c + 1];
-                    const double lz = lamz[m];
-                    if (2 * c < lb.nvalid)
-                        va = (la < 0.0 && m == 0) ? 0.0 : va / ((la < 0.0 ? 0.0 : la) + lz);
-                    if (2 * c + 1 < lb.nvalid)
-                        vb = (lbv < 0.0 && m == 0) ? 0.0 : vb / ((lbv < 0.0 ? 0.0 : lbv) + lz);
-                    h[k] = {va, vb};
+            for (int q = 0; q < kTasks; ++q) {
+                const int t = threadIdx.x + q * kFftBlock;
+                if (t >= total) break;
+                const int c = fastdiv(t, mg_half);
+                const int k = t - c * half;
+                const int km = k == 0 ? 0 : n - k;
+                const cplx zk = buf[c * n + k], zm = buf[c * n + km];
+                cplx A = {0.5 * (zk.re + zm.re), 0.5 * (zk.im - zm.im)};
+                cplx B = {0.5 * (zk.im + zm.im), -0.5 * (zk.re - zm.re)};
+                const double la = sLamXY[2 * c], lbv = sLamXY[2 * c + 1];
+                const double lz = lamz[k];
+                if (2 * c < lb.nvalid) {
+                    if (la < 0.0 && k == 0) {
+                        A = {0.0, 0.0};
+                    } else {
+                        const double d = (la < 0.0 ? 0.0 : la) + lz;
+                        A = {A.re / d, A.im / d};
+                    }
                 }
+                if (2 * c + 1 < lb.nvalid) {
+                    if (lbv < 0.0 && k == 0) {
+                        B = {0.0, 0.0};
+                    } else {
+                        const double d = (lbv < 0.0 ? 0.0 : lbv) + lz;
+                        B = {B.re / d, B.im / d};
+                    }
+                }
+                buf[c * n + k] = cplx{A.re - B.im, A.im + B.re};
+                if (km != k) buf[c * n + km] = cplx{A.re + B.im, B.re - A.im};
             }
             __syncthreads();
-#pragma unroll
-            for (int k = 0; k < kElemMax; ++k) {
-                const int t = threadIdx.x + k * kFftBlock;
-                if (t < total) buf[t] = h[k];
-            }
-            __syncthreads();
-            build_packed_inplace(buf, n, nlines, ax.mg_n);
             fft_lds<+1, RS>(buf, n, nlines, ax);
         }
     } else {
@@ -427,6 +445,7 @@ void fft_plan_divisors(FftAxisDev &ax)
 {
     ax.mg_n = fastdiv_magic((uint32_t)ax.n);
     ax.mg_G = fastdiv_magic((uint32_t)fft_group_lines(ax.n));
+    ax.mg_half = fastdiv_magic((uint32_t)(ax.n / 2 + 1));
     int Ls = 1;
     for (int s = 0; s < kMaxFactors; ++s) {
         const int p = s < ax.nfac ? ax.fac[s] : 1;
@@ -451,7 +470,7 @@ static hipError_t launch_pass(const FftJob *jobs, int njobs, int64_t max_groups,
                               hipStream_t st)
 {
     if (njobs <= 0 || max_groups <= 0) return hipSuccess;
-    const dim3 g((unsigned)max_groups, (unsigned)njobs), b(kFftBlock);
+    const dim3 g((unsigned)round_up_xcds(max_groups), (unsigned)njobs), b(kFftBlock);
     if (rs == 0)
         hipLaunchKernelGGL((k_fft_pass<KIND, AXIS, 0, FROM_SRC, SCALE>), g, b, 0, st, jobs);
     else if (rs == 1)

@@ -53,8 +53,19 @@ __global__ void __launch_bounds__(256) k_efield_b(const InstDev *__restrict__ in
     const int tx0 = rpb > 1 ? (int)threadIdx.x - tsub * nx : (int)threadIdx.x;
     const int xstep = rpb > 1 ? nx : (int)blockDim.x;
     const int64_t sy = nx, sz = (int64_t)nx * ny;
+    // XCD-aware rows: with a block count divisible by 8, XCD k (blocks
+    // k, k+8, ...) walks the contiguous rows [k, k+1) * rows / 8 in order, so
+    // the y and z neighbour rows a block reads were read moments before by
+    // blocks of the same XCD and come from its L2
+    int rbeg = (int)blockIdx.x * rpb, rstep = nblk * rpb, rend = rows;
+    if (nblk >= kXcds && nblk % kXcds == 0) {
+        const int k = (int)blockIdx.x % kXcds, t = (int)blockIdx.x / kXcds;
+        rbeg = (int)((int64_t)rows * k / kXcds) + t * rpb;
+        rstep = nblk / kXcds * rpb;
+        rend = (int)((int64_t)rows * (k + 1) / kXcds);
+    }
     double we = 0.0, we2 = 0.0, qn = 0.0;
-    for (int r = blockIdx.x * rpb + tsub; tsub < rpb && r < rows; r += nblk * rpb) {
+    for (int r = rbeg + tsub; tsub < rpb && r < rend; r += rstep) {
         const int iy = r % ny;
         const int iz = r / ny;
         const int64_t row = (int64_t)r * nx;
@@ -307,7 +318,7 @@ hipError_t launch_rho_from_acc_b(const InstDev *insts, const int *parity, int ni
 hipError_t launch_efield_b(const InstDev *insts, int ninst, int max_blocks, hipStream_t st)
 {
     if (ninst <= 0) return hipSuccess;
-    const int blocks = max_blocks > 0 ? max_blocks : 1;
+    const int blocks = (int)round_up_xcds(max_blocks > 0 ? max_blocks : 1);
     hipLaunchKernelGGL(k_efield_b, dim3(blocks, ninst), dim3(256), 0, st, insts);
     hipLaunchKernelGGL(k_reduce_e_b, dim3(ninst), dim3(256), 0, st, insts);
     return hipGetLastError();

@@ -89,7 +89,10 @@ struct WorkItem {
 };
 
 constexpr int kKeSlots = 2 * kMaxSpecies;  // per item: ke_old[8], ke_new[8]
-constexpr int kEfieldMaxBlocks = 4096;     // rows of e_partial
+#ifndef OHPIC_EFIELD_MAX_BLOCKS
+#define OHPIC_EFIELD_MAX_BLOCKS 4096
+#endif
+constexpr int kEfieldMaxBlocks = OHPIC_EFIELD_MAX_BLOCKS;  // rows of e_partial
 
 // k_efield_b: x rows shorter than 256 nodes share a 256-thread block
 // (256 / nx of them); the instance's block count depends on its grid only,
@@ -127,6 +130,22 @@ __device__ __forceinline__ int bin_slot(int v, int F, int B, int cap)
 {
     return v < F ? v : cap - B + (v - F);
 }
+
+// Workgroups are dispatched to the 8 XCDs round-robin by linear workgroup id,
+// and each XCD has its own L2.  xcd_remap(b, count) turns dispatch index b
+// (< count) into a logical index such that XCD k receives the contiguous
+// logical range [k*per, (k+1)*per) in order (per = count / 8; the count % 8
+// last ids map to themselves), so that neighbouring logical workgroups share
+// an L2.  A bijection on [0, count); grids round their x size up to a multiple
+// of 8 so that blockIdx.x % 8 is the XCD for every blockIdx.y.
+constexpr int kXcds = 8;
+__host__ __device__ inline int xcd_remap(int b, int count)
+{
+    const int full = count - count % kXcds;
+    if (b >= full) return b;
+    return (b % kXcds) * (full / kXcds) + b / kXcds;
+}
+__host__ __device__ inline int64_t round_up_xcds(int64_t n) { return (n + kXcds - 1) / kXcds * kXcds; }
 
 __device__ __forceinline__ int wrap_index(int i, int n)
 {
