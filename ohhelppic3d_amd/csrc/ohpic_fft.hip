@@ -255,68 +255,103 @@ __device__ void load_lines(const double *__restrict__ src, cplx *buf, const Line
         if (at[k] >= 0) bd[at[k]] = v[k];
 }
 
-// Halfcomplex value at position m of line (pair c, which) from the complex
-// spectrum X of the packed pair.
-__device__ __forceinline__ double hc_value(const cplx *X, int n, int c, int which, int m)
+
+// Task t of a (line pair c, position k < per_c) sweep: x lines take
+// consecutive positions of one pair per wavefront (rows stay contiguous), y
+// and z lines consecutive pairs at one position (the G lines of a position are
+// adjacent in memory).
+template <int AXIS>
+__device__ __forceinline__ void pair_task(int t, int per_c, uint32_t mg_per, int nlines,
+                                          uint32_t mg_nl, int &c, int &k)
 {
-    const int k = (2 * m <= n) ? m : n - m;
-    const cplx zk = X[c * n + k];
-    const cplx znk = X[c * n + (k == 0 ? 0 : n - k)];
-    if (which == 0) {
-        return (2 * m <= n) ? 0.5 * (zk.re + znk.re) : 0.5 * (zk.im - znk.im);
+    if (AXIS == 0) {
+        c = fastdiv(t, mg_per);
+        k = t - c * per_c;
     } else {
-        return (2 * m <= n) ? 0.5 * (zk.im + znk.im) : 0.5 * (znk.re - zk.re);
+        k = fastdiv(t, mg_nl);
+        c = t - k * nlines;
     }
 }
 
-// Packed complex spectrum Z[k] = A[k] + i B[k] from halfcomplex pairs
-// H[c][m] = (hc_a[m], hc_b[m]).
-__device__ __forceinline__ cplx packed_from_hc(const cplx *H, int n, int c, int k)
+// R2HC output: the task (c, k <= n/2) reads Z_k and Z_{n-k} once and writes
+// r_k (position k) and i_k (position n - k) of both lines of the pair, with
+// A_k = (Z_k + conj Z_{n-k}) / 2 and B_k = (Z_k - conj Z_{n-k}) / 2i.
+template <int AXIS>
+__device__ __forceinline__ void store_r2hc_pairs(const cplx *buf, double *__restrict__ dst,
+                                                 const LineBlock &lb, int n, int nlines,
+                                                 const FftAxisDev &ax)
 {
-    const cplx hk = H[c * n + k];
-    const cplx hnk = H[c * n + (k == 0 ? 0 : n - k)];
-    double ra, ia, rb, ib;
-    if (k == 0 || 2 * k == n) {
-        ra = hk.re;
-        ia = 0.0;
-        rb = hk.im;
-        ib = 0.0;
-    } else if (2 * k < n) {
-        ra = hk.re;
-        ia = hnk.re;
-        rb = hk.im;
-        ib = hnk.im;
-    } else {
-        ra = hnk.re;
-        ia = -hk.re;
-        rb = hnk.im;
-        ib = -hk.im;
-    }
-    return cplx{ra - ib, ia + rb};
-}
-
-constexpr int kElemMax = kFftLdsCplx / kFftBlock;  // LDS elements per thread
-
-// In place: buf <- packed spectrum built from the halfcomplex pairs in buf.
-__device__ void build_packed_inplace(cplx *buf, int n, int nlines, uint32_t mg_n)
-{
-    const int total = n * nlines;
-    cplx z[kElemMax];
-#pragma unroll
-    for (int k = 0; k < kElemMax; ++k) {
-        const int t = threadIdx.x + k * kFftBlock;
-        if (t < total) {
-            const int c = fastdiv(t, mg_n);
-            z[k] = packed_from_hc(buf, n, c, t - c * n);
+    const int half = n / 2 + 1;
+    const int total = half * nlines;
+    for (int t = threadIdx.x; t < total; t += kFftBlock) {
+        int c, k;
+        pair_task<AXIS>(t, half, ax.mg_half, nlines, ax.mg_nl, c, k);
+        const int km = k == 0 ? 0 : n - k;
+        const cplx zk = buf[c * n + k], zm = buf[c * n + km];
+        const double ar = 0.5 * (zk.re + zm.re), ai = 0.5 * (zk.im - zm.im);
+        const double br = 0.5 * (zk.im + zm.im), bi = 0.5 * (zm.re - zk.re);
+        const int64_t la = lb.base + (int64_t)(2 * c) * lb.gstride;
+        if (2 * c < lb.nvalid) {
+            dst[la + (int64_t)k * lb.estride] = ar;
+            if (km != k) dst[la + (int64_t)km * lb.estride] = ai;
+        }
+        if (2 * c + 1 < lb.nvalid) {
+            const int64_t lbb = la + lb.gstride;
+            dst[lbb + (int64_t)k * lb.estride] = br;
+            if (km != k) dst[lbb + (int64_t)km * lb.estride] = bi;
         }
     }
-    __syncthreads();
-#pragma unroll
-    for (int k = 0; k < kElemMax; ++k) {
-        const int t = threadIdx.x + k * kFftBlock;
-        if (t < total) buf[t] = z[k];
+}
+
+// HC2R input: the task (c, k <= n/2) loads positions k and n - k of both
+// halfcomplex lines and writes the packed spectrum Z = A + i B at k and n - k
+// (A_k = r_k + i i_k from line a, B_k likewise from line b).
+template <int AXIS>
+__device__ __forceinline__ void load_hc_packed(const double *__restrict__ src, cplx *buf,
+                                               const LineBlock &lb, int n, int nlines,
+                                               const FftAxisDev &ax)
+{
+    const int half = n / 2 + 1;
+    const int total = half * nlines;
+    for (int t = threadIdx.x; t < total; t += kFftBlock) {
+        int c, k;
+        pair_task<AXIS>(t, half, ax.mg_half, nlines, ax.mg_nl, c, k);
+        const int km = k == 0 ? 0 : n - k;
+        const int64_t la = lb.base + (int64_t)(2 * c) * lb.gstride;
+        double ak = 0.0, am = 0.0, bk = 0.0, bm = 0.0;
+        if (2 * c < lb.nvalid) {
+            ak = src[la + (int64_t)k * lb.estride];
+            am = src[la + (int64_t)km * lb.estride];
+        }
+        if (2 * c + 1 < lb.nvalid) {
+            bk = src[la + lb.gstride + (int64_t)k * lb.estride];
+            bm = src[la + lb.gstride + (int64_t)km * lb.estride];
+        }
+        if (km == k) {  // k = 0 and k = n/2: real r_k of both lines
+            buf[c * n + k] = cplx{ak, bk};
+        } else {
+            buf[c * n + k] = cplx{ak - bm, am + bk};
+            buf[c * n + km] = cplx{ak + bm, bk - am};
+        }
     }
-    __syncthreads();
+}
+
+// Complex output (HC2R and the Poisson pass): position m of the pair's
+// spectrum holds (line a, line b) = (re, im).
+template <int AXIS>
+__device__ __forceinline__ void store_complex_pairs(const cplx *buf, double *__restrict__ dst,
+                                                    const LineBlock &lb, int n, int nlines,
+                                                    const FftAxisDev &ax, double scale)
+{
+    const int total = n * nlines;
+    for (int t = threadIdx.x; t < total; t += kFftBlock) {
+        int c, m;
+        pair_task<AXIS>(t, n, ax.mg_n, nlines, ax.mg_nl, c, m);
+        const cplx z = buf[c * n + m];
+        const int64_t la = lb.base + (int64_t)(2 * c) * lb.gstride + (int64_t)m * lb.estride;
+        if (2 * c < lb.nvalid) dst[la] = z.re * scale;
+        if (2 * c + 1 < lb.nvalid) dst[la + lb.gstride] = z.im * scale;
+    }
 }
 
 // One axis pass for every job of the table (blockIdx.y = job, blockIdx.x =
@@ -363,7 +398,10 @@ __global__ void __launch_bounds__(kFftBlock)
         }
         sLamXY[g] = v;
     }
-    load_lines<AXIS>(src, buf, lb, G, n, ax);
+    if (KIND == kFftHC2R)
+        load_hc_packed<AXIS>(src, buf, lb, n, nlines, ax);
+    else
+        load_lines<AXIS>(src, buf, lb, G, n, ax);
     __syncthreads();
 
     if (KIND == kFftR2HC || KIND == kFftPoisson) {
@@ -416,27 +454,13 @@ __global__ void __launch_bounds__(kFftBlock)
             fft_lds<+1, RS>(buf, n, nlines, ax);
         }
     } else {
-        build_packed_inplace(buf, n, nlines, ax.mg_n);
         fft_lds<+1, RS>(buf, n, nlines, ax);
     }
 
-    const int total = G * n;
-#pragma unroll
-    for (int k = 0; k < kLineElems; ++k) {
-        const int e = threadIdx.x + k * kFftBlock;
-        if (e >= total) continue;
-        int g, j;
-        elem_of<AXIS>(e, n, G, ax, g, j);
-        if (g >= lb.nvalid) continue;
-        double v;
-        if (KIND == kFftR2HC) {
-            v = hc_value(buf, n, g >> 1, g & 1, j);
-        } else {
-            const cplx z = buf[(g >> 1) * n + j];
-            v = (g & 1) ? z.im : z.re;
-        }
-        dst[lb.base + (int64_t)g * lb.gstride + (int64_t)j * lb.estride] = v * out_scale;
-    }
+    if (KIND == kFftR2HC)
+        store_r2hc_pairs<AXIS>(buf, dst, lb, n, nlines, ax);
+    else
+        store_complex_pairs<AXIS>(buf, dst, lb, n, nlines, ax, out_scale);
 }
 
 }  // namespace
@@ -446,6 +470,7 @@ void fft_plan_divisors(FftAxisDev &ax)
     ax.mg_n = fastdiv_magic((uint32_t)ax.n);
     ax.mg_G = fastdiv_magic((uint32_t)fft_group_lines(ax.n));
     ax.mg_half = fastdiv_magic((uint32_t)(ax.n / 2 + 1));
+    ax.mg_nl = fastdiv_magic((uint32_t)(fft_group_lines(ax.n) / 2));
     int Ls = 1;
     for (int s = 0; s < kMaxFactors; ++s) {
         const int p = s < ax.nfac ? ax.fac[s] : 1;

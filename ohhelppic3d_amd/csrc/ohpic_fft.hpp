@@ -44,6 +44,7 @@ struct FftAxisDev {
     uint32_t mg_n;                 // n
     uint32_t mg_G;                 // fft_group_lines(n)
     uint32_t mg_half;              // n / 2 + 1
+    uint32_t mg_nl;                // fft_group_lines(n) / 2: line pairs per group
     uint32_t mg_tpl[kMaxFactors];  // n / fac[s]: butterflies per line in stage s
     uint32_t mg_Ls[kMaxFactors];   // fac[0] * ... * fac[s-1]
 };
