@@ -315,6 +315,30 @@ def run_copy_calibration(gib: float = 2.0, reps: int = 5):
             "copy_GBps": 16 * n / dt / 1e9}
 
 
+def run_field_solve(n=(256, 256, 256), reps: int = 20):
+    """The field solve alone (five FFT passes with the fused Poisson z pass,
+    E, the energy sums) on a random charge grid: host-timed mean over `reps`
+    solves; algorithmic bytes per FFT pass are one read and one write of the
+    nx*ny*nz FP64 array."""
+    import ohhelppic3d_amd as ohp
+
+    n = tuple(int(v) for v in n)
+    cfg = ohp.SimConfig(n=n, qm=(-1.0, 0.01), q_macro=(-1.0, 1.0), capacity=(64, 64), dt=0.1)
+    with ohp.Simulation(cfg) as sim:
+        sim.seed_uniform_maxwell([64, 64], [1.0, 0.1])
+        rng = np.random.default_rng(1)
+        sim.put_grid(ohp.FIELD_RHO, rng.standard_normal(n[0] * n[1] * n[2]))
+        sim.field_solve()
+        t = time.perf_counter()
+        for _ in range(reps):
+            sim.field_solve()
+        sim.synchronize()
+        ms = (time.perf_counter() - t) / reps * 1e3
+    ng = n[0] * n[1] * n[2]
+    return {"workload": f"field_solve_{n[0]}x{n[1]}x{n[2]}", "reps": reps, "ms_per_solve": ms,
+            "fft_pass_bytes": 16 * ng, "efield_bytes": 40 * ng}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -328,6 +352,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true")
+    ap.add_argument("--field-solve", action="store_true",
+                    help="time the field solve alone on a random charge grid")
+    ap.add_argument("--field-grid", type=int, nargs=3, default=(256, 256, 256))
+    ap.add_argument("--field-reps", type=int, default=20)
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--ensemble-instances", type=int, default=128)
     ap.add_argument("--ensemble-steps", type=int, default=500)
@@ -338,6 +366,9 @@ def main():
     args = ap.parse_args()
     if args.copy_calibration:
         print(json.dumps(run_copy_calibration()), flush=True)
+        return
+    if args.field_solve:
+        print(json.dumps(run_field_solve(args.field_grid, args.field_reps)), flush=True)
         return
     if args.serve:
         print(json.dumps(run_serving_bench(args)), flush=True)
