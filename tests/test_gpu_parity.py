@@ -42,7 +42,10 @@ def test_fft_impulse_response_reference_kat(ohp):
 
 
 @pytest.mark.parametrize("n", [(8, 8, 8), (6, 10, 12), (16, 4, 9), (32, 32, 32), (24, 48, 16),
-                               (5, 7, 3), (1, 1, 7), (64, 2, 3), (96, 1, 4)])
+                               (5, 7, 3), (1, 1, 7), (64, 2, 3), (96, 1, 4),
+                               # every radix up to 13 and the longest axes
+                               (11, 13, 22), (26, 14, 1), (1024, 2, 3), (2, 1024, 2),
+                               (3, 2, 1024), (256, 6, 5), (2, 640, 3), (1, 1, 1)])
 def test_fft_r2hc_matches_oracle_and_roundtrips(ohp, orc, n):
     rng = np.random.default_rng(7)
     a = rng.standard_normal(n[0] * n[1] * n[2])
@@ -57,7 +60,8 @@ def test_fft_r2hc_matches_oracle_and_roundtrips(ohp, orc, n):
 
 
 # --------------------------------------------------------------------- Poisson
-@pytest.mark.parametrize("n", [(8, 8, 8), (16, 12, 10), (32, 32, 32), (24, 16, 48)])
+@pytest.mark.parametrize("n", [(8, 8, 8), (16, 12, 10), (32, 32, 32), (24, 16, 48),
+                               (11, 13, 14), (256, 4, 6), (5, 1024, 2), (2, 3, 1024)])
 def test_field_solve_matches_oracle(ohp, orc, n):
     rng = np.random.default_rng(11)
     rho = rng.standard_normal(n[0] * n[1] * n[2])
