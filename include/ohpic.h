@@ -219,12 +219,20 @@ int ohpic_group_attach(ohpic_group *g, ohpic_ctx *ctx);
 int ohpic_group_detach(ohpic_group *g, ohpic_ctx *ctx);
 int ohpic_group_size(const ohpic_group *g, int *n);
 int ohpic_group_contains(const ohpic_group *g, const ohpic_ctx *ctx, int *yes);
-/* Advance every member by nsteps (same sequence as ohpic_step).  When a
- * member's step fails (e.g. a particle moved more than two box lengths) the
- * other members still complete that step, the failed members are detached
- * (their contexts keep the message and refuse further steps) and the first
- * failure's code is returned. */
+/* Advance every member by nsteps (same sequence as ohpic_step), in launch
+ * sequences of `batch` steps (ohpic_group_set_batch; default 8) while that
+ * many remain, then single steps; each sequence is one graph launch and one
+ * host synchronisation.  A member whose bins overflow in a sequence keeps its
+ * state for the rest of it; afterwards that step is redone as a list and the
+ * member's remaining steps are taken alone, so every member ends nsteps
+ * further.  When a member's step fails (e.g. a particle moved more than two
+ * box lengths) the other members still complete the sequence, the failed
+ * members are detached (their contexts keep the message and refuse further
+ * steps) and the first failure's code is returned; ohpic_group_stats counts
+ * the steps taken. */
 int ohpic_group_step(ohpic_group *g, int nsteps);
+/* steps per launch sequence, 1 to 16 */
+int ohpic_group_set_batch(ohpic_group *g, int steps);
 /* enable != 0 (default): replay captured hipGraphs; 0: launch directly */
 int ohpic_group_use_graph(ohpic_group *g, int enable);
 /* enable != 0 (default): the particle step of all members is one persistent

@@ -720,6 +720,11 @@ void fft_twiddles(int n, double2 *tw)
 }  // namespace ohpic
 
 // ================================================================== groups
+// Steps per launch sequence of a group: each sequence is one graph launch and
+// one host synchronisation (with one read-back of every step's flags).
+constexpr int kGroupMaxBatch = 16;
+constexpr int kGroupDefaultBatch = 8;
+
 struct ohpic_group {
     int device = 0;
     hipStream_t st = nullptr;
@@ -746,8 +751,11 @@ struct ohpic_group {
     bool persistent = true;          // queued particle launch over a resident grid
     unsigned int *d_queue = nullptr;  // its work counter
     int queue_blocks = 0;             // resident workgroups of that launch
-    hipGraph_t graph = nullptr;
+    int batch = kGroupDefaultBatch;  // steps per launch sequence (one host sync each)
+    hipGraph_t graph = nullptr;       // one step
     hipGraphExec_t exec = nullptr;
+    hipGraph_t graph_b = nullptr;     // `batch` steps
+    hipGraphExec_t exec_b = nullptr;
     int64_t steps = 0, captures = 0;
     std::string err;
 };
@@ -779,8 +787,10 @@ void drop_graph(ohpic_group *g)
 {
     if (g->exec) (void)hipGraphExecDestroy(g->exec);
     if (g->graph) (void)hipGraphDestroy(g->graph);
-    g->exec = nullptr;
-    g->graph = nullptr;
+    if (g->exec_b) (void)hipGraphExecDestroy(g->exec_b);
+    if (g->graph_b) (void)hipGraphDestroy(g->graph_b);
+    g->exec = g->exec_b = nullptr;
+    g->graph = g->graph_b = nullptr;
 }
 
 template <typename T>
@@ -806,8 +816,8 @@ int group_reserve(ohpic_group *g, int ninst, int64_t nitems)
         GHIP(g, hipMalloc((void **)&g->d_insts, cap * sizeof(InstDev)));
         GHIP(g, hipMalloc((void **)&g->d_jobs, cap * sizeof(FftJob)));
         GHIP(g, hipMalloc((void **)&g->d_parity, cap * sizeof(int)));
-        GHIP(g, hipMalloc((void **)&g->d_flags, (size_t)cap * kFlagSlots * 8));
-        GHIP(g, hipHostMalloc((void **)&g->h_flags, (size_t)cap * kFlagSlots * 8,
+        GHIP(g, hipMalloc((void **)&g->d_flags, (size_t)cap * kGroupMaxBatch * kFlagSlots * 8));
+        GHIP(g, hipHostMalloc((void **)&g->h_flags, (size_t)cap * kGroupMaxBatch * kFlagSlots * 8,
                               hipHostMallocDefault));
         g->cap_insts = cap;
     }
@@ -908,45 +918,60 @@ int group_refresh(ohpic_group *g)
 }
 
 // One step of every member: solve, fused particle kernel, KE sums, flags.
-hipError_t group_enqueue(ohpic_group *g)
+// Enqueues k consecutive steps of every member; step s leaves the members'
+// flags in d_flags[s][member], copied to h_flags in one transfer at the end.
+// A member whose bins overflow keeps its parity on the device, so its later
+// steps of the sequence repeat that step from the same state.
+hipError_t group_enqueue(ohpic_group *g, int k)
 {
     const int n = (int)g->members.size();
-    hipError_t e = solve_passes(g->d_insts, g->d_parity, g->d_jobs, n, true, g->max_ng, g->grp,
-                                g->rs, g->max_rows, g->st);
-    if (e == hipSuccess)
-        e = launch_prepare_next_b(g->d_insts, g->d_parity, n, g->max_ng, g->max_ntiles,
-                                  g->persistent ? g->d_queue : nullptr, g->st);
-    if (e == hipSuccess) {
-        if (g->persistent)
-            e = launch_particles_queue(g->precision, g->weight_order, g->T, g->d_insts,
-                                       g->d_items, g->nitems, g->d_parity, g->d_queue,
-                                       g->queue_blocks, g->st);
-        else
-            e = launch_particles(kModeFused, g->precision, g->weight_order, g->T, g->d_insts,
-                                 g->d_items, g->nitems, g->d_parity, g->st);
+    hipError_t e = hipSuccess;
+    for (int s = 0; s < k && e == hipSuccess; ++s) {
+        e = solve_passes(g->d_insts, g->d_parity, g->d_jobs, n, true, g->max_ng, g->grp, g->rs,
+                         g->max_rows, g->st);
+        if (e == hipSuccess)
+            e = launch_prepare_next_b(g->d_insts, g->d_parity, n, g->max_ng, g->max_ntiles,
+                                      g->persistent ? g->d_queue : nullptr, g->st);
+        if (e == hipSuccess) {
+            if (g->persistent)
+                e = launch_particles_queue(g->precision, g->weight_order, g->T, g->d_insts,
+                                           g->d_items, g->nitems, g->d_parity, g->d_queue,
+                                           g->queue_blocks, g->st);
+            else
+                e = launch_particles(kModeFused, g->precision, g->weight_order, g->T, g->d_insts,
+                                     g->d_items, g->nitems, g->d_parity, g->st);
+        }
+        if (e == hipSuccess) e = launch_reduce_ke_b(g->d_insts, n, g->st);
+        if (e == hipSuccess)
+            e = launch_finish_b(g->d_insts, g->d_parity, n,
+                                g->d_flags + (size_t)s * n * kFlagSlots, g->st);
     }
-    if (e == hipSuccess) e = launch_reduce_ke_b(g->d_insts, n, g->st);
-    if (e == hipSuccess) e = launch_finish_b(g->d_insts, g->d_parity, n, g->d_flags, g->st);
     if (e == hipSuccess)
-        e = hipMemcpyAsync(g->h_flags, g->d_flags, (size_t)n * kFlagSlots * 8,
+        e = hipMemcpyAsync(g->h_flags, g->d_flags, (size_t)k * n * kFlagSlots * 8,
                            hipMemcpyDeviceToHost, g->st);
     return e;
 }
 
-int group_capture(ohpic_group *g)
+// Captures the k-step sequence (k = 1 or g->batch) into its graph.
+int group_capture(ohpic_group *g, int k)
 {
-    drop_graph(g);
+    hipGraph_t &graph = k == 1 ? g->graph : g->graph_b;
+    hipGraphExec_t &exec = k == 1 ? g->exec : g->exec_b;
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    exec = nullptr;
+    graph = nullptr;
     GHIP(g, hipStreamBeginCapture(g->st, hipStreamCaptureModeThreadLocal));
-    const hipError_t e = group_enqueue(g);
-    hipGraph_t graph = nullptr;
-    const hipError_t e2 = hipStreamEndCapture(g->st, &graph);
+    const hipError_t e = group_enqueue(g, k);
+    hipGraph_t captured = nullptr;
+    const hipError_t e2 = hipStreamEndCapture(g->st, &captured);
     if (e != hipSuccess || e2 != hipSuccess) {
-        if (graph) (void)hipGraphDestroy(graph);
+        if (captured) (void)hipGraphDestroy(captured);
         return gfail(g, OHPIC_EHIP, "graph capture failed: %s",
                      hipGetErrorString(e != hipSuccess ? e : e2));
     }
-    g->graph = graph;
-    GHIP(g, hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0));
+    graph = captured;
+    GHIP(g, hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
     g->captures += 1;
     return OHPIC_OK;
 }
@@ -955,7 +980,7 @@ int group_step(ohpic_group *g, int nsteps)
 {
     GHIP(g, hipSetDevice(g->device));
     const int n = (int)g->members.size();
-    for (int it = 0; it < nsteps && n > 0; ++it) {
+    for (int done = 0; done < nsteps && n > 0;) {
         for (int i = 0; i < n; ++i) {
             ohpic_ctx *m = g->members[i];
             if (!m->binned)
@@ -969,27 +994,51 @@ int group_step(ohpic_group *g, int nsteps)
         }
         int rc = group_refresh(g);
         if (rc) return rc;
+        // whole sequences of `batch` steps while that many remain, then single
+        // steps: one host synchronisation per sequence
+        const int k = (g->batch > 1 && nsteps - done >= g->batch) ? g->batch : 1;
         if (g->use_graph) {
-            if (!g->exec && (rc = group_capture(g))) return rc;
-            GHIP(g, hipGraphLaunch(g->exec, g->st));
+            if (!(k == 1 ? g->exec : g->exec_b) && (rc = group_capture(g, k))) return rc;
+            GHIP(g, hipGraphLaunch(k == 1 ? g->exec : g->exec_b, g->st));
         } else {
-            GHIP(g, group_enqueue(g));
+            GHIP(g, group_enqueue(g, k));
         }
         GHIP(g, hipStreamSynchronize(g->st));
-        // every member's step is accounted for before a failure is reported:
-        // failed members leave the group, the others have completed the step
+        // every member's steps are accounted for before a failure is reported:
+        // failed members leave the group, the others have completed the k steps
         int first_rc = OHPIC_OK;
         std::string msg;
         std::vector<ohpic_ctx *> failed;
         for (int i = 0; i < n; ++i) {
             ohpic_ctx *m = g->members[i];
-            rc = absorb_flags(m, g->h_flags + (size_t)i * kFlagSlots);
-            if (rc == 1) {
-                // the device kept this member's parity: redo its step alone
-                rc = rerun_as_list(m, kModeFusedList);
-            } else if (rc == 0) {
-                m->par ^= 1;
-                g->dev_par[i] ^= 1;
+            // a step whose bins overflowed left the member's state unchanged
+            // (the device kept its parity): count the steps that went through,
+            // redo the overflowed one as a list and take the rest alone
+            int ok = 0;
+            bool overflowed = false;
+            rc = OHPIC_OK;
+            for (int s = 0; s < k; ++s) {
+                const unsigned long long *f = g->h_flags + ((size_t)s * n + i) * kFlagSlots;
+                if (f[kFlagOverflow] && !f[kFlagRange]) {
+                    if (!overflowed) m->fast += (int64_t)f[kFlagFast];
+                    overflowed = true;
+                    continue;
+                }
+                if ((rc = absorb_flags(m, f)) < 0) break;
+                ++ok;
+            }
+            if (rc == OHPIC_OK) {
+                if (ok & 1) {
+                    m->par ^= 1;
+                    g->dev_par[i] ^= 1;
+                }
+                m->step += ok;
+                m->rho_valid = true;
+                if (ok < k) {
+                    rc = rerun_as_list(m, kModeFusedList);
+                    if (rc == OHPIC_OK) m->step += 1;
+                    for (int r = ok + 1; r < k && rc == OHPIC_OK; ++r) rc = one_step(m);
+                }
             }
             if (rc < 0) {
                 m->failed = true;
@@ -998,12 +1047,10 @@ int group_step(ohpic_group *g, int nsteps)
                 char buf[64];
                 snprintf(buf, sizeof buf, "%smember %d: ", msg.empty() ? "" : "; ", i);
                 msg += buf + m->err;
-                continue;
             }
-            m->step += 1;
-            m->rho_valid = true;
         }
-        g->steps += 1;
+        g->steps += k;
+        done += k;
         if (first_rc != OHPIC_OK) {
             for (ohpic_ctx *m : failed) (void)group_detach(g, m);
             return gfail(g, first_rc, "%s (failed members were detached)", msg.c_str());
@@ -2010,6 +2057,20 @@ int ohpic_group_use_graph(ohpic_group *g, int enable)
     (void)hipSetDevice(g->device);
     (void)hipStreamSynchronize(g->st);
     drop_graph(g);
+    return OHPIC_OK;
+}
+
+int ohpic_group_set_batch(ohpic_group *g, int steps)
+{
+    if (!g) return fail(nullptr, OHPIC_EINVAL, "group is NULL");
+    if (steps < 1 || steps > kGroupMaxBatch)
+        return gfail(g, OHPIC_EINVAL, "batch of %d steps: 1 to %d", steps, kGroupMaxBatch);
+    (void)hipSetDevice(g->device);
+    (void)hipStreamSynchronize(g->st);
+    if (steps != g->batch) {
+        g->batch = steps;
+        drop_graph(g);
+    }
     return OHPIC_OK;
 }
 

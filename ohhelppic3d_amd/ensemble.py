@@ -85,10 +85,12 @@ def sample_ensemble(count: int = 128, seed: int = 2024, axes: Sequence[int] = AX
 
 def run_ensemble(specs: Sequence[InstanceSpec], steps: int, warmup: int = 2,
                  tune: Optional[dict] = None, tune_sample: int = 16, use_graph: bool = True,
-                 precision: int = L.MIXED,
+                 precision: int = L.MIXED, batch: int = 8,
                  progress: Optional[Callable[[str], None]] = None) -> dict:
-    """Advance every instance ``steps`` steps as one group and time each group
-    step (host wall clock around the synchronous step)."""
+    """Advance every instance ``steps`` steps as one group, in launch
+    sequences of ``batch`` steps, and time each sequence (host wall clock
+    around the synchronous call); a step's time is its sequence's over its
+    length."""
     say = progress or (lambda s: None)
     t0 = time.perf_counter()
     if tune is None:
@@ -98,22 +100,26 @@ def run_ensemble(specs: Sequence[InstanceSpec], steps: int, warmup: int = 2,
             f"({tune['candidates']} candidates, {time.perf_counter() - t0:.1f} s)")
     t1 = time.perf_counter()
     sims = []
-    grp = Group(use_graph=use_graph)
+    grp = Group(use_graph=use_graph, batch=batch)
     try:
         for s in specs:
             sims.append(s.create(tune["tile"], tune["chunks"], precision))
             grp.attach(sims[-1])
         say(f"[ensemble] {len(sims)} instances created and seeded in "
             f"{time.perf_counter() - t1:.1f} s")
-        for _ in range(warmup):
-            grp.step(1)
+        if warmup > 0:
+            grp.step(warmup)
         per_step = []
-        for k in range(steps):
+        done = 0
+        while done < steps:
+            k = min(batch, steps - done)
             ts = time.perf_counter()
-            grp.step(1)
-            per_step.append(time.perf_counter() - ts)
-            if (k + 1) % max(1, steps // 5) == 0:
-                say(f"[ensemble] step {k + 1}/{steps}: {per_step[-1] * 1e3:.2f} ms")
+            grp.step(k)
+            dt = time.perf_counter() - ts
+            per_step.extend([dt / k] * k)
+            if (done + k) // max(1, steps // 5) != done // max(1, steps // 5):
+                say(f"[ensemble] step {done + k}/{steps}: {dt / k * 1e3:.2f} ms")
+            done += k
         stats = grp.stats()
         energy = [sim.diagnostics()["total_energy"] for sim in sims]
     finally:
@@ -130,7 +136,8 @@ def run_ensemble(specs: Sequence[InstanceSpec], steps: int, warmup: int = 2,
         "step_ms_max": float(a.max()), "completion_s": total,
         "tile": list(tune["tile"]), "chunks": tune["chunks"],
         "autotune_particle_steps_per_s": tune.get("particle_steps_per_s"),
-        "graph": use_graph, "graph_captures": stats["captures"], "work_items": stats["items"],
+        "graph": use_graph, "batch": batch, "graph_captures": stats["captures"],
+        "work_items": stats["items"],
         "energy_finite": bool(np.all(np.isfinite(energy))),
     }
 

@@ -255,7 +255,8 @@ class Group:
     hipGraph.  Members must share device, precision, weight order and tile;
     each member's result equals stepping it alone with ``Simulation.step``."""
 
-    def __init__(self, device: int = 0, use_graph: bool = True, persistent: bool = True):
+    def __init__(self, device: int = 0, use_graph: bool = True, persistent: bool = True,
+                 batch: Optional[int] = None):
         self._lib = L.lib()
         self._g = C.c_void_p()
         check(self._lib.ohpic_group_create(int(device), C.byref(self._g)))
@@ -264,6 +265,8 @@ class Group:
             self.use_graph(False)
         if not persistent:
             self.use_persistent(False)
+        if batch is not None:
+            self.set_batch(batch)
 
     def _chk(self, rc):
         L.check_group(rc, self._g)
@@ -308,8 +311,10 @@ class Group:
         return bool(y.value)
 
     def step(self, nsteps: int = 1) -> None:
-        """Advance every member; on a member failure the others complete the
-        step, failed members are detached, and OhpicError is raised."""
+        """Advance every member by nsteps, in launch sequences of `batch` steps
+        (one host synchronisation each) while that many remain; on a member
+        failure the others complete the sequence, failed members are detached,
+        and OhpicError is raised (``stats()["steps"]`` counts the steps taken)."""
         rc = self._lib.ohpic_group_step(self._g, int(nsteps))
         if rc != L.OK:
             self.members = [m for m in self.members if self.contains(m)]
@@ -320,6 +325,10 @@ class Group:
 
     def use_persistent(self, on: bool) -> None:
         self._chk(self._lib.ohpic_group_use_persistent(self._g, 1 if on else 0))
+
+    def set_batch(self, steps: int) -> None:
+        """Steps per launch sequence (1 to 16, default 8)."""
+        self._chk(self._lib.ohpic_group_set_batch(self._g, int(steps)))
 
     def stats(self) -> dict:
         s, c, i = C.c_int64(), C.c_int64(), C.c_int64()

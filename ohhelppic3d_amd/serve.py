@@ -211,11 +211,15 @@ class ServeReport:
 
 class ServingLoop:
     """Admit arriving runs into one device-resident group and advance the
-    group one step at a time until every run has finished."""
+    group until every run has finished: each group call takes up to `batch`
+    steps (one launch sequence, one host synchronisation), never past the end
+    of an active run, and with the "steps" clock never past the next arrival;
+    with the wall clock an arrival waits at most one such call."""
 
     def __init__(self, budget_bytes: int = HBM_BUDGET, max_active: int = 256,
                  tile=(8, 8, 8), chunks: int = 1, precision: int = L.MIXED,
-                 use_graph: bool = True, track_ids: bool = False, device: int = 0):
+                 use_graph: bool = True, track_ids: bool = False, device: int = 0,
+                 batch: int = 8):
         self.budget = int(budget_bytes)
         self.max_active = int(max_active)
         self.tile = tuple(tile)
@@ -224,6 +228,7 @@ class ServingLoop:
         self.use_graph = use_graph
         self.track_ids = track_ids
         self.device = device
+        self.batch = int(batch)
 
     def config(self, req: Request) -> SimConfig:
         return req.spec.config(self.tile, self.chunks, self.precision, self.track_ids,
@@ -247,7 +252,7 @@ class ServingLoop:
             raise ValueError("request ids must be unique")
         sched = Scheduler(self.budget, self.max_active, self.footprint)
         active: Dict[int, list] = {}
-        grp = Group(self.device, self.use_graph)
+        grp = Group(self.device, self.use_graph, batch=self.batch)
         t0 = time.perf_counter()
         nsteps = 0
         active_sum = 0
@@ -310,22 +315,27 @@ class ServingLoop:
                     if sched.waiting:  # cannot happen: the head fits an empty budget
                         raise RuntimeError("serving loop stalled with waiting requests")
                     continue
+                k = min(self.batch, min(entry[2] for entry in active.values()))
+                if clock == "steps" and pi < len(pending):
+                    k = min(k, max(1, int(math.ceil(pending[pi].arrival)) - nsteps))
                 err = None
+                before = grp.stats()["steps"]
                 try:
-                    grp.step(1)
+                    grp.step(k)
                 except OhpicError as e:
                     err = str(e)
-                nsteps += 1
-                active_sum += len(active)
+                taken = grp.stats()["steps"] - before
+                nsteps += taken
+                active_sum += len(active) * taken
                 for rid in list(active):
                     entry = active[rid]
                     if err is not None and not grp.contains(entry[1]):
                         finish(rid, "failed", err)
                         continue
-                    entry[2] -= 1
+                    entry[2] -= taken
                     if entry[2] <= 0:
                         finish(rid, "done")
-                if progress and nsteps % 200 == 0:
+                if progress and nsteps // 200 != (nsteps - taken) // 200:
                     say(f"[serve] step {nsteps}: {len(active)} active, "
                         f"{len(sched.waiting)} waiting, {sched.used / 1e9:.1f} GB admitted")
         finally:

@@ -97,6 +97,7 @@ def test_group_and_autotune_errors_without_gpu(L):
         lib.ohpic_group_destroy(g)
         return
     assert lib.ohpic_group_step(None, 1) == L.EINVAL
+    assert lib.ohpic_group_set_batch(None, 4) == L.EINVAL
     assert lib.ohpic_group_attach(None, None) == L.EINVAL
     assert lib.ohpic_group_size(None, None) == L.EINVAL
     assert lib.ohpic_group_destroy(None) == L.OK

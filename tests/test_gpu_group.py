@@ -72,10 +72,10 @@ def run_alone(ohp, precision, steps, **kw):
     return snaps
 
 
-def run_group(ohp, precision, steps, use_graph=True, persistent=True, **kw):
+def run_group(ohp, precision, steps, use_graph=True, persistent=True, batch=None, **kw):
     sims = [make(ohp, n, ppc, seed, precision, **kw) for n, ppc, seed in CASES]
     try:
-        with ohp.Group(use_graph=use_graph, persistent=persistent) as grp:
+        with ohp.Group(use_graph=use_graph, persistent=persistent, batch=batch) as grp:
             for s in sims:
                 grp.attach(s)
             assert len(grp) == len(CASES)
@@ -117,6 +117,18 @@ def test_group_persistent_launch_equals_per_item_launch(ohp, precision):
         assert_same(a, b, f"{precision} member {i}")
 
 
+@pytest.mark.parametrize("use_graph", [True, False])
+def test_group_step_sequences_equal_alone(ohp, use_graph):
+    # 10 steps in sequences of 4: two 4-step launches, then two single steps
+    alone = run_alone(ohp, ohp.MIXED, 10)
+    grouped, stats = run_group(ohp, ohp.MIXED, 10, use_graph=use_graph, batch=4)
+    assert stats["steps"] == 10
+    if use_graph:
+        assert stats["captures"] >= 2  # the 4-step and the 1-step sequence
+    for i, (a, g) in enumerate(zip(alone, grouped)):
+        assert_same(g, a, f"member {i}")
+
+
 def drifting_pair(ohp):
     # every particle starts in tile 0 and moves one cell into tile 1, whose
     # bin capacity is the minimum: the step overflows and is redone as a list
@@ -145,6 +157,26 @@ def test_group_member_overflow_is_redone(ohp):
             got = [snapshot(ohp, a), snapshot(ohp, b)]
             assert got[0]["diag"]["repacks"] >= 1
             assert got[0]["diag"]["repacks"] == ref[0]["diag"]["repacks"]
+    for i in range(2):
+        assert_same(got[i], ref[i], f"member {i}")
+
+
+def test_overflow_inside_a_step_sequence_is_redone(ohp):
+    # the overflowing member keeps its state for the rest of the sequence,
+    # then the step is redone as a list and its remaining steps run alone
+    with drifting_pair(ohp) as a, make(ohp, (16, 16, 24), 4, 9, ohp.FP64) as b:
+        a.step(8)
+        b.step(8)
+        ref = [snapshot(ohp, a), snapshot(ohp, b)]
+    with drifting_pair(ohp) as a, make(ohp, (16, 16, 24), 4, 9, ohp.FP64) as b:
+        with ohp.Group(batch=4) as grp:
+            grp.attach(a)
+            grp.attach(b)
+            grp.step(8)
+            assert grp.stats()["steps"] == 8
+            got = [snapshot(ohp, a), snapshot(ohp, b)]
+            assert got[0]["diag"]["step"] == 8 and got[1]["diag"]["step"] == 8
+            assert got[0]["diag"]["repacks"] >= 1
     for i in range(2):
         assert_same(got[i], ref[i], f"member {i}")
 
