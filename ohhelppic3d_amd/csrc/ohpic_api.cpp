@@ -507,29 +507,29 @@ int do_deposit(ohpic_ctx *ctx)
     return OHPIC_OK;
 }
 
-// Poisson solve of the instance table `insts` (n instances, FFT jobs `jobs`):
-// rho from the int64 grid (from_acc), five axis passes, node E + energies.
-hipError_t solve_passes(const InstDev *insts, const int *parity, const FftJob *jobs, int n,
-                        bool from_acc, int64_t max_ng, const int64_t grp[3], const int rs[3],
-                        int max_eblocks, hipStream_t st)
+// Poisson solve of the instance table `insts` (n instances, FFT jobs `jobs`)
+// from rho: five axis passes, node E and its energy partials (reduced when
+// reduce_e).
+hipError_t solve_passes(const FftJob *jobs, const InstDev *insts, int n, const int64_t grp[3],
+                        const int rs[3], int max_eblocks, bool reduce_e, hipStream_t st)
 {
-    hipError_t e = hipSuccess;
-    if (from_acc) e = launch_rho_from_acc_b(insts, parity, n, max_ng, st);
-    if (e == hipSuccess) e = fft_pass(kPassR2hcXFromSrc, jobs, n, grp[0], rs[0], st);
+    hipError_t e = fft_pass(kPassR2hcXFromSrc, jobs, n, grp[0], rs[0], st);
     if (e == hipSuccess) e = fft_pass(kPassR2hcY, jobs, n, grp[1], rs[1], st);
     if (e == hipSuccess) e = fft_pass(kPassPoissonZ, jobs, n, grp[2], rs[2], st);
     if (e == hipSuccess) e = fft_pass(kPassHc2rY, jobs, n, grp[1], rs[1], st);
     if (e == hipSuccess) e = fft_pass(kPassHc2rXScaled, jobs, n, grp[0], rs[0], st);
-    if (e == hipSuccess) e = launch_efield_b(insts, n, max_eblocks, st);
+    if (e == hipSuccess) e = launch_efield_b(insts, n, max_eblocks, reduce_e, st);
     return e;
 }
 
-int do_solve(ohpic_ctx *ctx, bool from_acc)
+// The field solve from rho (FFT passes, E); reduce_e: also the field-energy
+// sums (a step leaves them to its end-of-step reduction).
+int do_solve(ohpic_ctx *ctx, bool reduce_e)
 {
     timer_begin(ctx, 1);
-    HIPCHK(ctx, solve_passes(ctx->d_inst, parity_of(ctx), ctx->d_job, 1, from_acc, ctx->ng,
-                             ctx->fft_grp, ctx->fft_rs,
-                             efield_blocks_of(ctx->n[0], ctx->n[1], ctx->n[2]), ctx->st));
+    HIPCHK(ctx, solve_passes(ctx->d_job, ctx->d_inst, 1, ctx->fft_grp, ctx->fft_rs,
+                             efield_blocks_of(ctx->n[0], ctx->n[1], ctx->n[2]), reduce_e,
+                             ctx->st));
     timer_end(ctx, 1);
     return OHPIC_OK;
 }
@@ -551,14 +551,16 @@ int one_step(ohpic_ctx *ctx)
 {
     int rc;
     if (!ctx->rho_valid && (rc = do_deposit(ctx))) return rc;
-    if ((rc = do_solve(ctx, true))) return rc;
-    HIPCHK(ctx, launch_prepare_next_b(ctx->d_inst, parity_of(ctx), 1, ctx->ng, ctx->ntiles,
-                                      nullptr, ctx->st));
+    // rho from the charge grid and the next buffer's zeroing in one sweep,
+    // the solve from rho, the step, and all energy sums in one reduction
+    HIPCHK(ctx, launch_begin_step_b(ctx->d_inst, parity_of(ctx), 1, ctx->ng, ctx->ntiles,
+                                    nullptr, ctx->st));
+    if ((rc = do_solve(ctx, false))) return rc;
     timer_begin(ctx, 0);
     HIPCHK(ctx, launch_particles(kModeFused, ctx->cfg.precision, ctx->cfg.weight_order, ctx->T,
                                  ctx->d_inst, ctx->d_items, ctx->nitems, parity_of(ctx), ctx->st));
     timer_end(ctx, 0);
-    HIPCHK(ctx, launch_reduce_ke_b(ctx->d_inst, 1, ctx->st));
+    HIPCHK(ctx, launch_end_step_b(ctx->d_inst, nullptr, 1, nullptr, ctx->st));
     if ((rc = read_flags(ctx))) return rc;
     timers_collect(ctx);
     rc = absorb_flags(ctx, ctx->h_flags);
@@ -927,11 +929,10 @@ hipError_t group_enqueue(ohpic_group *g, int k)
     const int n = (int)g->members.size();
     hipError_t e = hipSuccess;
     for (int s = 0; s < k && e == hipSuccess; ++s) {
-        e = solve_passes(g->d_insts, g->d_parity, g->d_jobs, n, true, g->max_ng, g->grp, g->rs,
-                         g->max_rows, g->st);
+        e = launch_begin_step_b(g->d_insts, g->d_parity, n, g->max_ng, g->max_ntiles,
+                                g->persistent ? g->d_queue : nullptr, g->st);
         if (e == hipSuccess)
-            e = launch_prepare_next_b(g->d_insts, g->d_parity, n, g->max_ng, g->max_ntiles,
-                                      g->persistent ? g->d_queue : nullptr, g->st);
+            e = solve_passes(g->d_jobs, g->d_insts, n, g->grp, g->rs, g->max_rows, false, g->st);
         if (e == hipSuccess) {
             if (g->persistent)
                 e = launch_particles_queue(g->precision, g->weight_order, g->T, g->d_insts,
@@ -941,10 +942,9 @@ hipError_t group_enqueue(ohpic_group *g, int k)
                 e = launch_particles(kModeFused, g->precision, g->weight_order, g->T, g->d_insts,
                                      g->d_items, g->nitems, g->d_parity, g->st);
         }
-        if (e == hipSuccess) e = launch_reduce_ke_b(g->d_insts, n, g->st);
         if (e == hipSuccess)
-            e = launch_finish_b(g->d_insts, g->d_parity, n,
-                                g->d_flags + (size_t)s * n * kFlagSlots, g->st);
+            e = launch_end_step_b(g->d_insts, g->d_parity, n,
+                                  g->d_flags + (size_t)s * n * kFlagSlots, g->st);
     }
     if (e == hipSuccess)
         e = hipMemcpyAsync(g->h_flags, g->d_flags, (size_t)k * n * kFlagSlots * 8,
@@ -1625,7 +1625,7 @@ int ohpic_field_solve(ohpic_ctx *ctx)
 {
     if (!ctx) return fail(nullptr, OHPIC_EINVAL, "ctx is NULL");
     HIPCHK(ctx, hipSetDevice(ctx->device));
-    int rc = do_solve(ctx, false);
+    int rc = do_solve(ctx, true);
     if (rc) return rc;
     HIPCHK(ctx, hipStreamSynchronize(ctx->st));
     timers_collect(ctx);

@@ -185,17 +185,62 @@ __global__ void k_prepare_next_b(const InstDev *__restrict__ insts, const int *_
     if (t0 < kFlagSlots) I->flags[t0] = 0ull;
 }
 
-// After a re-binning step: out[i*kFlagSlots + k] := flags[k] of instance i,
-// and the instance's parity flips unless a bin overflowed (the host then
-// re-runs that instance's step in list mode from the unchanged buffer).
-__global__ void k_finish_b(const InstDev *__restrict__ insts, int *__restrict__ parity, int ninst,
-                           unsigned long long *__restrict__ out)
+// Start of a step: rho := the current charge grid * 2^-charge_bits, and the
+// zeroing of k_prepare_next_b (next charge grid and bin words, flags, queue
+// counter) in the same sweep.
+__global__ void k_begin_step_b(const InstDev *__restrict__ insts, const int *__restrict__ parity,
+                               unsigned int *__restrict__ queue)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= ninst) return;
-    const unsigned long long *f = insts[i].flags;
-    for (int k = 0; k < kFlagSlots; ++k) out[i * kFlagSlots + k] = f[k];
-    if (f[kFlagOverflow] == 0ull) parity[i] ^= 1;
+    if (queue && blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) *queue = 0u;
+    const InstDev *I = insts + blockIdx.y;
+    const int par = parity[blockIdx.y];
+    const int64_t n = (int64_t)I->n[0] * I->n[1] * I->n[2];
+    const int64_t t0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    const unsigned long long *__restrict__ acc = I->rho_acc[par];
+    unsigned long long *__restrict__ next = I->rho_acc[par ^ 1];
+    double *__restrict__ rho = I->rho;
+    const double scale = I->fix_inv;
+    for (int64_t i = t0; i < n; i += stride) {
+        rho[i] = (double)(long long)acc[i] * scale;
+        next[i] = 0ull;
+    }
+    for (int s = 0; s < I->nspecies; ++s) {
+        unsigned long long *__restrict__ w = I->sp[s].count[par ^ 1];
+        for (int64_t t = t0; t < I->ntiles; t += stride) w[t] = 0ull;
+    }
+    if (t0 < kFlagSlots) I->flags[t0] = 0ull;
+}
+
+// End of a step: blocks x < kKeSlots reduce the kinetic-energy slots (as
+// k_reduce_ke_b), block kKeSlots the field-energy partials of k_efield_b (as
+// k_reduce_e_b) and, when out is not null, finishes the instance's step in a
+// group: out[i*kFlagSlots + k] := flags[k] of instance i, and the parity flips
+// unless a bin overflowed (the host then redoes that step in list mode from
+// the unchanged buffer).
+__global__ void __launch_bounds__(256) k_end_step_b(const InstDev *__restrict__ insts,
+                                                    int *__restrict__ parity,
+                                                    unsigned long long *__restrict__ out)
+{
+    __shared__ double red[256];
+    const InstDev *I = insts + blockIdx.y;
+    if ((int)blockIdx.x < kKeSlots) {
+        const int k = blockIdx.x;
+        const double v = block_ordered_sum(I->ke_partial, I->nitems, kKeSlots, k, red);
+        if (threadIdx.x == 0) I->diag[kDiagKeOld + k] = v;
+        return;
+    }
+    const int count = efield_blocks_of(I->n[0], I->n[1], I->n[2]);
+    for (int k = 0; k < 3; ++k) {
+        const double v = block_ordered_sum(I->e_partial, count, 3, k, red);
+        if (threadIdx.x == 0) I->diag[kDiagWe + k] = v;
+    }
+    if (out && threadIdx.x == 0) {
+        const int i = blockIdx.y;
+        const unsigned long long *f = I->flags;
+        for (int k = 0; k < kFlagSlots; ++k) out[i * kFlagSlots + k] = f[k];
+        if (f[kFlagOverflow] == 0ull) parity[i] ^= 1;
+    }
 }
 
 __global__ void k_load_uniform(Pcg32Dev rng0, uint64_t offset, int64_t np, int nx, int ny,
@@ -315,12 +360,33 @@ hipError_t launch_rho_from_acc_b(const InstDev *insts, const int *parity, int ni
     return hipGetLastError();
 }
 
-hipError_t launch_efield_b(const InstDev *insts, int ninst, int max_blocks, hipStream_t st)
+hipError_t launch_efield_b(const InstDev *insts, int ninst, int max_blocks, bool reduce,
+                           hipStream_t st)
 {
     if (ninst <= 0) return hipSuccess;
     const int blocks = (int)round_up_xcds(max_blocks > 0 ? max_blocks : 1);
     hipLaunchKernelGGL(k_efield_b, dim3(blocks, ninst), dim3(256), 0, st, insts);
-    hipLaunchKernelGGL(k_reduce_e_b, dim3(ninst), dim3(256), 0, st, insts);
+    if (reduce) hipLaunchKernelGGL(k_reduce_e_b, dim3(ninst), dim3(256), 0, st, insts);
+    return hipGetLastError();
+}
+
+hipError_t launch_begin_step_b(const InstDev *insts, const int *parity, int ninst,
+                               int64_t max_ng, int max_ntiles, unsigned int *queue,
+                               hipStream_t st)
+{
+    if (ninst <= 0) return hipSuccess;
+    const int64_t n = max_ng > max_ntiles ? max_ng : max_ntiles;
+    hipLaunchKernelGGL(k_begin_step_b, dim3(grid_blocks(n, ninst), ninst), dim3(256), 0, st,
+                       insts, parity, queue);
+    return hipGetLastError();
+}
+
+hipError_t launch_end_step_b(const InstDev *insts, int *parity, int ninst,
+                             unsigned long long *out, hipStream_t st)
+{
+    if (ninst <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_end_step_b, dim3(kKeSlots + 1, ninst), dim3(256), 0, st, insts,
+                       parity, out);
     return hipGetLastError();
 }
 
@@ -339,15 +405,6 @@ hipError_t launch_reduce_ke_b(const InstDev *insts, int ninst, hipStream_t st)
 {
     if (ninst <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_reduce_ke_b, dim3(kKeSlots, ninst), dim3(256), 0, st, insts);
-    return hipGetLastError();
-}
-
-hipError_t launch_finish_b(const InstDev *insts, int *parity, int ninst, unsigned long long *out,
-                           hipStream_t st)
-{
-    if (ninst <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_finish_b, dim3((ninst + 255) / 256), dim3(256), 0, st, insts, parity,
-                       ninst, out);
     return hipGetLastError();
 }
 

@@ -83,10 +83,22 @@ hipError_t launch_fill_f64(double *p, int64_t n, double v, hipStream_t st);
 // rho := int64 charge grid of the current buffer * 2^-charge_bits
 hipError_t launch_rho_from_acc_b(const InstDev *insts, const int *parity, int ninst,
                                  int64_t max_ng, hipStream_t st);
-// node E from phi + block partials of sum rho*phi, sum E^2, sum rho; then the
-// partials are reduced into diag[kDiagWe..kDiagQnet]
+// node E from phi + block partials of sum rho*phi, sum E^2, sum rho; with
+// reduce, the partials are then reduced into diag[kDiagWe..kDiagQnet] (else
+// launch_end_step_b does it)
 // max_blocks: the largest efield_blocks_of() over the instances
-hipError_t launch_efield_b(const InstDev *insts, int ninst, int max_blocks, hipStream_t st);
+hipError_t launch_efield_b(const InstDev *insts, int ninst, int max_blocks, bool reduce,
+                           hipStream_t st);
+// rho from the current charge grid and the zeroing of launch_prepare_next_b,
+// in one launch at the start of a step
+hipError_t launch_begin_step_b(const InstDev *insts, const int *parity, int ninst,
+                               int64_t max_ng, int max_ntiles, unsigned int *queue,
+                               hipStream_t st);
+// launch_reduce_ke_b, the field-energy reduction of launch_efield_b and (out
+// not null, group steps) flags to out[i * kFlagSlots + k] with a parity flip
+// unless instance i overflowed a bin, in one launch at the end of a step
+hipError_t launch_end_step_b(const InstDev *insts, int *parity, int ninst,
+                             unsigned long long *out, hipStream_t st);
 // zero the next buffer's bin words and charge grid, the flags, and *queue
 // (if not null: the counter of a following launch_particles_queue)
 hipError_t launch_prepare_next_b(const InstDev *insts, const int *parity, int ninst,
@@ -94,10 +106,6 @@ hipError_t launch_prepare_next_b(const InstDev *insts, const int *parity, int ni
                                  hipStream_t st);
 // diag[kDiagKeOld..] := sums of ke_partial over the instance's items
 hipError_t launch_reduce_ke_b(const InstDev *insts, int ninst, hipStream_t st);
-// out[i * kFlagSlots + k] := flags[k] of instance i; parity[i] ^= 1 unless
-// instance i overflowed a bin
-hipError_t launch_finish_b(const InstDev *insts, int *parity, int ninst, unsigned long long *out,
-                           hipStream_t st);
 
 // seeded loaders (ohpic_fields.hip, ohpic_particles_mixed.hip)
 hipError_t launch_load_uniform(Pcg32Dev rng0, uint64_t offset, int64_t np, int nx, int ny,
