@@ -1015,10 +1015,11 @@ int group_step(ohpic_group *g, int nsteps)
             // (the device kept its parity): count the steps that went through,
             // redo the overflowed one as a list and take the rest alone
             int ok = 0;
-            bool overflowed = false;
+            bool overflowed = false, last_ok = false;
             rc = OHPIC_OK;
             for (int s = 0; s < k; ++s) {
                 const unsigned long long *f = g->h_flags + ((size_t)s * n + i) * kFlagSlots;
+                last_ok = false;
                 if (f[kFlagOverflow] && !f[kFlagRange]) {
                     if (!overflowed) m->fast += (int64_t)f[kFlagFast];
                     overflowed = true;
@@ -1026,6 +1027,7 @@ int group_step(ohpic_group *g, int nsteps)
                 }
                 if ((rc = absorb_flags(m, f)) < 0) break;
                 ++ok;
+                last_ok = true;
             }
             if (rc == OHPIC_OK) {
                 if (ok & 1) {
@@ -1034,11 +1036,22 @@ int group_step(ohpic_group *g, int nsteps)
                 }
                 m->step += ok;
                 m->rho_valid = true;
-                if (ok < k) {
+                int r = ok;
+                if (r < k && !last_ok) {
+                    // the last step of the sequence overflowed: E still belongs
+                    // to the member's current state, so that step is redone as
+                    // a list (as a single step would be)
                     rc = rerun_as_list(m, kModeFusedList);
-                    if (rc == OHPIC_OK) m->step += 1;
-                    for (int r = ok + 1; r < k && rc == OHPIC_OK; ++r) rc = one_step(m);
+                    if (rc == OHPIC_OK) {
+                        m->step += 1;
+                        ++r;
+                    }
                 }
+                // an overflow followed by a step that went through (bin
+                // arrivals race with the front reservation near a full bin)
+                // leaves E one step behind: the rest run alone, each with its
+                // own field solve
+                for (; r < k && rc == OHPIC_OK; ++r) rc = one_step(m);
             }
             if (rc < 0) {
                 m->failed = true;
