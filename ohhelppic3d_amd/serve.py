@@ -2,9 +2,11 @@
 
 Requests (a grid, particles per cell, a step count and the plasma parameters
 of one run) arrive over time.  The loop keeps every admitted run's particles
-and fields resident on the device and advances all of them together, one
-``Group`` step at a time, so runs that arrive while others are on the GPU
-join the next step and their work overlaps on the device.  Admission is FIFO
+and fields resident on the device and advances all of them together in
+``Group`` launch sequences of a few steps, so runs that arrive while others
+are on the GPU join at the next sequence and their work overlaps on the
+device; new runs are created by worker threads while the group steps.
+Admission is FIFO
 against a device-memory budget (the library's footprint estimate per run) and
 a slot limit; a run that finishes frees its slot and memory at once.  A run
 whose footprint exceeds the whole budget is rejected; a run that fails
@@ -25,6 +27,7 @@ import json
 import math
 import time
 from collections import deque
+from concurrent.futures import ThreadPoolExecutor
 from dataclasses import asdict, dataclass
 from typing import Callable, Dict, List, Optional, Sequence, Tuple
 
@@ -214,12 +217,15 @@ class ServingLoop:
     group until every run has finished: each group call takes up to `batch`
     steps (one launch sequence, one host synchronisation), never past the end
     of an active run, and with the "steps" clock never past the next arrival;
-    with the wall clock an arrival waits at most one such call."""
+    with the wall clock an arrival waits at most one such call.  With the wall
+    clock, admitted runs are created (allocated, seeded, binned on their own
+    streams) by `creators` worker threads while the group keeps stepping, and
+    join the group at the first step after they are ready."""
 
     def __init__(self, budget_bytes: int = HBM_BUDGET, max_active: int = 256,
                  tile=(8, 8, 8), chunks: int = 1, precision: int = L.MIXED,
                  use_graph: bool = True, track_ids: bool = False, device: int = 0,
-                 batch: int = 8):
+                 batch: int = 8, creators: int = 2):
         self.budget = int(budget_bytes)
         self.max_active = int(max_active)
         self.tile = tuple(tile)
@@ -229,6 +235,19 @@ class ServingLoop:
         self.track_ids = track_ids
         self.device = device
         self.batch = int(batch)
+        self.creators = int(creators)
+
+    def _create(self, req: Request):
+        """(Simulation, None), or (None, error message) if creation failed."""
+        sim = None
+        try:
+            sim = req.spec.create(self.tile, self.chunks, self.precision, self.track_ids,
+                                  self.device)
+            return sim, None
+        except OhpicError as e:
+            if sim is not None:
+                sim.close()
+            return None, str(e)
 
     def config(self, req: Request) -> SimConfig:
         return req.spec.config(self.tile, self.chunks, self.precision, self.track_ids,
@@ -253,6 +272,11 @@ class ServingLoop:
         sched = Scheduler(self.budget, self.max_active, self.footprint)
         active: Dict[int, list] = {}
         grp = Group(self.device, self.use_graph, batch=self.batch)
+        # wall clock: runs are created by worker threads while the group steps
+        # (the library's calls release the interpreter lock); steps clock:
+        # created in line, so the mix is reproducible
+        pool = ThreadPoolExecutor(self.creators) if clock == "wall" and self.creators > 0 else None
+        creating: Dict[int, tuple] = {}
         t0 = time.perf_counter()
         nsteps = 0
         active_sum = 0
@@ -260,6 +284,25 @@ class ServingLoop:
 
         def now() -> float:
             return time.perf_counter() - t0 if clock == "wall" else float(nsteps)
+
+        def join(r: Request, made) -> None:
+            # made: (Simulation, None) or (None, error message)
+            sim, err = made
+            o = outcomes[r.rid]
+            if sim is not None:
+                try:
+                    grp.attach(sim)
+                except OhpicError as e:
+                    sim.close()
+                    sim, err = None, str(e)
+            if sim is None:
+                sched.complete(r)
+                o.status = "failed"
+                o.error = err
+                return
+            active[r.rid] = [r, sim, r.steps]
+            if r.steps == 0:
+                finish(r.rid, "done")
 
         def finish(rid: int, status: str, err: str = "") -> None:
             req, sim, _ = active.pop(rid)
@@ -288,23 +331,20 @@ class ServingLoop:
                         o.error = (f"footprint {self.footprint(r)} B exceeds the budget "
                                    f"{self.budget} B")
                 for r in sched.admit():
-                    o = outcomes[r.rid]
-                    o.admitted = now()
-                    sim = None
-                    try:
-                        sim = r.spec.create(self.tile, self.chunks, self.precision,
-                                            self.track_ids, self.device)
-                        grp.attach(sim)
-                    except OhpicError as e:
-                        if sim is not None:
-                            sim.close()
-                        sched.complete(r)
-                        o.status = "failed"
-                        o.error = str(e)
-                        continue
-                    active[r.rid] = [r, sim, r.steps]
-                    if r.steps == 0:
-                        finish(r.rid, "done")
+                    outcomes[r.rid].admitted = now()
+                    if pool is not None:
+                        creating[r.rid] = (r, pool.submit(self._create, r))
+                    else:
+                        join(r, self._create(r))
+                for rid in [rid for rid, (_, fut) in creating.items() if fut.done()]:
+                    r, fut = creating.pop(rid)
+                    join(r, fut.result())
+                if not active and creating:
+                    # nothing to step: wait for the first run being created
+                    rid = next(iter(creating))
+                    r, fut = creating.pop(rid)
+                    join(r, fut.result())
+                    continue
                 if not active:
                     if pi < len(pending):
                         if clock == "wall":
@@ -339,6 +379,16 @@ class ServingLoop:
                     say(f"[serve] step {nsteps}: {len(active)} active, "
                         f"{len(sched.waiting)} waiting, {sched.used / 1e9:.1f} GB admitted")
         finally:
+            for rid, (r, fut) in list(creating.items()):
+                sim, _ = fut.result()
+                if sim is not None:
+                    sim.close()
+                sched.complete(r)
+                outcomes[rid].status = "failed"
+                outcomes[rid].error = "serving loop stopped"
+            creating.clear()
+            if pool is not None:
+                pool.shutdown(wait=True)
             for rid in list(active):
                 finish(rid, "failed", "serving loop stopped")
             grp.close()

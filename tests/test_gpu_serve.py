@@ -71,6 +71,36 @@ def test_serving_mix_equals_alone(ohp):
         assert np.array_equal(a["counts"], m["counts"]), r.rid
 
 
+def test_wall_clock_serving_creates_runs_while_stepping(ohp):
+    # wall clock: runs arriving while others step are created by worker
+    # threads and join later; each result still equals the run alone
+    from ohhelppic3d_amd import serve
+    R = serve.Request
+    reqs = [R(0, 0.0, (32, 32, 32), (8, 8), 60, 1.0, 0.5, 21),
+            R(1, 0.02, (16, 24, 16), (4, 6), 30, 1.5, 0.2, 22),
+            R(2, 0.04, (24, 16, 32), (5, 3), 25, 0.7, 0.9, 23),
+            R(3, 0.06, (16, 16, 16), (6, 4), 40, 2.0, 0.0, 24)]
+    kw = dict(tile=(8, 8, 8), chunks=1, track_ids=True)
+    mix = {}
+    rep = serve.ServingLoop(creators=2, **kw).run(
+        reqs, clock="wall", on_complete=lambda r, sim: mix.__setitem__(r.rid, snapshot(ohp, sim)))
+    assert rep.summary()["done"] == len(reqs), [(o.rid, o.status, o.error) for o in rep.outcomes]
+    for r in reqs:
+        one = serve.Request(**{**r.to_dict(), "arrival": 0.0, "n": tuple(r.n),
+                               "ppc": tuple(r.ppc)})
+        alone = {}
+        serve.ServingLoop(creators=0, **kw).run(
+            [one], clock="steps",
+            on_complete=lambda q, sim: alone.__setitem__(q.rid, snapshot(ohp, sim)))
+        a, m = alone[r.rid], mix[r.rid]
+        assert a["step"] == m["step"] == r.steps
+        for k, (fa, fm) in enumerate(zip(a["fields"], m["fields"])):
+            assert np.array_equal(fa, fm), (r.rid, "field", k)
+        for sp in range(2):
+            for key in ("pid", "x", "vx"):
+                assert np.array_equal(a["parts"][sp][key], m["parts"][sp][key]), (r.rid, sp, key)
+
+
 def test_serving_rejects_oversized_and_survives_failures(ohp):
     from ohhelppic3d_amd import serve
     R = serve.Request
