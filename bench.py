@@ -271,7 +271,7 @@ def run_serving_bench(args):
     tune = ohp.autotune(cfgs, sample=8, steps=2)
     log(f"[serve] autotune: tile {tune['tile']}, chunks {tune['chunks']} "
         f"({time.perf_counter() - t:.1f} s)")
-    loop = serve.ServingLoop(tile=tune["tile"], chunks=tune["chunks"])
+    loop = serve.ServingLoop(tile=tune["tile"], chunks=tune["chunks"], creators=args.serve_creators)
     log(f"[serve] capacity run: {n} requests at t=0")
     rep = loop.run(probe, progress=log)
     cap = rep.summary()
@@ -292,7 +292,8 @@ def run_serving_bench(args):
         "config": {"workload": f"serving{n}_mixed", "grids": [list(g) for g in serve.GRIDS],
                    "grid_weights": list(serve.GRID_WEIGHTS), "ppc_range": [16, 64],
                    "steps_range": [200, 2000], "budget_bytes": serve.HBM_BUDGET,
-                   "tile": list(tune["tile"]), "chunks": tune["chunks"]},
+                   "tile": list(tune["tile"]), "chunks": tune["chunks"],
+                   "creators": args.serve_creators, "batch": loop.batch},
         "capacity_requests_per_s": cap_rps, "levels": levels,
     }
 
@@ -363,6 +364,8 @@ def main():
     ap.add_argument("--serve", action="store_true",
                     help="run the serving-loop workload instead (its own JSON line)")
     ap.add_argument("--serve-requests", type=int, default=48)
+    ap.add_argument("--serve-creators", type=int, default=2,
+                    help="worker threads creating admitted runs (0: in the loop)")
     args = ap.parse_args()
     if args.copy_calibration:
         print(json.dumps(run_copy_calibration()), flush=True)
