@@ -377,7 +377,7 @@ def main():
         print(json.dumps(run_serving_bench(args)), flush=True)
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path "
+        print("bench.py measures one GPU: the library has no multi-GPU path "
               "(one MI355X holds the whole 256^3 domain)", file=sys.stderr)
         sys.exit(2)
     res = run_gpu(args)
