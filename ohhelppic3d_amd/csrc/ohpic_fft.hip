@@ -49,7 +49,10 @@ __device__ __forceinline__ cplx cmul(cplx a, cplx b)
 template <int SIGN>
 __device__ __forceinline__ cplx twiddle(const double2 *__restrict__ tw, int k)
 {
-    const double2 w = tw[k];
+    // one 16-byte global load (a native vector type: HIP's double2 class does
+    // not bind to a global-address-space reference)
+    typedef double d2v __attribute__((ext_vector_type(2)));
+    const d2v w = gload(reinterpret_cast<const d2v *>(tw), k);
     return cplx{w.x, SIGN < 0 ? w.y : -w.y};
 }
 
@@ -230,7 +233,7 @@ constexpr int kLineElems = 2 * kFftLdsCplx / kFftBlock;  // real elements per th
 // thread issues all of its (up to kLineElems) global loads before the first
 // LDS write, so each lane keeps that many reads in flight.
 template <int AXIS>
-__device__ void load_lines(const double *__restrict__ src, cplx *buf, const LineBlock &lb,
+__device__ void load_lines(const gmem_t<double> *__restrict__ src, cplx *buf, const LineBlock &lb,
                            int G, int n, const FftAxisDev &ax)
 {
     double *bd = reinterpret_cast<double *>(buf);
@@ -277,7 +280,7 @@ __device__ __forceinline__ void pair_task(int t, int per_c, uint32_t mg_per, int
 // r_k (position k) and i_k (position n - k) of both lines of the pair, with
 // A_k = (Z_k + conj Z_{n-k}) / 2 and B_k = (Z_k - conj Z_{n-k}) / 2i.
 template <int AXIS>
-__device__ __forceinline__ void store_r2hc_pairs(const cplx *buf, double *__restrict__ dst,
+__device__ __forceinline__ void store_r2hc_pairs(const cplx *buf, gmem_t<double> *__restrict__ dst,
                                                  const LineBlock &lb, int n, int nlines,
                                                  const FftAxisDev &ax)
 {
@@ -307,7 +310,7 @@ __device__ __forceinline__ void store_r2hc_pairs(const cplx *buf, double *__rest
 // halfcomplex lines and writes the packed spectrum Z = A + i B at k and n - k
 // (A_k = r_k + i i_k from line a, B_k likewise from line b).
 template <int AXIS>
-__device__ __forceinline__ void load_hc_packed(const double *__restrict__ src, cplx *buf,
+__device__ __forceinline__ void load_hc_packed(const gmem_t<double> *__restrict__ src, cplx *buf,
                                                const LineBlock &lb, int n, int nlines,
                                                const FftAxisDev &ax)
 {
@@ -339,7 +342,7 @@ __device__ __forceinline__ void load_hc_packed(const double *__restrict__ src, c
 // Complex output (HC2R and the Poisson pass): position m of the pair's
 // spectrum holds (line a, line b) = (re, im).
 template <int AXIS>
-__device__ __forceinline__ void store_complex_pairs(const cplx *buf, double *__restrict__ dst,
+__device__ __forceinline__ void store_complex_pairs(const cplx *buf, gmem_t<double> *__restrict__ dst,
                                                     const LineBlock &lb, int n, int nlines,
                                                     const FftAxisDev &ax, double scale)
 {
@@ -379,8 +382,8 @@ __global__ void __launch_bounds__(kFftBlock)
     // neighbouring line groups read the two halves of the same 128-byte
     // lines (y and z passes): keep them on one XCD
     const int grp = xcd_remap((int)blockIdx.x, (int)ngroups);
-    const double *__restrict__ src = FROM_SRC ? J.src : J.data;
-    double *__restrict__ dst = J.data;
+    const gmem_t<double> *__restrict__ src = as_global(FROM_SRC ? J.src : J.data);
+    gmem_t<double> *__restrict__ dst = as_global(J.data);
     const double out_scale = SCALE ? J.inv_n : 1.0;
     const int nlines = (G + 1) / 2;
     const LineBlock lb = line_block<AXIS>(grp, G, nx, ny, nz);
@@ -394,7 +397,7 @@ __global__ void __launch_bounds__(kFftBlock)
             const int l = (int)lb.base + g;
             const int iy = l / nx;
             const int ix = l - iy * nx;
-            v = (l == 0) ? -1.0 : J.lam[0][ix] + J.lam[1][iy];
+            v = (l == 0) ? -1.0 : gload(J.lam[0], ix) + gload(J.lam[1], iy);
         }
         sLamXY[g] = v;
     }
@@ -414,7 +417,7 @@ __global__ void __launch_bounds__(kFftBlock)
             // inverse transform is (n a', n b') as HC2R of the halfcomplex
             // quotients gives.  One task per (pair, k <= n/2) owns slots k and
             // n - k, so the update needs no barrier of its own.
-            const double *__restrict__ lamz = J.lam[2];
+            const gmem_t<double> *__restrict__ lamz = as_global(J.lam[2]);
             const int half = n / 2 + 1;
             const int total = half * nlines;
             constexpr int kTasks = (kFftLdsCplx / 2 + kFftLdsCplx / 16 + kFftBlock - 1) / kFftBlock + 1;

@@ -37,11 +37,11 @@ __global__ void __launch_bounds__(256) k_efield_b(const InstDev *__restrict__ in
 {
     __shared__ double red[3][4];
     const InstDev *I = insts + blockIdx.y;
-    const double *__restrict__ phi = I->phi;
-    const double *__restrict__ rho = I->rho;
-    double *__restrict__ ex = I->e[0];
-    double *__restrict__ ey = I->e[1];
-    double *__restrict__ ez = I->e[2];
+    const gmem_t<double> *__restrict__ phi = as_global(I->phi);
+    const gmem_t<double> *__restrict__ rho = as_global(I->rho);
+    gmem_t<double> *__restrict__ ex = as_global(I->e[0]);
+    gmem_t<double> *__restrict__ ey = as_global(I->e[1]);
+    gmem_t<double> *__restrict__ ez = as_global(I->e[2]);
     const int nx = I->n[0], ny = I->n[1], nz = I->n[2];
     const int rows = ny * nz;
     const int rpb = efield_rows_per_block(nx);
@@ -77,14 +77,14 @@ __global__ void __launch_bounds__(256) k_efield_b(const InstDev *__restrict__ in
             const int64_t o = row + ix;
             const int64_t oxp = ix + 1 < nx ? o + 1 : row;
             const int64_t oxm = ix > 0 ? o - 1 : row + nx - 1;
-            const double p0 = phi[o];
+            // all eight loads are issued before the stores
+            const double p0 = phi[o], q = rho[o];
             const double vx = 0.5 * ((phi[oxm] - p0) + (p0 - phi[oxp]));
             const double vy = 0.5 * ((phi[o + dym] - p0) + (p0 - phi[o + dyp]));
             const double vz = 0.5 * ((phi[o + dzm] - p0) + (p0 - phi[o + dzp]));
             ex[o] = vx;
             ey[o] = vy;
             ez[o] = vz;
-            const double q = rho[o];
             we += q * p0;
             we2 += vx * vx + vy * vy + vz * vz;
             qn += q;

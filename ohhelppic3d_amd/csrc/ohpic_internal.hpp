@@ -109,6 +109,33 @@ __host__ __device__ inline int efield_blocks_of(int nx, int ny, int nz)
 void set_global_error(const char *msg);
 
 // ---------------------------------------------------------------- helpers
+// Every particle and grid buffer is device (global) memory, but pointers read
+// from the instance table or a job table are generic to the compiler, which
+// then emits flat instructions; those also count against the LDS wait
+// counter, so each wait for an LDS access would wait for every load and store
+// in flight.
+// Accesses through these helpers compile to global_* instructions.
+template <typename T>
+using gmem_t = __attribute__((address_space(1))) T;
+
+template <typename T>
+__device__ __forceinline__ gmem_t<T> *as_global(T *p)
+{
+    return (gmem_t<T> *)p;
+}
+
+template <typename T>
+__device__ __forceinline__ T gload(const T *p, int64_t i)
+{
+    return as_global(p)[i];
+}
+
+template <typename T>
+__device__ __forceinline__ void gstore(T *p, int64_t i, T v)
+{
+    as_global(p)[i] = v;
+}
+
 __device__ __forceinline__ long long fix_round(double w, double qfix)
 {
     // round(w * qfix) to an integer for |w * qfix| < 2^51: one FMA against

@@ -43,32 +43,6 @@ __device__ __forceinline__ double apply_periodic(double p, int n)
 
 __device__ __forceinline__ int lane_id() { return __lane_id(); }
 
-// Every particle and grid buffer is device (global) memory, but pointers read
-// from the instance table are generic to the compiler, which then emits flat
-// instructions; those also count against the LDS wait counter, so each wait
-// for an LDS access would wait for every particle load and store in flight.
-// Accesses through these helpers compile to global_* instructions.
-template <typename T>
-using gmem_t = __attribute__((address_space(1))) T;
-
-template <typename T>
-__device__ __forceinline__ gmem_t<T> *as_global(T *p)
-{
-    return (gmem_t<T> *)p;
-}
-
-template <typename T>
-__device__ __forceinline__ T gload(const T *p, int64_t i)
-{
-    return as_global(p)[i];
-}
-
-template <typename T>
-__device__ __forceinline__ void gstore(T *p, int64_t i, T v)
-{
-    as_global(p)[i] = v;
-}
-
 // Particle state streams through each step once: with OHPIC_NT set, its
 // loads and stores carry the non-temporal hint (measured 3-7% slower on the
 // 256^3 x 64 ppc step, so off by default).
