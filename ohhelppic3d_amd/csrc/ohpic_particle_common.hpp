@@ -43,6 +43,17 @@ __device__ __forceinline__ double apply_periodic(double p, int n)
 
 __device__ __forceinline__ int lane_id() { return __lane_id(); }
 
+// The value lane `src` holds, for a wave-uniform `src`: v_readlane through a
+// scalar register (a shuffle would route all 64 lanes through the LDS unit,
+// which the gather and deposit keep busy).
+__device__ __forceinline__ int lane_bcast(int v, int src) { return __builtin_amdgcn_readlane(v, src); }
+__device__ __forceinline__ unsigned long long lane_bcast(unsigned long long v, int src)
+{
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, src);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), src);
+    return ((unsigned long long)hi << 32) | lo;
+}
+
 // Particle state streams through each step once: with OHPIC_NT set, its
 // loads and stores carry the non-temporal hint (measured 3-7% slower on the
 // 256^3 x 64 ppc step, so off by default).
@@ -91,11 +102,11 @@ __device__ __forceinline__ unsigned long long wave_counter_add(unsigned long lon
     unsigned long long result = 0ull;
     for (int r = 0; r < kAggRounds && todo; ++r) {
         const int leader = __ffsll((unsigned long long)todo) - 1;
-        const int tl = __shfl(t, leader, 64);
+        const int tl = lane_bcast(t, leader);
         const unsigned long long m = __ballot(valid && t == tl) & todo;
         unsigned long long old = 0ull;
         if (lane == leader) old = gatomic_add(&counter[tl], (unsigned long long)__popcll(m));
-        old = __shfl(old, leader, 64);
+        old = lane_bcast(old, leader);
         if ((m >> lane) & 1ull) result = old + (unsigned long long)__popcll(m & lt);
         todo &= ~m;
     }

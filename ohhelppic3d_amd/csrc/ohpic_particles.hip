@@ -397,8 +397,8 @@ __global__ void __launch_bounds__(kBlock)
                             b = bin_front(old);
                             lim = cap - bin_back(old);
                         }
-                        b = __shfl(b, leader, 64);
-                        lim = __shfl(lim, leader, 64);
+                        b = lane_bcast(b, leader);
+                        lim = lane_bcast(lim, leader);
                         if (stay) {
                             const int slot = b + (int)__popcll(m & lanemask_lt);
                             if (slot < lim)
