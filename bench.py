@@ -95,6 +95,7 @@ def run_gpu(args, workload=None):
     t_total = time.perf_counter() - t_start
     kt = sim.kernel_times()
     diag = sim.diagnostics()
+    tile = list(sim.tile)
     sim.close()
 
     total_p = 2 * npcl
@@ -128,7 +129,7 @@ def run_gpu(args, workload=None):
         "data": "synthetic (device-seeded PCG32 uniform Maxwellian)",
         "config": {"workload": workload, "grid": list(n), "ppc_per_species": ppc,
                    "species": 2, "particles": total_p, "dt": 0.1, "b0z": 0.5,
-                   "tile": list(args.tile), "slack": args.slack},
+                   "tile": tile, "slack": args.slack},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "push_fused", "kernel_avg_ms": push_avg,
@@ -347,7 +348,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="uniform256_ppc64_fp64", choices=list(WORKLOADS))
     ap.add_argument("--slack", type=float, default=0.15)
-    ap.add_argument("--tile", type=int, nargs=3, default=(8, 8, 8))
+    ap.add_argument("--tile", type=int, nargs=3, default=(0, 0, 0),
+                    help="cells per tile (0 0 0: the library's default, 8^3 at 256^3)")
     ap.add_argument("--chunks", type=int, default=0)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")

@@ -70,7 +70,8 @@ typedef struct {
     double b0[3];          /* uniform magnetic field */
     int32_t precision;     /* OHPIC_FP64 or OHPIC_MIXED */
     int32_t weight_order;  /* OHPIC_WEIGHTS_* */
-    int32_t tile[3];       /* cells per tile; 0 = default (8 if n >= 32 else 4) */
+    int32_t tile[3];       /* cells per tile; 0 = default (8^3 if the grid holds
+                              at least 512 such tiles, else 4^3) */
     double slack;          /* bin capacity slack fraction; <= 0 = 0.15 */
     int32_t chunks;        /* work items per tile; 0 = fill the device */
     int32_t track_ids;     /* keep a 64-bit particle id beside each particle */

@@ -149,12 +149,15 @@ int64_t species_alloc(const ohpic_config &c, int s, int ntiles)
     return (int64_t)std::ceil((double)c.capacity[s] * (1.0 + slack)) + (int64_t)ntiles * 80 + 64;
 }
 
+// Default tile: 8^3 cells when the grid holds at least 512 such tiles, else
+// 4^3 (fewer, larger tiles leave CUs idle: at 32^3 and 48^3 the 4^3 tiles
+// step 1.2-2.1x faster, from 96^3 up the 8^3 tiles 1.1-1.9x).
 void pick_tile(const ohpic_config &c, int T[3])
 {
     for (int a = 0; a < 3; ++a) T[a] = c.tile[a];
     if (T[0] <= 0 || T[1] <= 0 || T[2] <= 0) {
-        const int m = std::min(c.nx, std::min(c.ny, c.nz));
-        const int t = m >= 32 ? 8 : 4;
+        const int64_t tiles8 = (int64_t)((c.nx + 7) / 8) * ((c.ny + 7) / 8) * ((c.nz + 7) / 8);
+        const int t = tiles8 >= 512 ? 8 : 4;
         T[0] = T[1] = T[2] = t;
     }
 }
