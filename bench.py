@@ -59,7 +59,7 @@ def make_sim(ohp, wl, args, track_ids=False):
     return sim, cfg, npcl
 
 
-def run_gpu(args, workload=None):
+def run_gpu(args, workload=None, timing=True):
     import ohhelppic3d_amd as ohp
 
     workload = workload or args.workload
@@ -80,7 +80,9 @@ def run_gpu(args, workload=None):
     # energy drift is measured from the first diagnostics available: after
     # the warm-up, or after the first timed step without warm-up
     e_start = sim.diagnostics()["total_energy"] if args.warmup > 0 else None
-    sim.enable_timing(True)
+    # per-kernel HIP events for the roofline (they add gaps between launches,
+    # which only small grids notice)
+    sim.enable_timing(timing)
     per_step = []
     t_start = time.perf_counter()
     for i in range(args.steps):
@@ -409,7 +411,7 @@ def main():
         log("[bench] the CPU baseline's workload on the GPU")
         small = argparse.Namespace(**vars(args))
         small.steps, small.warmup = 100, 5
-        r = run_gpu(small, "uniform32_ppc16_fp64")
+        r = run_gpu(small, "uniform32_ppc16_fp64", timing=False)
         cpu = res["cpu_baseline"]
         res["cpu_workload_on_gpu"] = {
             "workload": "uniform32_ppc16_fp64", "value": r["value"], "unit": "particle-steps/s",
