@@ -370,6 +370,32 @@ def test_partial_tiles_match_oracle(ohp, orc, precision):
         assert counts.sum() == ref_counts.sum()
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_results_do_not_depend_on_tile(ohp, orc, precision):
+    # particles, cell counts and rho/phi/E are the same bit for bit whatever
+    # the tile shape: the deposit is integer, and the mixed form keeps
+    # positions at 2^-24 cells relative to the tile origin for every tile
+    n = (16, 16, 16)
+    sp = two_species(orc, n, 8)
+    prec = ohp.FP64 if precision == "fp64" else ohp.MIXED
+    out = []
+    for tile in ((4, 4, 4), (8, 8, 8), (8, 8, 4)):
+        with sim_for(sp, n, tile=tile, precision=prec) as sim:
+            sim.step(3)
+            grids = [sim.grid(w) for w in (ohp.FIELD_RHO, ohp.FIELD_PHI, ohp.FIELD_EX,
+                                           ohp.FIELD_EY, ohp.FIELD_EZ)]
+            parts = [by_pid(sim.store_particles(s, True)) for s in range(2)]
+            out.append((grids, parts, sim.cell_counts()))
+    g0, p0, c0 = out[0]
+    for g, p, c in out[1:]:
+        for a, b in zip(g, g0):
+            assert np.array_equal(a, b)
+        for s in range(2):
+            for k in p0[s]:
+                assert np.array_equal(p[s][k], p0[s][k]), k
+        assert np.array_equal(c, c0)
+
+
 def test_empty_species_and_empty_instance(ohp, orc):
     n = (16, 16, 16)
     sp = two_species(orc, n, 2)
