@@ -363,7 +363,8 @@ def main():
     ap.add_argument("--field-reps", type=int, default=20)
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--no-small", action="store_true",
-                    help="skip the GPU run of the CPU baseline's workload (32^3 x 16 ppc)")
+                    help="skip the GPU run of the CPU baseline's workload (32^3 x 16 ppc), "
+                         "which follows the CPU baseline")
     ap.add_argument("--ensemble-instances", type=int, default=128)
     ap.add_argument("--ensemble-steps", type=int, default=500)
     ap.add_argument("--ensemble-seq-steps", type=int, default=10)
@@ -405,7 +406,7 @@ def main():
         res["cpu_baseline"] = run_cpu_baseline(args.cpu_seconds)
     else:
         res["cpu_baseline"] = None
-    if not args.no_small and args.workload == "uniform256_ppc64_fp64":
+    if not args.no_small and res["cpu_baseline"] and args.workload == "uniform256_ppc64_fp64":
         # the CPU baseline's own workload (32^3 x 16 ppc x 2 species, FP64) on
         # the GPU, stepped one call per step like the headline
         log("[bench] the CPU baseline's workload on the GPU")
@@ -416,7 +417,7 @@ def main():
         res["cpu_workload_on_gpu"] = {
             "workload": "uniform32_ppc16_fp64", "value": r["value"], "unit": "particle-steps/s",
             "ms_per_step": r["ms_per_step"], "steps": small.steps,
-            "vs_cpu_baseline": r["value"] / cpu["value"] if cpu else None}
+            "vs_cpu_baseline": r["value"] / cpu["value"]}
     print(json.dumps(res), flush=True)
 
 
