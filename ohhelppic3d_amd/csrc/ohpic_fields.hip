@@ -113,12 +113,34 @@ __global__ void __launch_bounds__(256) k_efield_b(const InstDev *__restrict__ in
     }
 }
 
-// Sum of partial[i*stride + k] over i < count, in index order, by one block.
+// threads of the blocks that reduce per-item and per-block partial sums
+constexpr int kSumBlock = 1024;
+
+// Sum of partial[i*stride + k] over i < count by one block, in an order that
+// depends on count alone.  Each thread keeps kSumLanes running sums (entries
+// i, i + blockDim, ... of one trip go to sums 0, 1, ...), so that many loads
+// are in flight; a single running sum made the reduction over the 32768 work
+// items of a 256^3 step wait for one load at a time.
 __device__ double block_ordered_sum(const double *__restrict__ partial, int count, int stride,
                                     int k, double *red)
 {
+    constexpr int kSumLanes = 8;
+    const int nb = (int)blockDim.x;
+    double acc[kSumLanes];
+#pragma unroll
+    for (int a = 0; a < kSumLanes; ++a) acc[a] = 0.0;
+    int i = threadIdx.x;
+    for (; i + (kSumLanes - 1) * nb < count; i += kSumLanes * nb) {
+#pragma unroll
+        for (int a = 0; a < kSumLanes; ++a)
+            acc[a] += gload(partial, (int64_t)(i + a * nb) * stride + k);
+    }
+#pragma unroll
+    for (int a = 0; a < kSumLanes; ++a)
+        if (i + a * nb < count) acc[a] += gload(partial, (int64_t)(i + a * nb) * stride + k);
     double s = 0.0;
-    for (int i = threadIdx.x; i < count; i += blockDim.x) s += partial[(int64_t)i * stride + k];
+#pragma unroll
+    for (int a = 0; a < kSumLanes; ++a) s += acc[a];
     red[threadIdx.x] = s;
     __syncthreads();
     for (int h = blockDim.x / 2; h > 0; h >>= 1) {
@@ -131,9 +153,9 @@ __device__ double block_ordered_sum(const double *__restrict__ partial, int coun
 }
 
 // blockIdx.x = instance: diag[kDiagWe + k] := sum of the instance's e_partial rows
-__global__ void __launch_bounds__(256) k_reduce_e_b(const InstDev *__restrict__ insts)
+__global__ void __launch_bounds__(kSumBlock) k_reduce_e_b(const InstDev *__restrict__ insts)
 {
-    __shared__ double red[256];
+    __shared__ double red[kSumBlock];
     const InstDev *I = insts + blockIdx.x;
     const int count = efield_blocks_of(I->n[0], I->n[1], I->n[2]);
     for (int k = 0; k < 3; ++k) {
@@ -142,15 +164,24 @@ __global__ void __launch_bounds__(256) k_reduce_e_b(const InstDev *__restrict__ 
     }
 }
 
-// blockIdx.y = instance, blockIdx.x = slot k: diag[kDiagKeOld + k] := sum
-// over the instance's items of ke_partial[item][k] (slots of absent species
-// sum zeros)
-__global__ void __launch_bounds__(256) k_reduce_ke_b(const InstDev *__restrict__ insts)
+// Kinetic-energy slot k of an instance (k < kMaxSpecies: before the push, else
+// after; species k % kMaxSpecies) summed over its items: row k of
+// ke_partial[kKeSlots][nitems], contiguous, so that one block streams it.
+// Slots of absent species are zero (their rows are not written).
+__device__ double ke_slot_sum(const InstDev *I, int k, double *red)
 {
-    __shared__ double red[256];
+    if (k % kMaxSpecies >= I->nspecies) return 0.0;
+    return block_ordered_sum(I->ke_partial + (int64_t)k * I->nitems, I->nitems, 1, 0, red);
+}
+
+// blockIdx.y = instance, blockIdx.x = slot k: diag[kDiagKeOld + k] := sum
+// over the instance's items of its kinetic-energy slot k
+__global__ void __launch_bounds__(kSumBlock) k_reduce_ke_b(const InstDev *__restrict__ insts)
+{
+    __shared__ double red[kSumBlock];
     const InstDev *I = insts + blockIdx.y;
     const int k = blockIdx.x;
-    const double v = block_ordered_sum(I->ke_partial, I->nitems, kKeSlots, k, red);
+    const double v = ke_slot_sum(I, k, red);
     if (threadIdx.x == 0) I->diag[kDiagKeOld + k] = v;
 }
 
@@ -218,15 +249,15 @@ __global__ void k_begin_step_b(const InstDev *__restrict__ insts, const int *__r
 // group: out[i*kFlagSlots + k] := flags[k] of instance i, and the parity flips
 // unless a bin overflowed (the host then redoes that step in list mode from
 // the unchanged buffer).
-__global__ void __launch_bounds__(256) k_end_step_b(const InstDev *__restrict__ insts,
+__global__ void __launch_bounds__(kSumBlock) k_end_step_b(const InstDev *__restrict__ insts,
                                                     int *__restrict__ parity,
                                                     unsigned long long *__restrict__ out)
 {
-    __shared__ double red[256];
+    __shared__ double red[kSumBlock];
     const InstDev *I = insts + blockIdx.y;
     if ((int)blockIdx.x < kKeSlots) {
         const int k = blockIdx.x;
-        const double v = block_ordered_sum(I->ke_partial, I->nitems, kKeSlots, k, red);
+        const double v = ke_slot_sum(I, k, red);
         if (threadIdx.x == 0) I->diag[kDiagKeOld + k] = v;
         return;
     }
@@ -366,7 +397,7 @@ hipError_t launch_efield_b(const InstDev *insts, int ninst, int max_blocks, bool
     if (ninst <= 0) return hipSuccess;
     const int blocks = (int)round_up_xcds(max_blocks > 0 ? max_blocks : 1);
     hipLaunchKernelGGL(k_efield_b, dim3(blocks, ninst), dim3(256), 0, st, insts);
-    if (reduce) hipLaunchKernelGGL(k_reduce_e_b, dim3(ninst), dim3(256), 0, st, insts);
+    if (reduce) hipLaunchKernelGGL(k_reduce_e_b, dim3(ninst), dim3(kSumBlock), 0, st, insts);
     return hipGetLastError();
 }
 
@@ -385,7 +416,7 @@ hipError_t launch_end_step_b(const InstDev *insts, int *parity, int ninst,
                              unsigned long long *out, hipStream_t st)
 {
     if (ninst <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_end_step_b, dim3(kKeSlots + 1, ninst), dim3(256), 0, st, insts,
+    hipLaunchKernelGGL(k_end_step_b, dim3(kKeSlots + 1, ninst), dim3(kSumBlock), 0, st, insts,
                        parity, out);
     return hipGetLastError();
 }
@@ -404,7 +435,7 @@ hipError_t launch_prepare_next_b(const InstDev *insts, const int *parity, int ni
 hipError_t launch_reduce_ke_b(const InstDev *insts, int ninst, hipStream_t st)
 {
     if (ninst <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_reduce_ke_b, dim3(kKeSlots, ninst), dim3(256), 0, st, insts);
+    hipLaunchKernelGGL(k_reduce_ke_b, dim3(kKeSlots, ninst), dim3(kSumBlock), 0, st, insts);
     return hipGetLastError();
 }
 

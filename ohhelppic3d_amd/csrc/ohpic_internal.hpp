@@ -74,9 +74,9 @@ struct InstDev {
     double *phi;
     double *e[3];
     double *diag;                 // kDiagSlots
-    double *ke_partial;           // [items][kKeSlots], reduced in item order
+    double *ke_partial;           // [kKeSlots][nitems], each row reduced by one block
     double *e_partial;            // [kEfieldMaxBlocks][3] field-energy partial sums
-    int nitems;                   // work items of this instance (rows of ke_partial)
+    int nitems;                   // work items of this instance (row length of ke_partial)
     unsigned long long *flags;    // kFlagSlots
 };
 

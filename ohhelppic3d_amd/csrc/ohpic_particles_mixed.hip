@@ -544,8 +544,8 @@ __global__ void __launch_bounds__(kBlock)
                 const double so = block_sum(keo, sRed);
                 const double sn = block_sum(ken, sRed);
                 if (tid == 0) {
-                    gstore(I->ke_partial, (int64_t)it.slot * kKeSlots + s, 0.5 * S.mass * so);
-                    gstore(I->ke_partial, (int64_t)it.slot * kKeSlots + kMaxSpecies + s,
+                    gstore(I->ke_partial, (int64_t)s * I->nitems + it.slot, 0.5 * S.mass * so);
+                    gstore(I->ke_partial, (int64_t)(kMaxSpecies + s) * I->nitems + it.slot,
                            0.5 * S.mass * sn);
                 }
             }
