@@ -1,0 +1,35 @@
+"""CPU checks of bench.py's contract: the refusal of more than one GPU, the
+headline workload, and the CPU-baseline record (the oracle timed on the
+host, a short sample here)."""
+import json
+import os
+import subprocess
+import sys
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_more_than_one_gpu_is_refused():
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"],
+                       capture_output=True, text=True, timeout=120, cwd=ROOT)
+    assert r.returncode == 2
+    assert "one GPU" in r.stderr
+    assert r.stdout.strip() == ""
+
+
+def test_headline_workload_is_256_cubed_64_ppc_fp64():
+    import bench
+    wl = bench.WORKLOADS["uniform256_ppc64_fp64"]
+    assert wl["n"] == (256, 256, 256) and wl["ppc"] == 64 and wl["precision"] == 0
+    assert bench.WORKLOADS["uniform256_ppc64_mixed"]["precision"] == 1
+    assert bench.WORKLOADS["uniform32_ppc16_fp64"]["n"] == (32, 32, 32)
+
+
+def test_cpu_baseline_record(orc):
+    import bench
+    rec = bench.run_cpu_baseline(0.2)
+    json.dumps(rec)
+    assert rec["kind"] == "port" and rec["unit"] == "particle-steps/s"
+    assert rec["value"] > 0 and rec["single_thread_value"] > 0
+    assert rec["cores"] >= 1 and rec["ranks"] == 1
+    assert "32^3" in rec["sample"]
