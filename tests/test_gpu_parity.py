@@ -115,8 +115,10 @@ def test_deposit_matches_oracle(ohp, orc, wo):
 
 
 # ----------------------------------------------------------------- full step
-def _compare_step(ohp, orc, n, ppc, nsteps, tol_x, tol_v, wo=0, tile=(0, 0, 0), b0=(0, 0, 0.5)):
-    sp = two_species(orc, n, ppc)
+def _compare_step(ohp, orc, n, ppc, nsteps, tol_x, tol_v, wo=0, tile=(0, 0, 0), b0=(0, 0, 0.5),
+                  sp=None):
+    if sp is None:
+        sp = two_species(orc, n, ppc)
     osim = orc.OracleSim(n, sp, 0.1, b0, wo)
     with sim_for(sp, n, weight_order=wo, tile=tile, b0=b0) as sim:
         sim.step(nsteps)
@@ -318,6 +320,24 @@ def test_hump_loader_matches_oracle(ohp, orc):
                                                     vt)
         assert np.array_equal(g[s]["x"], x) and np.array_equal(g[s]["z"], z)
         assert rel_l2(g[s]["vy"], vy) <= 1e-14
+
+
+def test_two_steps_from_hump_load_match_oracle(ohp, orc):
+    # the Gaussian-hump density n(r) = n0 (1 + 9 exp(-|r-c|^2 / (2 sigma^2)))
+    # at 24^3 with sigma 4 (cells hold ~1x to 10x the edge density): the
+    # imbalanced bins against the oracle, at the FP64 bars
+    n = (24, 24, 24)
+    ppc = 8
+    npcl = ppc * n[0] * n[1] * n[2]
+    sp = []
+    for s, (qm, vt) in enumerate(((-1.0, 1.0), (0.01, 0.1))):
+        rng0 = orc.Pcg32(42, 52)
+        x, y, z, vx, vy, vz = orc.load_hump_maxwell(rng0, s << 40, 2052, npcl, n, 9.0, 4.0, vt)
+        sp.append(dict(qm=qm, q=(1.0 if qm > 0 else -1.0) / ppc, x=x, y=y, z=z, vx=vx, vy=vy,
+                       vz=vz))
+    counts = orc.cell_counts(n, sp[0]["x"], sp[0]["y"], sp[0]["z"])
+    assert counts.max() >= 5 * max(1, counts.min())
+    _compare_step(ohp, orc, n, ppc, 2, TOL, TOL, sp=sp)
 
 
 def test_errors_are_reported(ohp, orc):
