@@ -177,3 +177,26 @@ def test_mixed_two_steps_standard_weights_within_1e5(ohp, orc):
           f"Ex {rel_l2(ex, osim.ex):.2e}")
     assert rel_l2(rho, osim.rho) <= TOL_MIXED
     assert rel_l2(ex, osim.ex) <= TOL_MIXED
+
+
+def test_mixed_from_hump_load_within_1e5(ohp, orc):
+    # the Gaussian-hump density (cells holding ~1x to 10x the edge density):
+    # two mixed steps against the FP64 oracle, standard weights
+    n = (24, 24, 24)
+    ppc = 8
+    npcl = ppc * n[0] * n[1] * n[2]
+    sp = []
+    for s, (qm, vt) in enumerate(((-1.0, 1.0), (0.01, 0.1))):
+        rng0 = orc.Pcg32(42, 52)
+        x, y, z, vx, vy, vz = orc.load_hump_maxwell(rng0, s << 40, 2052, npcl, n, 9.0, 4.0, vt)
+        sp.append(dict(qm=qm, q=(1.0 if qm > 0 else -1.0) / ppc, x=x, y=y, z=z, vx=vx, vy=vy,
+                       vz=vz))
+    osim = orc.OracleSim(n, sp, 0.1, (0, 0, 0.5), 1)
+    osim.step(2)
+    with sim_for(sp, n, precision=ohp.MIXED, weight_order=1) as sim:
+        sim.step(2)
+        rho = sim.grid(ohp.FIELD_RHO)
+        e = [sim.grid(w) for w in (ohp.FIELD_EX, ohp.FIELD_EY, ohp.FIELD_EZ)]
+    assert rel_l2(rho, osim.rho) <= TOL_MIXED
+    for a, b in zip(e, (osim.ex, osim.ey, osim.ez)):
+        assert rel_l2(a, b) <= TOL_MIXED
