@@ -322,6 +322,35 @@ def test_hump_loader_matches_oracle(ohp, orc):
         assert rel_l2(g[s]["vy"], vy) <= 1e-14
 
 
+def test_three_species_match_oracle(ohp, orc):
+    # electrons, ions and a second, heavier ion species (per-species bins and
+    # kinetic-energy slots beyond two)
+    n = (16, 12, 8)
+    ppc = 6
+    g = n[0] * n[1] * n[2]
+    rng = orc.Pcg32(42, 52)
+    sp = []
+    for qm, q, vt in ((-1.0, -2.0, 1.0), (0.01, 1.0, 0.1), (0.005, 1.0, 0.07)):
+        x, y, z, vx, vy, vz = orc.load_uniform_maxwell(rng, ppc * g, n, vt)
+        sp.append(dict(qm=qm, q=q / ppc, x=x, y=y, z=z, vx=vx, vy=vy, vz=vz))
+    osim = orc.OracleSim(n, sp, 0.1, (0, 0, 0.5), 0)
+    with sim_for(sp, n) as sim:
+        sim.step(2)
+        osim.step(2)
+        rho, phi = sim.grid(ohp.FIELD_RHO), sim.grid(ohp.FIELD_PHI)
+        d = sim.diagnostics()
+        parts = [by_pid(sim.store_particles(s, True)) for s in range(3)]
+        counts = sim.cell_counts()
+    assert rel_l2(rho, osim.rho) <= TOL and rel_l2(phi, osim.phi) <= TOL
+    ref_counts = np.zeros(g, np.int32)
+    for s in range(3):
+        for k in ("x", "y", "z", "vx", "vy", "vz"):
+            assert rel_l2(parts[s][k], osim.sp[s][k]) <= TOL, (s, k)
+        ref_counts += orc.cell_counts(n, osim.sp[s]["x"], osim.sp[s]["y"], osim.sp[s]["z"])
+    assert np.array_equal(counts, ref_counts)
+    assert np.allclose(d["ke_new"][:3], osim.ke_new[:3], rtol=1e-12, atol=0)
+
+
 def test_two_steps_from_hump_load_match_oracle(ohp, orc):
     # the Gaussian-hump density n(r) = n0 (1 + 9 exp(-|r-c|^2 / (2 sigma^2)))
     # at 24^3 with sigma 4 (cells hold ~1x to 10x the edge density): the
