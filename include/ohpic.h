@@ -189,6 +189,10 @@ int ohpic_wrap_batch(int device, int64_t n, double *p, int32_t nn, int applicati
 
 /* --- diagnostics ------------------------------------------------------------*/
 int ohpic_diagnostics(ohpic_ctx *ctx, ohpic_diag *out);
+/* The energy history of one step in four numbers: out[0] kinetic energy
+ * (time-centred, all species), out[1] field energy 1/2 sum rho*phi, out[2]
+ * their sum, out[3] net charge sum rho (as ohpic_diagnostics reports them). */
+int ohpic_energy(ohpic_ctx *ctx, double out[4]);
 /* per-cell particle counts (cell ix + nx*(iy + ny*iz)); ispec < 0 = all */
 int ohpic_cell_counts(ohpic_ctx *ctx, int ispec, int32_t *counts);
 /* Kernel timing (HIP events on the context stream).  enable != 0 starts

@@ -29,7 +29,7 @@ EXPORTS = (
     "ohpic_seed_uniform_maxwell", "ohpic_seed_hump_maxwell", "ohpic_deposit",
     "ohpic_field_solve", "ohpic_gather_push", "ohpic_boundary", "ohpic_step",
     "ohpic_synchronize", "ohpic_get_field", "ohpic_put_field", "ohpic_fft3d_r2r",
-    "ohpic_diagnostics", "ohpic_cell_counts", "ohpic_enable_timing", "ohpic_kernel_times",
+    "ohpic_diagnostics", "ohpic_energy", "ohpic_cell_counts", "ohpic_enable_timing", "ohpic_kernel_times",
     "ohpic_memory_footprint", "ohpic_estimate_footprint", "ohpic_boris_batch",
     "ohpic_interp_batch", "ohpic_scatter_batch", "ohpic_wrap_batch",
     "ohpic_group_create", "ohpic_group_destroy", "ohpic_group_attach", "ohpic_group_detach",
@@ -145,6 +145,7 @@ def lib():
         "ohpic_put_field": [vp, C.c_int, dp, C.c_int, i32p, i32p, C.c_int],
         "ohpic_fft3d_r2r": [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, dp, dp],
         "ohpic_diagnostics": [vp, C.POINTER(Diag)],
+        "ohpic_energy": [vp, dp],
         "ohpic_cell_counts": [vp, C.c_int, i32p],
         "ohpic_enable_timing": [vp, C.c_int],
         "ohpic_kernel_times": [vp, C.c_int, vp, dp, i64p, C.POINTER(C.c_int)],

@@ -1890,6 +1890,19 @@ int ohpic_diagnostics(ohpic_ctx *ctx, ohpic_diag *out)
     return OHPIC_OK;
 }
 
+int ohpic_energy(ohpic_ctx *ctx, double out[4])
+{
+    if (!ctx || !out) return fail(ctx, OHPIC_EINVAL, "NULL argument");
+    ohpic_diag d;
+    const int rc = ohpic_diagnostics(ctx, &d);
+    if (rc) return rc;
+    out[0] = d.total_energy - d.field_energy;
+    out[1] = d.field_energy;
+    out[2] = d.total_energy;
+    out[3] = d.net_charge;
+    return OHPIC_OK;
+}
+
 int ohpic_cell_counts(ohpic_ctx *ctx, int ispec, int32_t *counts)
 {
     if (!ctx || !counts) return fail(ctx, OHPIC_EINVAL, "NULL argument");

@@ -10,6 +10,7 @@ module m_ohpic_c
     public :: ohpic_seed_uniform_maxwell, ohpic_deposit, ohpic_field_solve
     public :: ohpic_gather_push, ohpic_boundary, ohpic_step, ohpic_synchronize
     public :: ohpic_get_field, ohpic_put_field, ohpic_fft3d_r2r, ohpic_diagnostics
+    public :: ohpic_energy
     public :: ohpic_cell_counts, ohpic_boris_batch, ohpic_interp_batch
     public :: ohpic_scatter_batch, ohpic_wrap_batch
     public :: ohpic_snapshot_open, ohpic_snapshot_write, ohpic_snapshot_write_host
@@ -168,6 +169,13 @@ module m_ohpic_c
             import :: c_int, c_ptr, ohpic_diag
             type(c_ptr), value :: ctx
             type(ohpic_diag), intent(out) :: diag
+        end function
+
+        ! [kinetic, field, total, net charge] of the last step
+        integer(c_int) function ohpic_energy(ctx, out) bind(c, name='ohpic_energy')
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: ctx
+            real(c_double), intent(out) :: out(4)
         end function
 
         integer(c_int) function ohpic_cell_counts(ctx, ispec, counts) &

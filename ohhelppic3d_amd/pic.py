@@ -226,6 +226,12 @@ class Simulation:
                     net_charge=d.net_charge, total_energy=d.total_energy, repacks=d.repacks,
                     fast_particles=d.fast_particles)
 
+    def energy(self) -> np.ndarray:
+        """[kinetic, field, total, net charge] of the last step (ohpic_energy)."""
+        out = np.zeros(4)
+        self._chk(self._lib.ohpic_energy(self._ctx, L.dptr(out)))
+        return out
+
     def cell_counts(self, ispec: int = -1) -> np.ndarray:
         c = np.zeros(self.n[0] * self.n[1] * self.n[2], np.int32)
         self._chk(self._lib.ohpic_cell_counts(self._ctx, ispec, L.i32ptr(c)))

@@ -72,6 +72,8 @@ def test_errors_without_gpu_or_with_bad_config(L):
     assert b"nspecies" in lib.ohpic_last_error(None)
     assert lib.ohpic_create(None, C.byref(ctx)) == L.EINVAL
     assert lib.ohpic_config_default(None) == L.EINVAL
+    out = (C.c_double * 4)()
+    assert lib.ohpic_energy(None, out) == L.EINVAL
 
 
 def test_host_mirror_validation(L):

@@ -322,6 +322,21 @@ def test_hump_loader_matches_oracle(ohp, orc):
         assert rel_l2(g[s]["vy"], vy) <= 1e-14
 
 
+def test_energy_matches_diagnostics_and_oracle(ohp, orc):
+    n = (16, 16, 16)
+    sp = two_species(orc, n, 8)
+    osim = orc.OracleSim(n, sp, 0.1, (0, 0, 0.5), 0)
+    with sim_for(sp, n) as sim:
+        sim.step(2)
+        osim.step(2)
+        e = sim.energy()
+        d = sim.diagnostics()
+    assert e[1] == d["field_energy"] and e[2] == d["total_energy"] and e[3] == d["net_charge"]
+    ke = 0.5 * (osim.ke_old.sum() + osim.ke_new.sum())
+    assert abs(e[0] - ke) <= 1e-12 * ke
+    assert abs(e[1] - osim.field_energy) <= 1e-10 * abs(osim.field_energy)
+
+
 def test_three_species_match_oracle(ohp, orc):
     # electrons, ions and a second, heavier ion species (per-species bins and
     # kinetic-energy slots beyond two)
