@@ -81,6 +81,36 @@ def test_field_solve_matches_oracle(ohp, orc, n):
     assert abs(d["net_charge"] - rho.sum()) <= 1e-10
 
 
+def test_field_solve_256_cubed_properties(ohp):
+    # the headline grid, through properties that hold at any size: the 7-point
+    # Laplacian of phi gives back rho minus its mean, phi has zero mean, E is
+    # the node-centred difference of phi, and the field energy is 1/2 rho.phi
+    n = (256, 256, 256)
+    rng = np.random.default_rng(5)
+    rho = rng.standard_normal(n[0] * n[1] * n[2])
+    cfg = ohp.SimConfig(n=n, qm=[-1.0], q_macro=[-1.0], capacity=[0])
+    with ohp.Simulation(cfg) as sim:
+        sim.put_grid(ohp.FIELD_RHO, rho)
+        sim.field_solve()
+        phi = sim.grid(ohp.FIELD_PHI)
+        ex = sim.grid(ohp.FIELD_EX)
+        ez = sim.grid(ohp.FIELD_EZ)
+        d = sim.diagnostics()
+    p = phi.reshape(n[2], n[1], n[0])
+    lap = -6.0 * p
+    for ax in range(3):
+        lap += np.roll(p, 1, axis=ax) + np.roll(p, -1, axis=ax)
+    scale = np.max(np.abs(phi))
+    assert np.max(np.abs(-lap.reshape(-1) - (rho - rho.mean()))) <= 1e-12 * scale
+    assert abs(phi.mean()) <= 1e-12 * scale
+    ex_ref = 0.5 * ((np.roll(p, 1, axis=2) - p) + (p - np.roll(p, -1, axis=2)))
+    ez_ref = 0.5 * ((np.roll(p, 1, axis=0) - p) + (p - np.roll(p, -1, axis=0)))
+    assert np.max(np.abs(ex - ex_ref.reshape(-1))) <= 1e-15 * np.max(np.abs(ex_ref))
+    assert np.max(np.abs(ez - ez_ref.reshape(-1))) <= 1e-15 * np.max(np.abs(ez_ref))
+    we = 0.5 * rho @ phi
+    assert abs(d["field_energy"] - we) <= 1e-12 * abs(we)
+
+
 def test_field_get_with_halo_is_periodic(ohp):
     n = (6, 5, 4)
     rng = np.random.default_rng(3)
