@@ -57,6 +57,7 @@ module m_ohpic
         procedure :: get_field => sim_get_field
         procedure :: put_field => sim_put_field
         procedure :: diagnostics => sim_diagnostics
+        procedure :: energy => sim_energy
     end type
 
     type, extends(t_ParticleMover) :: t_ParticleMoverHip
@@ -271,6 +272,13 @@ contains
         type(ohpic_diag), intent(out) :: diag
         call ohpic_check(ohpic_diagnostics(self%ctx, diag), self%ctx)
     end subroutine
+
+    ! [kinetic, field, total, net charge] of the last step
+    function sim_energy(self) result(e)
+        class(t_OhpicSimulation), intent(inout) :: self
+        real(c_double) :: e(4)
+        call ohpic_check(ohpic_energy(self%ctx, e), self%ctx)
+    end function
 
     ! ---------------------------------------------------------------- mover
     function create_particle_mover_hip(name, device) result(mover)

@@ -48,6 +48,7 @@ program ohpic_fortran_check
     class(t_MPIFFTSolver3d), allocatable :: fft
     double precision, allocatable :: ebi(:, :)
     double precision :: impulse(1, 1, 63), spectrum(1, 1, 63)
+    double precision :: en(4)
 
     call get_command_argument(1, inpath)
     call get_command_argument(2, outpath)
@@ -125,6 +126,10 @@ program ohpic_fortran_check
     call sim2%init(n, qm, q, np, dt, b0, track_ids=.true.)
     call sim2%load(parts)
     call sim2%step(nsteps)
+    ! energy history entry of the last step: kinetic + field = total
+    en = sim2%energy()
+    if (.not. (en(1) > 0d0 .and. abs(en(1) + en(2) - en(3)) <= 1d-12 * abs(en(3)))) &
+        error stop 'ohpic_energy: inconsistent kinetic, field and total energy'
 
     ! ---- part 5: snapshots in the reference's phisp layout --------------------
     if (len_trim(h5path) > 0) then
