@@ -30,15 +30,34 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+W_REF, W_STD = 0, 1  # CIC weight order: the reference's mirrored one / standard
+
 WORKLOADS = {
-    # name: grid, mean ppc per species, precision, density hump amplitude
-    "uniform256_ppc64_fp64": dict(n=(256, 256, 256), ppc=64, precision=0, hump=0.0),
-    "uniform256_ppc64_mixed": dict(n=(256, 256, 256), ppc=64, precision=1, hump=0.0),
-    "hump256_ppc64_mixed": dict(n=(256, 256, 256), ppc=64, precision=1, hump=9.0),
-    "uniform128_ppc64_fp64": dict(n=(128, 128, 128), ppc=64, precision=0, hump=0.0),
-    "uniform128_ppc64_mixed": dict(n=(128, 128, 128), ppc=64, precision=1, hump=0.0),
-    "uniform32_ppc16_fp64": dict(n=(32, 32, 32), ppc=16, precision=0, hump=0.0),
+    # name: grid, mean ppc per species, precision, density hump amplitude,
+    # CIC weight order, and the FP64 run with the same load and weights that
+    # a mixed run is compared with
+    "uniform256_ppc64_fp64": dict(n=(256, 256, 256), ppc=64, precision=0, hump=0.0, wo=W_REF),
+    "uniform256_ppc64_fp64_stdw": dict(n=(256, 256, 256), ppc=64, precision=0, hump=0.0,
+                                       wo=W_STD),
+    "uniform256_ppc64_mixed_stdw": dict(n=(256, 256, 256), ppc=64, precision=1, hump=0.0,
+                                        wo=W_STD, fp64="uniform256_ppc64_fp64_stdw"),
+    "uniform256_ppc64_mixed": dict(n=(256, 256, 256), ppc=64, precision=1, hump=0.0, wo=W_REF,
+                                   fp64="uniform256_ppc64_fp64"),
+    "hump256_ppc64_fp64_stdw": dict(n=(256, 256, 256), ppc=64, precision=0, hump=9.0, wo=W_STD),
+    "hump256_ppc64_mixed_stdw": dict(n=(256, 256, 256), ppc=64, precision=1, hump=9.0,
+                                     wo=W_STD, fp64="hump256_ppc64_fp64_stdw"),
+    "uniform128_ppc64_fp64": dict(n=(128, 128, 128), ppc=64, precision=0, hump=0.0, wo=W_REF),
+    "uniform128_ppc64_mixed": dict(n=(128, 128, 128), ppc=64, precision=1, hump=0.0, wo=W_REF),
+    "uniform32_ppc16_fp64": dict(n=(32, 32, 32), ppc=16, precision=0, hump=0.0, wo=W_REF),
 }
+
+# (mixed workload, its FP64 run) pairs of the default line, in run order; the
+# first pair's FP64 run is the headline
+MIXED_PAIRS = (("uniform256_ppc64_mixed_stdw", "uniform256_ppc64_fp64_stdw"),
+               ("hump256_ppc64_mixed_stdw", "hump256_ppc64_fp64_stdw"),
+               ("uniform256_ppc64_mixed", "uniform256_ppc64_fp64"))
+# rho/E bar of the mixed form against FP64 from the same seeded state
+MIXED_TOL = 1e-5
 
 
 def make_sim(ohp, wl, args, track_ids=False):
@@ -47,8 +66,8 @@ def make_sim(ohp, wl, args, track_ids=False):
     npcl = ppc * n[0] * n[1] * n[2]
     cfg = ohp.SimConfig(n=n, qm=[-1.0, 0.01], q_macro=[-1.0 / ppc, 1.0 / ppc],
                         capacity=[npcl, npcl], dt=0.1, b0=(0.0, 0.0, 0.5),
-                        precision=wl["precision"], slack=args.slack, track_ids=track_ids,
-                        tile=tuple(args.tile), chunks=args.chunks)
+                        precision=wl["precision"], weight_order=wl["wo"], slack=args.slack,
+                        track_ids=track_ids, tile=tuple(args.tile), chunks=args.chunks)
     sim = ohp.Simulation(cfg)
     if wl["hump"] > 0:
         # n(r) = n0 (1 + 9 exp(-|r-c|^2 / (2 16^2))), mean ppc over the box
@@ -59,8 +78,12 @@ def make_sim(ohp, wl, args, track_ids=False):
     return sim, cfg, npcl
 
 
-def run_gpu(args, workload=None, timing=True):
+def run_gpu(args, workload=None, timing=True, record=0):
+    """Create and seed one workload, take the warm-up steps (recording rho,
+    phi and E after each of the first `record` of them, from the seeded
+    state), then time `args.steps` single steps."""
     import ohhelppic3d_amd as ohp
+    from ohhelppic3d_amd import ensemble
 
     workload = workload or args.workload
     wl = WORKLOADS[workload]
@@ -73,7 +96,9 @@ def run_gpu(args, workload=None, timing=True):
         f"est. footprint {ohp.estimate_footprint(cfg) / 1e9:.1f} GB")
     log(f"[bench] created + loaded in {time.time() - t0:.1f} s, tile {sim.tile}, "
         f"device bytes {sim.memory_footprint() / 1e9:.1f} GB")
-    for i in range(args.warmup):
+    record = min(record, args.warmup)
+    fields = ensemble.record_fields(sim, record) if record > 0 else None
+    for i in range(record, args.warmup):
         sim.step(1)
         log(f"[bench] warmup step {i + 1}/{args.warmup} done")
     sim.synchronize()
@@ -128,9 +153,11 @@ def run_gpu(args, workload=None, timing=True):
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64" if wl["precision"] == 0 else "f32+f64",
-        "data": "synthetic (device-seeded PCG32 uniform Maxwellian)",
+        "data": "synthetic (device-seeded PCG32 " +
+                ("Gaussian-hump" if wl["hump"] > 0 else "uniform") + " Maxwellian)",
         "config": {"workload": workload, "grid": list(n), "ppc_per_species": ppc,
                    "species": 2, "particles": total_p, "dt": 0.1, "b0z": 0.5,
+                   "weights": "reference (mirrored)" if wl["wo"] == W_REF else "standard",
                    "tile": tile, "slack": args.slack},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
@@ -142,47 +169,69 @@ def run_gpu(args, workload=None, timing=True):
                    "drift_over_timed_steps": (abs(diag["total_energy"] - e_start) / abs(e_start)
                                               if e_start else None)},
     }
+    if fields is not None:
+        res["_fields"] = fields
     return res
 
 
-def mixed_error_vs_fp64(args):
-    """FP64 and mixed runs from the same seeded state at 128^3 x 64 ppc:
-    relative L2 of rho, phi and E after the first solve (the loaded state)
-    and after the second (one push later), for both CIC weight orders."""
-    import ohhelppic3d_amd as ohp
+def run_mixed_pairs(args, headline):
+    """Each mixed workload beside the FP64 run with the same load and CIC
+    weights: throughput ratio, and rho/phi/E relative L2 against that FP64
+    run after each of the first warm-up steps from the same seeded state
+    (step1 = the loaded state's fields, later steps one push each)."""
+    from ohhelppic3d_amd import ensemble
 
-    out = {"setup": "128^3 x 64 ppc x 2 species, seed 42/52; step0 = fields of the "
-                    "loaded state, step1 = fields after one push"}
-    for wname, wo in (("reference_weights", ohp.WEIGHTS_REFERENCE),
-                      ("standard_weights", ohp.WEIGHTS_STANDARD)):
-        grids = {}
-        for prec in ("fp64", "mixed"):
-            wl = WORKLOADS[f"uniform128_ppc64_{prec}"]
-            n, ppc = wl["n"], wl["ppc"]
-            npcl = ppc * n[0] * n[1] * n[2]
-            cfg = ohp.SimConfig(n=n, qm=[-1.0, 0.01], q_macro=[-1.0 / ppc, 1.0 / ppc],
-                                capacity=[npcl, npcl], precision=wl["precision"],
-                                slack=args.slack, weight_order=wo, tile=tuple(args.tile))
-            sim = ohp.Simulation(cfg)
-            sim.seed_uniform_maxwell([npcl, npcl], [1.0, 0.1], seed=(42, 52))
-            res = []
-            for _ in range(2):
-                sim.step(1)
-                res.append([sim.grid(w) for w in (ohp.FIELD_RHO, ohp.FIELD_PHI, ohp.FIELD_EX,
-                                                  ohp.FIELD_EY, ohp.FIELD_EZ)])
-            sim.close()
-            grids[prec] = res
-        out[wname] = {}
-        for st in range(2):
-            e = {}
-            for k, a, b in zip(("rho", "phi", "Ex", "Ey", "Ez"), grids["mixed"][st],
-                               grids["fp64"][st]):
-                e[k] = float(np.linalg.norm(a - b) / np.linalg.norm(b))
-            out[wname][f"step{st}"] = e
+    rec = min(4, args.warmup)
+    fp64_runs = {"uniform256_ppc64_fp64": headline}
+    out = {}
+    for mname, fname in MIXED_PAIRS:
+        if fname not in fp64_runs:
+            log(f"[bench] FP64 workload {fname}")
+            fp64_runs[fname] = run_gpu(args, fname, record=rec)
+        f = fp64_runs[fname]
+        log(f"[bench] mixed-precision workload {mname}")
+        r = run_gpu(args, mname, record=rec)
+        entry = {"value": r["value"], "ms_per_step": r["ms_per_step"],
+                 "weights": r["config"]["weights"], "load": r["data"],
+                 "fp64_run": {"workload": fname, "value": f["value"],
+                              "ms_per_step": f["ms_per_step"],
+                              "push_avg_ms": f["roofline"]["kernel_avg_ms"]},
+                 "vs_fp64": r["value"] / f["value"], "roofline": r["roofline"],
+                 "energy": r["energy"]}
+        if "_fields" in r and "_fields" in f:
+            err = ensemble.field_errors(r.pop("_fields"), f["_fields"])
+            worst = err["max_rho_E_after_push"]
+            err["bar"] = MIXED_TOL
+            err["meets_bar"] = worst is not None and worst <= MIXED_TOL
+            entry["error_vs_fp64"] = err
+            log(f"[bench] {mname} vs {fname}: ratio {entry['vs_fp64']:.3f}, worst rho/E "
+                f"rel L2 after a push {worst:.2e}")
+        out[mname] = entry
+        # an FP64 run is kept (host fields) only until its last mixed partner
+        if all(p[1] != fname for p in MIXED_PAIRS[MIXED_PAIRS.index((mname, fname)) + 1:]):
+            fp64_runs[fname].pop("_fields", None)
+    for r in fp64_runs.values():
+        r.pop("_fields", None)
+    # the mixed headline: the fastest entry whose fields meet the bar
+    ok = [k for k, v in out.items() if v.get("error_vs_fp64", {}).get("meets_bar")]
+    out["headline"] = max(ok, key=lambda k: out[k]["vs_fp64"]) if ok else None
+    out["fp64_same_weights"] = {k: {"value": v["value"], "ms_per_step": v["ms_per_step"],
+                                    "roofline": v["roofline"]}
+                                for k, v in fp64_runs.items() if k != "uniform256_ppc64_fp64"}
     return out
 
 
 def _cpu_model() -> str:
+    """The host CPU's model name as lscpu prints it (/proc/cpuinfo without
+    lscpu)."""
+    import subprocess
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            if line.strip().startswith("Model name:"):
+                return line.split(":", 1)[1].strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
     try:
         with open("/proc/cpuinfo", "r") as f:
             for line in f:
@@ -193,11 +242,27 @@ def _cpu_model() -> str:
     return "unknown"
 
 
+def _cpu_share() -> dict:
+    """CPUs the host reports, the ones this process may run on, and the
+    cgroup CPU quota (cores' worth; None without a quota)."""
+    quota = None
+    try:
+        with open("/sys/fs/cgroup/cpu.max") as f:
+            q, per = f.read().split()[:2]
+            if q != "max":
+                quota = int(q) / int(per)
+    except (OSError, ValueError):
+        pass
+    return {"nproc": os.cpu_count(), "affinity": len(os.sched_getaffinity(0)),
+            "cgroup_cpus": quota, "omp_num_threads": os.environ.get("OMP_NUM_THREADS")}
+
+
 def run_cpu_baseline(seconds: float):
     """The FP64 oracle (a CPU restatement of the reference step, gcc -O2
     -fopenmp) on the host: 32^3 grid, 16 ppc per species (2^20 particles), one
-    MPI-free rank, timed on one thread and on the OpenMP default thread count
-    (per-thread charge arrays for the deposit).  ``value`` is the threaded
+    MPI-free rank, timed on one thread and on every CPU of the process's
+    share (the OpenMP default: OMP_NUM_THREADS, else the affinity mask;
+    per-thread charge arrays for the deposit).  ``value`` is the threaded
     rate; the one-thread rate is reported beside it."""
     import oracle as orc
 
@@ -230,8 +295,9 @@ def run_cpu_baseline(seconds: float):
 
     medt, kt = timed(threaded, 0.4 * seconds)
     return {"value": npart / medt, "unit": "particle-steps/s", "cores": used[0],
-            "kind": "port", "single_thread_value": npart / med1,
-            "cpu_model": _cpu_model(), "ranks": 1, "compiler": "gcc -O2 -fopenmp",
+            "threads": used[0], "kind": "port", "single_thread_value": npart / med1,
+            "cpu_model": _cpu_model(), "host": _cpu_share(), "ranks": 1,
+            "compiler": "gcc -O2 -fopenmp",
             "sample": f"FP64 oracle step, 32^3 grid x 16 ppc x 2 species ({npart} particles): "
                       f"{used[0]} threads, {kt} steps, median {medt * 1e3:.1f} ms; "
                       f"1 thread, {k1} steps, median {med1 * 1e3:.1f} ms"}
@@ -240,9 +306,12 @@ def run_cpu_baseline(seconds: float):
 def run_ensemble_bench(args):
     """128 independent mixed-precision instances (each axis from
     {16,24,32,48,64}, 8..64 ppc per species, v_th,e in [0.5,2], omega_ce in
-    [0,1], dt 0.1) advanced as one group: autotuned tile/chunks, one launch
-    per kernel and step replayed from a hipGraph.  Also times the same
-    instances stepped one after another for the batching gain."""
+    [0,1], dt 0.1, standard CIC weights) advanced as one group: autotuned
+    tile/chunks, one launch per kernel and step replayed from a hipGraph.
+    Also times the same instances stepped one after another for the
+    batching gain, and states the error of three instances against their
+    FP64 runs from the same seeded state (a member's fields equal its run
+    alone bit for bit)."""
     from ohhelppic3d_amd import ensemble
 
     specs = ensemble.sample_ensemble(args.ensemble_instances, seed=2024)
@@ -254,18 +323,35 @@ def run_ensemble_bench(args):
     seq = ensemble.run_sequential(specs, steps=args.ensemble_seq_steps, tune=tune)
     res["sequential"] = seq
     res["batching_gain"] = res["value"] / seq["value"]
+    # the largest instance, one with a radix-3 axis and the smallest
+    by_size = sorted(specs, key=lambda s: s.cells)
+    picks = [by_size[-1], next((s for s in specs if 24 in s.n or 48 in s.n), by_size[0]),
+             by_size[0]]
+    errs = []
+    for s in picks:
+        e = ensemble.error_vs_fp64(s, 4, tune["tile"], tune["chunks"])
+        errs.append(e)
+        log(f"[ensemble] instance {s.n} x {s.ppc} ppc vs FP64: worst rho/E rel L2 after a "
+            f"push {e['max_rho_E_after_push']:.2e}")
+    worst = max(e["max_rho_E_after_push"] for e in errs)
+    res["error_vs_fp64"] = {"instances": errs, "max_rho_E_after_push": worst,
+                            "bar": MIXED_TOL, "meets_bar": worst <= MIXED_TOL}
     res["config"] = {"workload": f"ensemble{len(specs)}_mixed", "seed": 2024,
-                     "axes": list(ensemble.AXES), "ppc_range": [8, 64], "dt": 0.1}
+                     "axes": list(ensemble.AXES), "ppc_range": [8, 64], "dt": 0.1,
+                     "weights": "standard"}
     return res
 
 
-def run_serving_bench(args):
+def run_serving_bench(args, levels=(0.5, 0.8, 1.0)):
     """Serving loop: seeded requests (grids 32^3/64^3/96^3/128^3 weighted
-    4:3:2:1, 16..64 ppc per species, 200..2000 steps, mixed precision)
-    admitted against a 240 GB budget.  Capacity is measured with every
-    request arriving at once; then Poisson arrivals at 50/80/100% of it."""
+    4:3:2:1, 16..64 ppc per species, 200..2000 steps, mixed precision,
+    standard CIC weights) admitted against a 240 GB budget.  Capacity is
+    measured with every request arriving at once; then Poisson arrivals at
+    the given fractions of it.  One served run is also re-run alone (its
+    fields must equal the served ones bit for bit) and against its FP64 run
+    from the same seeded state."""
     import ohhelppic3d_amd as ohp
-    from ohhelppic3d_amd import serve
+    from ohhelppic3d_amd import ensemble, serve
 
     n = args.serve_requests
     probe = serve.sample_requests(n, rate=float("inf"), seed=7)
@@ -275,19 +361,43 @@ def run_serving_bench(args):
     log(f"[serve] autotune: tile {tune['tile']}, chunks {tune['chunks']} "
         f"({time.perf_counter() - t:.1f} s)")
     loop = serve.ServingLoop(tile=tune["tile"], chunks=tune["chunks"], creators=args.serve_creators)
+    # the served run checked against its run alone: the 64^3 request with the
+    # fewest steps
+    cand = [r for r in probe if tuple(r.n) == (64, 64, 64)] or list(probe)
+    watch = min(cand, key=lambda r: r.steps)
+    served = {}
+
+    def keep(req, sim):
+        if req.rid == watch.rid:
+            served["rho"] = sim.grid(ohp.FIELD_RHO)
+            served["phi"] = sim.grid(ohp.FIELD_PHI)
+
     log(f"[serve] capacity run: {n} requests at t=0")
-    rep = loop.run(probe, progress=log)
+    rep = loop.run(probe, on_complete=keep, progress=log)
     cap = rep.summary()
     cap_rps = cap["done"] / cap["makespan"]
     log(f"[serve] capacity {cap_rps:.3f} requests/s, "
         f"{cap['particle_steps_per_s']:.3e} particle-steps/s")
-    levels = {"saturated": cap}
-    for k, load in enumerate((0.5, 0.8, 1.0)):
+    out_levels = {"saturated": cap}
+    for k, load in enumerate(levels):
         reqs = serve.sample_requests(n, rate=load * cap_rps, seed=8 + k)
         log(f"[serve] offered load {int(load * 100)}%: {n} requests at "
             f"{load * cap_rps:.3f}/s")
-        levels[f"load{int(load * 100)}"] = loop.run(reqs, progress=log).summary()
-    top = levels["load100"]
+        out_levels[f"load{int(load * 100)}"] = loop.run(reqs, progress=log).summary()
+    # the watched request alone, same tile and weights: equal bit for bit
+    with watch.spec.create(loop.tile, loop.chunks, loop.precision,
+                           weight_order=loop.weight_order) as sim:
+        sim.step(watch.steps)
+        alone = sim.grid(ohp.FIELD_RHO), sim.grid(ohp.FIELD_PHI)
+    same = ("rho" in served and np.array_equal(alone[0], served["rho"])
+            and np.array_equal(alone[1], served["phi"]))
+    log(f"[serve] request {watch.rid} ({watch.n}, {watch.steps} steps): served == alone: {same}")
+    err = ensemble.error_vs_fp64(watch.spec, 4, loop.tile, loop.chunks, loop.weight_order)
+    err["bar"] = MIXED_TOL
+    err["meets_bar"] = err["max_rho_E_after_push"] <= MIXED_TOL
+    log(f"[serve] request {watch.rid} vs FP64: worst rho/E rel L2 after a push "
+        f"{err['max_rho_E_after_push']:.2e}")
+    top = out_levels[f"load{int(levels[-1] * 100)}"]
     return {
         "metric": "serving loop: p50/p99 time-to-completion and aggregate particle-steps/s",
         "value": top["particle_steps_per_s"], "unit": "particle-steps/s", "n_gpus": 1,
@@ -296,8 +406,11 @@ def run_serving_bench(args):
                    "grid_weights": list(serve.GRID_WEIGHTS), "ppc_range": [16, 64],
                    "steps_range": [200, 2000], "budget_bytes": serve.HBM_BUDGET,
                    "tile": list(tune["tile"]), "chunks": tune["chunks"],
-                   "creators": args.serve_creators, "batch": loop.batch},
-        "capacity_requests_per_s": cap_rps, "levels": levels,
+                   "creators": args.serve_creators, "batch": loop.batch, "weights": "standard"},
+        "capacity_requests_per_s": cap_rps, "levels": out_levels,
+        "served_equals_alone": {"rid": watch.rid, "grid": list(watch.n), "steps": watch.steps,
+                                "bitwise_equal": bool(same)},
+        "error_vs_fp64": err,
     }
 
 
@@ -347,7 +460,9 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=4,
+                    help="untimed steps; the first four record the fields the mixed runs "
+                         "are compared on")
     ap.add_argument("--workload", default="uniform256_ppc64_fp64", choices=list(WORKLOADS))
     ap.add_argument("--slack", type=float, default=0.15)
     ap.add_argument("--tile", type=int, nargs=3, default=(0, 0, 0),
@@ -362,6 +477,8 @@ def main():
     ap.add_argument("--field-grid", type=int, nargs=3, default=(256, 256, 256))
     ap.add_argument("--field-reps", type=int, default=20)
     ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--no-serving", action="store_true",
+                    help="skip the compact serving-loop leg of the default line")
     ap.add_argument("--no-small", action="store_true",
                     help="skip the GPU run of the CPU baseline's workload (32^3 x 16 ppc), "
                          "which follows the CPU baseline")
@@ -369,11 +486,15 @@ def main():
     ap.add_argument("--ensemble-steps", type=int, default=500)
     ap.add_argument("--ensemble-seq-steps", type=int, default=10)
     ap.add_argument("--serve", action="store_true",
-                    help="run the serving-loop workload instead (its own JSON line)")
+                    help="run the full serving-loop workload instead (its own JSON line)")
     ap.add_argument("--serve-requests", type=int, default=48)
     ap.add_argument("--serve-creators", type=int, default=2,
                     help="worker threads creating admitted runs (0: in the loop)")
     args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: the library has no multi-GPU path "
+              "(one MI355X holds the whole 256^3 domain)", file=sys.stderr)
+        sys.exit(2)
     if args.copy_calibration:
         print(json.dumps(run_copy_calibration()), flush=True)
         return
@@ -383,30 +504,30 @@ def main():
     if args.serve:
         print(json.dumps(run_serving_bench(args)), flush=True)
         return
-    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU: the library has no multi-GPU path "
-              "(one MI355X holds the whole 256^3 domain)", file=sys.stderr)
-        sys.exit(2)
-    res = run_gpu(args)
-    if not args.no_mixed and args.workload == "uniform256_ppc64_fp64":
-        mixed = {}
-        for name in ("uniform256_ppc64_mixed", "hump256_ppc64_mixed"):
-            log(f"[bench] mixed-precision workload {name}")
-            r = run_gpu(args, name)
-            mixed[name] = {"value": r["value"], "ms_per_step": r["ms_per_step"],
-                           "vs_fp64": r["value"] / res["value"], "roofline": r["roofline"],
-                           "energy": r["energy"]}
-        log("[bench] mixed vs FP64 one-step error at 128^3")
-        mixed["error_vs_fp64"] = mixed_error_vs_fp64(args)
-        res["mixed"] = mixed
-    if not args.no_ensemble and args.workload == "uniform256_ppc64_fp64":
+    headline = args.workload == "uniform256_ppc64_fp64"
+    res = run_gpu(args, record=4 if headline and not args.no_mixed else 0)
+    if not args.no_mixed and headline:
+        res["mixed"] = run_mixed_pairs(args, res)
+    res.pop("_fields", None)
+    if not args.no_ensemble and headline:
         res["ensemble"] = run_ensemble_bench(args)
+    if not args.no_serving and headline:
+        # a compact serving leg: 24 requests, capacity, then 80% of it
+        srv = argparse.Namespace(**vars(args))
+        srv.serve_requests = 24
+        r = run_serving_bench(srv, levels=(0.8,))
+        res["serving"] = {k: r[k] for k in ("config", "capacity_requests_per_s", "levels",
+                                            "served_equals_alone", "error_vs_fp64")}
+        lv = r["levels"]["load80"]
+        res["serving"].update({"ttc_p50_s": lv["ttc_p50"], "ttc_p99_s": lv["ttc_p99"],
+                               "particle_steps_per_s": lv["particle_steps_per_s"],
+                               "mean_active": lv["mean_active"], "offered_load": 0.8})
     if not args.no_cpu:
         log("[bench] timing the CPU baseline (oracle)")
         res["cpu_baseline"] = run_cpu_baseline(args.cpu_seconds)
     else:
         res["cpu_baseline"] = None
-    if not args.no_small and res["cpu_baseline"] and args.workload == "uniform256_ppc64_fp64":
+    if not args.no_small and res["cpu_baseline"] and headline:
         # the CPU baseline's own workload (32^3 x 16 ppc x 2 species, FP64) on
         # the GPU, stepped one call per step like the headline
         log("[bench] the CPU baseline's workload on the GPU")

@@ -73,7 +73,7 @@ typedef struct {
     int32_t tile[3];       /* cells per tile; 0 = default (8^3 if the grid holds
                               at least 512 such tiles, else 4^3) */
     double slack;          /* bin capacity slack fraction; <= 0 = 0.15 */
-    int32_t chunks;        /* work items per tile; 0 = fill the device */
+    int32_t chunks;        /* work items per tile, 0 to 32767; 0 = fill the device */
     int32_t track_ids;     /* keep a 64-bit particle id beside each particle */
     int32_t device;        /* HIP device ordinal */
     int32_t charge_bits;   /* fixed-point fraction bits of the charge grid; 0 = derived */

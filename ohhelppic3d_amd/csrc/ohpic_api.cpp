@@ -1130,6 +1130,8 @@ const char *ohpic_last_error(const ohpic_ctx *ctx)
 int ohpic_estimate_footprint(const ohpic_config *cfg, int64_t *bytes)
 {
     if (!cfg || !bytes) return fail(nullptr, OHPIC_EINVAL, "NULL argument");
+    if (cfg->chunks < 0 || cfg->chunks > kMaxChunks)
+        return fail(nullptr, OHPIC_EINVAL, "chunks must be in [0, %d] (0 = automatic)", kMaxChunks);
     int num_cus = 256;
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, cfg->device) == hipSuccess)
@@ -1156,6 +1158,8 @@ int ohpic_create(const ohpic_config *cfg, ohpic_ctx **out)
                     kMaxFftLen);
     if (c.weight_order != 0 && c.weight_order != 1)
         return fail(nullptr, OHPIC_EINVAL, "weight_order must be 0 or 1");
+    if (c.chunks < 0 || c.chunks > kMaxChunks)
+        return fail(nullptr, OHPIC_EINVAL, "chunks must be in [0, %d] (0 = automatic)", kMaxChunks);
     for (int s = 0; s < c.nspecies; ++s) {
         if (c.qm[s] == 0.0) return fail(nullptr, OHPIC_EINVAL, "species %d: qm == 0", s);
         if (c.capacity[s] < 0) return fail(nullptr, OHPIC_EINVAL, "species %d: capacity < 0", s);

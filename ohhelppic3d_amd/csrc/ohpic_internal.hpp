@@ -87,6 +87,8 @@ struct WorkItem {
     int16_t nchunks;   // slices per bin
     int32_t slot;      // row of the per-item partial-sum table
 };
+// largest chunks per tile a configuration may ask for (WorkItem::nchunks)
+constexpr int kMaxChunks = 32767;
 
 constexpr int kKeSlots = 2 * kMaxSpecies;  // per item: ke_old[8], ke_new[8]
 #ifndef OHPIC_EFIELD_MAX_BLOCKS

@@ -8,6 +8,14 @@ step one launch of each field kernel over a table of per-instance FFT jobs
 and one launch of the particle kernel over the work items of every instance,
 replayed from a captured hipGraph.  Tile shape and chunks per tile come from
 ``autotune``.
+
+The mixed-precision form uses the standard CIC weight order by default
+(``InstanceSpec.config(weight_order=...)``): with the reference's mirrored
+weights (linear_interpolator.f90:54-57) a particle's deposit jumps by two
+nodes when it crosses a cell face, so any 32-bit position that lands on the
+other side of a face than the FP64 one moves a whole macro charge, and rho
+after a push differs from FP64 by ~5e-4 relative L2.  With standard weights
+the deposit is continuous and the mixed form stays within 1e-7 of FP64.
 """
 from __future__ import annotations
 
@@ -45,19 +53,22 @@ class InstanceSpec:
         return sum(self.counts)
 
     def config(self, tile=(0, 0, 0), chunks: int = 0, precision: int = L.MIXED,
-               track_ids: bool = False, device: int = 0, slack: float = 0.3) -> SimConfig:
+               track_ids: bool = False, device: int = 0, slack: float = 0.3,
+               weight_order: int = L.WEIGHTS_STANDARD) -> SimConfig:
         # bins 30% above their count: small instances have few particles per
         # tile and larger relative fluctuations, and a bin that overflows
         # costs that step a list-mode retry
         return SimConfig(n=self.n, qm=(-1.0, 0.01),
                          q_macro=(-1.0 / self.ppc[0], 1.0 / self.ppc[1]),
                          capacity=self.counts, dt=self.dt, b0=(0.0, 0.0, self.omega_ce),
-                         precision=precision, tile=tuple(tile), chunks=chunks,
-                         track_ids=track_ids, device=device, slack=slack)
+                         precision=precision, weight_order=weight_order, tile=tuple(tile),
+                         chunks=chunks, track_ids=track_ids, device=device, slack=slack)
 
     def create(self, tile=(0, 0, 0), chunks: int = 0, precision: int = L.MIXED,
-               track_ids: bool = False, device: int = 0) -> Simulation:
-        sim = Simulation(self.config(tile, chunks, precision, track_ids, device))
+               track_ids: bool = False, device: int = 0,
+               weight_order: int = L.WEIGHTS_STANDARD) -> Simulation:
+        sim = Simulation(self.config(tile, chunks, precision, track_ids, device,
+                                     weight_order=weight_order))
         try:
             sim.seed_uniform_maxwell(list(self.counts), [self.vth_e, self.vth_e / 10.0],
                                      seed=(self.seed, 52))
@@ -86,7 +97,8 @@ def sample_ensemble(count: int = 128, seed: int = 2024, axes: Sequence[int] = AX
 def run_ensemble(specs: Sequence[InstanceSpec], steps: int, warmup: int = 2,
                  tune: Optional[dict] = None, tune_sample: int = 16, use_graph: bool = True,
                  precision: int = L.MIXED, batch: int = 8,
-                 progress: Optional[Callable[[str], None]] = None) -> dict:
+                 progress: Optional[Callable[[str], None]] = None,
+                 weight_order: int = L.WEIGHTS_STANDARD) -> dict:
     """Advance every instance ``steps`` steps as one group, in launch
     sequences of ``batch`` steps, and time each sequence (host wall clock
     around the synchronous call); a step's time is its sequence's over its
@@ -94,7 +106,7 @@ def run_ensemble(specs: Sequence[InstanceSpec], steps: int, warmup: int = 2,
     say = progress or (lambda s: None)
     t0 = time.perf_counter()
     if tune is None:
-        cfgs = [s.config(precision=precision) for s in specs]
+        cfgs = [s.config(precision=precision, weight_order=weight_order) for s in specs]
         tune = autotune(cfgs, sample=tune_sample, steps=3)
         say(f"[ensemble] autotune: tile {tune['tile']}, chunks {tune['chunks']} "
             f"({tune['candidates']} candidates, {time.perf_counter() - t0:.1f} s)")
@@ -103,7 +115,8 @@ def run_ensemble(specs: Sequence[InstanceSpec], steps: int, warmup: int = 2,
     grp = Group(use_graph=use_graph, batch=batch)
     try:
         for s in specs:
-            sims.append(s.create(tune["tile"], tune["chunks"], precision))
+            sims.append(s.create(tune["tile"], tune["chunks"], precision,
+                                 weight_order=weight_order))
             grp.attach(sims[-1])
         say(f"[ensemble] {len(sims)} instances created and seeded in "
             f"{time.perf_counter() - t1:.1f} s")
@@ -136,17 +149,19 @@ def run_ensemble(specs: Sequence[InstanceSpec], steps: int, warmup: int = 2,
         "step_ms_max": float(a.max()), "completion_s": total,
         "tile": list(tune["tile"]), "chunks": tune["chunks"],
         "autotune_particle_steps_per_s": tune.get("particle_steps_per_s"),
-        "graph": use_graph, "batch": batch, "graph_captures": stats["captures"],
+        "weight_order": weight_order, "graph": use_graph, "batch": batch,
+        "graph_captures": stats["captures"],
         "work_items": stats["items"],
         "energy_finite": bool(np.all(np.isfinite(energy))),
     }
 
 
 def run_sequential(specs: Sequence[InstanceSpec], steps: int, tune: dict,
-                   precision: int = L.MIXED) -> dict:
+                   precision: int = L.MIXED, weight_order: int = L.WEIGHTS_STANDARD) -> dict:
     """The same instances stepped one after another with Simulation.step
     (no group): the batching gain is value(group) / value(sequential)."""
-    sims = [s.create(tune["tile"], tune["chunks"], precision) for s in specs]
+    sims = [s.create(tune["tile"], tune["chunks"], precision, weight_order=weight_order)
+            for s in specs]
     try:
         for sim in sims:
             sim.step(1)
@@ -164,3 +179,50 @@ def run_sequential(specs: Sequence[InstanceSpec], steps: int, tune: dict,
     npart = sum(s.particles for s in specs)
     return {"value": npart * steps / total, "steps": steps,
             "step_ms": total / steps * 1e3}
+
+
+FIELD_NAMES = ("rho", "phi", "Ex", "Ey", "Ez")
+
+
+def record_fields(sim: Simulation, steps: int) -> List[List[np.ndarray]]:
+    """rho, phi, Ex, Ey, Ez after each of ``steps`` single steps (the first
+    holds the loaded state's fields: a step deposits and solves before it
+    pushes)."""
+    out = []
+    for _ in range(steps):
+        sim.step(1)
+        out.append([sim.grid(w) for w in (L.FIELD_RHO, L.FIELD_PHI, L.FIELD_EX, L.FIELD_EY,
+                                          L.FIELD_EZ)])
+    return out
+
+
+def field_errors(got: Sequence[Sequence[np.ndarray]],
+                 ref: Sequence[Sequence[np.ndarray]]) -> dict:
+    """Relative L2 of each field per step: {"step1": {"rho": ..}, ..} plus the
+    largest rho/E error over the steps after the first push."""
+    out = {}
+    worst = 0.0
+    for k, (g, r) in enumerate(zip(got, ref)):
+        e = {}
+        for name, a, b in zip(FIELD_NAMES, g, r):
+            den = float(np.linalg.norm(b))
+            e[name] = float(np.linalg.norm(a - b) / (den if den > 0 else 1.0))
+        out[f"step{k + 1}"] = e
+        if k > 0:
+            worst = max(worst, e["rho"], e["Ex"], e["Ey"], e["Ez"])
+    out["max_rho_E_after_push"] = worst if len(got) > 1 else None
+    return out
+
+
+def error_vs_fp64(spec: InstanceSpec, steps: int = 4, tile=(0, 0, 0), chunks: int = 0,
+                  weight_order: int = L.WEIGHTS_STANDARD) -> dict:
+    """The mixed-precision run of one instance against its FP64 run from the
+    same seeded state, field by field over its first ``steps`` steps."""
+    res = {}
+    for prec in (L.FP64, L.MIXED):
+        with spec.create(tile, chunks, prec, weight_order=weight_order) as sim:
+            res[prec] = record_fields(sim, steps)
+    out = field_errors(res[L.MIXED], res[L.FP64])
+    out["instance"] = {"n": list(spec.n), "ppc": list(spec.ppc), "seed": spec.seed,
+                       "weight_order": weight_order}
+    return out

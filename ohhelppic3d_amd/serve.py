@@ -225,12 +225,15 @@ class ServingLoop:
     def __init__(self, budget_bytes: int = HBM_BUDGET, max_active: int = 256,
                  tile=(8, 8, 8), chunks: int = 1, precision: int = L.MIXED,
                  use_graph: bool = True, track_ids: bool = False, device: int = 0,
-                 batch: int = 8, creators: int = 2):
+                 batch: int = 8, creators: int = 2,
+                 weight_order: int = L.WEIGHTS_STANDARD):
         self.budget = int(budget_bytes)
         self.max_active = int(max_active)
         self.tile = tuple(tile)
         self.chunks = int(chunks)
         self.precision = precision
+        # one CIC weight order for every run: a group's members share it
+        self.weight_order = int(weight_order)
         self.use_graph = use_graph
         self.track_ids = track_ids
         self.device = device
@@ -242,7 +245,7 @@ class ServingLoop:
         sim = None
         try:
             sim = req.spec.create(self.tile, self.chunks, self.precision, self.track_ids,
-                                  self.device)
+                                  self.device, weight_order=self.weight_order)
             return sim, None
         except OhpicError as e:
             if sim is not None:
@@ -251,7 +254,7 @@ class ServingLoop:
 
     def config(self, req: Request) -> SimConfig:
         return req.spec.config(self.tile, self.chunks, self.precision, self.track_ids,
-                               self.device)
+                               self.device, weight_order=self.weight_order)
 
     def footprint(self, req: Request) -> int:
         return estimate_footprint(self.config(req))

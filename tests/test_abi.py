@@ -118,7 +118,8 @@ def test_create_rejects_oversized_axes_and_bad_modes(L):
     c = L.Config()
     ctx = C.c_void_p()
     for field, value, msg in (("nx", 2048, b"axis lengths"), ("precision", 7, b"precision"),
-                              ("weight_order", 2, b"weight_order")):
+                              ("weight_order", 2, b"weight_order"), ("chunks", 32768, b"chunks"),
+                              ("chunks", -1, b"chunks")):
         lib.ohpic_config_default(C.byref(c))
         setattr(c, field, value)
         assert lib.ohpic_create(C.byref(c), C.byref(ctx)) == L.EINVAL, field
@@ -140,6 +141,11 @@ def test_footprint_estimate_grows_with_particles_and_slack(L):
     c.slack = 0.5
     assert lib.ohpic_estimate_footprint(C.byref(c), C.byref(b3)) == 0
     assert b1.value < b2.value < b3.value
+    # chunks per tile beyond the work item's 16-bit field are refused
+    c.chunks = 40000
+    assert lib.ohpic_estimate_footprint(C.byref(c), C.byref(b3)) == L.EINVAL
+    assert b"chunks" in lib.ohpic_last_error(None)
+    c.chunks = 0
     c.precision = L.MIXED
     b4 = C.c_int64()
     assert lib.ohpic_estimate_footprint(C.byref(c), C.byref(b4)) == 0

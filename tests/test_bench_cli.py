@@ -23,6 +23,23 @@ def test_headline_workload_is_256_cubed_64_ppc_fp64():
     assert wl["n"] == (256, 256, 256) and wl["ppc"] == 64 and wl["precision"] == 0
     assert bench.WORKLOADS["uniform256_ppc64_mixed"]["precision"] == 1
     assert bench.WORKLOADS["uniform32_ppc16_fp64"]["n"] == (32, 32, 32)
+    # the FP64 drop-in headline keeps the reference's mirrored CIC weights
+    assert wl["wo"] == bench.W_REF
+
+
+def test_mixed_runs_are_compared_with_fp64_of_the_same_load_and_weights():
+    import bench
+    seen = set()
+    for m, f in bench.MIXED_PAIRS:
+        wm, wf = bench.WORKLOADS[m], bench.WORKLOADS[f]
+        assert wm["precision"] == 1 and wf["precision"] == 0
+        assert wm["fp64"] == f
+        for k in ("n", "ppc", "hump", "wo"):
+            assert wm[k] == wf[k], (m, k)
+        seen.add(wm["wo"])
+    # both weight orders are timed, and the configuration-2 hump load
+    assert seen == {bench.W_REF, bench.W_STD}
+    assert any(bench.WORKLOADS[m]["hump"] == 9.0 for m, _ in bench.MIXED_PAIRS)
 
 
 def test_cpu_baseline_record(orc):
@@ -33,3 +50,7 @@ def test_cpu_baseline_record(orc):
     assert rec["value"] > 0 and rec["single_thread_value"] > 0
     assert rec["cores"] >= 1 and rec["ranks"] == 1
     assert "32^3" in rec["sample"]
+    assert rec["threads"] == rec["cores"]
+    host = rec["host"]
+    assert host["nproc"] >= 1 and 1 <= host["affinity"] <= host["nproc"]
+    assert rec["cpu_model"] and rec["cpu_model"] != "unknown"

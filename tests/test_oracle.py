@@ -104,6 +104,35 @@ def test_uniform_maxwell_loader(orc):
     assert c2.random() == rng.random()
 
 
+def test_maxwell_velocities_follow_reference_formula_exactly(orc):
+    """velocity_maxwell_distribution3d.f90:40-54 draws rands(1..4) and sets
+    v1 = 2 sigma sqrt(-2 log r1) cos(2 pi r2), v2 = ... sin(2 pi r2),
+    v3 = 2 sigma sqrt(-2 log r3) sin(2 pi r4), clipped at 6 sigma.  The
+    loader's resolution: sigma = v_th / 2 (so the std is v_th), log of 1 - r
+    (a PCG double can be 0), no clip.  The first particles' velocities equal
+    that formula on the same PCG draws bit for bit."""
+    vth = 1.7
+    rng = orc.Pcg32(42, 52)
+    check = rng.copy()
+    n = (8, 4, 6)
+    x, y, z, vx, vy, vz = orc.load_uniform_maxwell(rng, 5000, n, vth)
+    pi = 3.14159265358979  # science_constants.f90:4
+    sigma = vth / 2.0
+    check.double()  # random_fix
+    for i in range(4):
+        for _ in range(3):
+            check.double()  # x, y, z
+        r = [check.double() for _ in range(4)]
+        v1 = 2.0 * sigma * math.sqrt(-2.0 * math.log(1.0 - r[0])) * math.cos(2.0 * pi * r[1])
+        v2 = 2.0 * sigma * math.sqrt(-2.0 * math.log(1.0 - r[0])) * math.sin(2.0 * pi * r[1])
+        v3 = 2.0 * sigma * math.sqrt(-2.0 * math.log(1.0 - r[2])) * math.sin(2.0 * pi * r[3])
+        assert (vx[i], vy[i], vz[i]) == (v1, v2, v3), i
+    # no clip: among 15000 components some exceed 3 standard deviations
+    # (6 sigma of the reference's formula)
+    v = np.concatenate([vx, vy, vz])
+    assert np.max(np.abs(v)) > 3.0 * vth
+
+
 def test_hump_loader_density_profile(orc):
     n = (32, 32, 32)
     rng0 = orc.Pcg32(42, 52)
@@ -353,6 +382,38 @@ def _two_species(orc, n, ppc, seed=(42, 52), b=0.5, vth=(1.0, 0.1)):
         x, y, z, vx, vy, vz = orc.load_uniform_maxwell(rng, ppc * g, n, vt)
         sp.append(dict(qm=qm, q=q, x=x, y=y, z=z, vx=vx, vy=vy, vz=vz))
     return sp
+
+
+def test_step_order_deposit_solve_push(orc):
+    """mainstep's order (ohhelppic3d.f90:311-358): rho^n is the deposit of
+    the positions before the step, phi and E solve that rho, and each
+    particle is pushed (gather + Boris) in that E and wrapped twice.  One
+    orc_step equals that composition of the oracle's own pinned operators
+    bit for bit (a push before the deposit, or in the previous step's E,
+    would not)."""
+    n = (8, 8, 8)
+    sp = _two_species(orc, n, 2)
+    sim = orc.OracleSim(n, sp, 0.1, (0, 0, 0.5), 0)
+    sim.step(1)
+    rho = np.zeros(n[0] * n[1] * n[2])
+    for d in sp:
+        orc.deposit_periodic(n, d["x"], d["y"], d["z"], d["q"], 0, rho)
+    assert np.array_equal(sim.rho, rho)
+    phi = orc.poisson(n, rho)
+    assert np.array_equal(sim.phi, phi)
+    e = orc.efield(n, phi)
+    for k, a in zip(("ex", "ey", "ez"), e):
+        assert np.array_equal(getattr(sim, k), a)
+    for s, d in enumerate(sp):
+        for i in range(0, len(d["x"]), 37):
+            pos = (d["x"][i], d["y"][i], d["z"][i])
+            g = orc.gather_periodic(n, e, pos, 0)
+            eb = np.array([g[0], g[1], g[2], 0.0, 0.0, 0.5])
+            p, v = orc.boris_move(pos, (d["vx"][i], d["vy"][i], d["vz"][i]), d["qm"], eb, 0.1)
+            for a, k in enumerate("xyz"):
+                w = orc.apply_periodic(n[a], orc.apply_periodic(n[a], p[a]))
+                assert sim.sp[s][k][i] == w, (s, i, k)
+                assert sim.sp[s]["v" + k][i] == v[a], (s, i, k)
 
 
 def test_step_conserves_momentum_without_b(orc):
