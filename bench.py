@@ -485,6 +485,8 @@ def main():
     ap.add_argument("--ensemble-instances", type=int, default=128)
     ap.add_argument("--ensemble-steps", type=int, default=500)
     ap.add_argument("--ensemble-seq-steps", type=int, default=10)
+    ap.add_argument("--ensemble", action="store_true",
+                    help="run the ensemble workload alone (its own JSON line)")
     ap.add_argument("--serve", action="store_true",
                     help="run the full serving-loop workload instead (its own JSON line)")
     ap.add_argument("--serve-requests", type=int, default=48)
@@ -503,6 +505,9 @@ def main():
         return
     if args.serve:
         print(json.dumps(run_serving_bench(args)), flush=True)
+        return
+    if args.ensemble:
+        print(json.dumps(run_ensemble_bench(args)), flush=True)
         return
     headline = args.workload == "uniform256_ppc64_fp64"
     res = run_gpu(args, record=4 if headline and not args.no_mixed else 0)
