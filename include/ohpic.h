@@ -164,6 +164,20 @@ int ohpic_put_field(ohpic_ctx *ctx, int which, const double *values, int nelem,
 int ohpic_fft3d_r2r(int device, int nx, int ny, int nz, int backward, const double *in,
                     double *out);
 
+/* Allocation-free ohpic_fft3d_r2r.  No counterpart in the reference: for
+ * callers that cannot allocate device memory (tight serving loops, harnesses
+ * that forbid hipMalloc).  Same arguments, results bit for bit equal to
+ * ohpic_fft3d_r2r's; it calls no hipMalloc/hipFree and creates no stream (it
+ * runs on the null stream).  It works in one fixed device array of
+ * OHPIC_FFT_NOALLOC_MAX_POINTS doubles plus the axis tables (about 16 MiB),
+ * which each GPU it runs on holds until the process ends; nx*ny*nz above
+ * that maximum returns OHPIC_EINVAL.  Every call on a GPU shares that array:
+ * the library serialises its calls with a process-wide lock, and the array
+ * keeps the last call's contents. */
+#define OHPIC_FFT_NOALLOC_MAX_POINTS (1 << 21)
+int ohpic_fft3d_r2r_noalloc(int device, int nx, int ny, int nz, int backward, const double *in,
+                            double *out);
+
 /* --- per-particle operators over host arrays -----------------------------------
  * The reference's per-particle bindings, batched: each call uploads the
  * arrays, runs one kernel on `device` and copies the results back.

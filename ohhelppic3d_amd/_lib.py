@@ -29,6 +29,7 @@ EXPORTS = (
     "ohpic_seed_uniform_maxwell", "ohpic_seed_hump_maxwell", "ohpic_deposit",
     "ohpic_field_solve", "ohpic_gather_push", "ohpic_boundary", "ohpic_step",
     "ohpic_synchronize", "ohpic_get_field", "ohpic_put_field", "ohpic_fft3d_r2r",
+    "ohpic_fft3d_r2r_noalloc",
     "ohpic_diagnostics", "ohpic_energy", "ohpic_cell_counts", "ohpic_enable_timing", "ohpic_kernel_times",
     "ohpic_memory_footprint", "ohpic_estimate_footprint", "ohpic_boris_batch",
     "ohpic_interp_batch", "ohpic_scatter_batch", "ohpic_wrap_batch",
@@ -144,6 +145,7 @@ def lib():
         "ohpic_get_field": [vp, C.c_int, dp, C.c_int, i32p, i32p, C.c_int],
         "ohpic_put_field": [vp, C.c_int, dp, C.c_int, i32p, i32p, C.c_int],
         "ohpic_fft3d_r2r": [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, dp, dp],
+        "ohpic_fft3d_r2r_noalloc": [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, dp, dp],
         "ohpic_diagnostics": [vp, C.POINTER(Diag)],
         "ohpic_energy": [vp, dp],
         "ohpic_cell_counts": [vp, C.c_int, i32p],

@@ -1870,6 +1870,105 @@ int ohpic_fft3d_r2r(int device, int nx, int ny, int nz, int backward, const doub
     return OHPIC_OK;
 }
 
+// ------------------------------------------------------- allocation-free FFT
+// ohpic_fft3d_r2r_noalloc works in `scratch`: the transform buffer of up to
+// OHPIC_FFT_NOALLOC_MAX_POINTS values, the three axes' twiddle tables and the
+// job record.  Every GPU the library runs on holds its own copy of the array
+// until the process ends; host code reaches it only through the address
+// hipGetSymbolAddress returns on that device (g_noalloc_ptr).
+namespace {
+struct NoallocArea {
+    double buf[OHPIC_FFT_NOALLOC_MAX_POINTS];
+    double2 tw[3][kMaxFftLen];
+    FftJob job;
+};
+constexpr int kNoallocMaxDevices = 64;
+std::mutex g_noalloc_mu;                          // one call at a time, process-wide
+NoallocArea *g_noalloc_ptr[kNoallocMaxDevices];   // scratch's address on each device
+double2 g_noalloc_tw[3][kMaxFftLen];              // host copy of the twiddle tables
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[sizeof(NoallocArea)];
+
+int ohpic_fft3d_r2r_noalloc(int device, int nx, int ny, int nz, int backward, const double *in,
+                            double *out)
+{
+    if (!in || !out) return fail(nullptr, OHPIC_EINVAL, "NULL argument");
+    if (nx < 1 || ny < 1 || nz < 1) return fail(nullptr, OHPIC_EINVAL, "sizes must be >= 1");
+    const int64_t ng = (int64_t)nx * ny * nz;
+    if (ng > OHPIC_FFT_NOALLOC_MAX_POINTS)
+        return fail(nullptr, OHPIC_EINVAL,
+                    "nx*ny*nz = %lld exceeds the allocation-free maximum of %d points",
+                    (long long)ng, OHPIC_FFT_NOALLOC_MAX_POINTS);
+    if (device < 0 || device >= kNoallocMaxDevices)
+        return fail(nullptr, OHPIC_EINVAL, "device %d out of range", device);
+    const int n[3] = {nx, ny, nz};
+    FftJob job{};
+    int64_t grp[3];
+    int rs[3];
+    for (int a = 0; a < 3; ++a) {
+        FftAxisDev &ax = job.ax[a];
+        ax.n = n[a];
+        if (!fft_factorize(n[a], ax.fac, &ax.nfac))
+            return fail(nullptr, OHPIC_EINVAL,
+                        "axis length %d: unsupported (max %d, prime factors <= %d)", n[a],
+                        kMaxFftLen, kMaxRadix);
+        fft_plan_divisors(ax);
+        job.n[a] = n[a];
+        grp[a] = fft_groups(a, fft_group_lines(n[a]), nx, ny, nz);
+        rs[a] = fft_radix_set(ax);
+    }
+    job.inv_n = 1.0 / (double)ng;
+
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    if (hipSetDevice(device) != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(nullptr, OHPIC_ENODEV, "device %d unavailable", device);
+    }
+    if (!g_noalloc_ptr[device]) {
+        // first call on this device: its own address of the array
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            (void)hipGetLastError();
+            return fail(nullptr, OHPIC_EHIP, "hipGetSymbolAddress(scratch) on device %d: %s",
+                        device, hipGetErrorString(e));
+        }
+        g_noalloc_ptr[device] = static_cast<NoallocArea *>(p);
+    }
+    NoallocArea *d = g_noalloc_ptr[device];
+    for (int a = 0; a < 3; ++a) {
+        fft_twiddles(n[a], g_noalloc_tw[a]);
+        job.ax[a].tw = d->tw[a];
+    }
+    job.src = d->buf;
+    job.data = d->buf;
+    // the null stream; every copy below is complete when the final
+    // synchronisation returns, before the lock and the host tables are released
+    hipStream_t st = nullptr;
+    hipError_t e = hipMemcpyAsync(d->tw, g_noalloc_tw, sizeof g_noalloc_tw,
+                                  hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(&d->job, &job, sizeof job, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(d->buf, in, ng * sizeof(double), hipMemcpyHostToDevice, st);
+    if (!backward) {
+        const FftPass p[3] = {kPassR2hcX, kPassR2hcY, kPassR2hcZ};
+        for (int a = 0; a < 3 && e == hipSuccess; ++a) e = fft_pass(p[a], &d->job, 1, grp[a], rs[a], st);
+    } else {
+        const FftPass p[3] = {kPassHc2rXScaled, kPassHc2rY, kPassHc2rZ};
+        for (int a = 2; a >= 0 && e == hipSuccess; --a) e = fft_pass(p[a], &d->job, 1, grp[a], rs[a], st);
+    }
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(out, d->buf, ng * sizeof(double), hipMemcpyDeviceToHost, st);
+    const hipError_t es = hipStreamSynchronize(st);
+    if (e == hipSuccess) e = es;
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(nullptr, OHPIC_EHIP, "fft3d (allocation-free): %s", hipGetErrorString(e));
+    }
+    return OHPIC_OK;
+}
+
 int ohpic_diagnostics(ohpic_ctx *ctx, ohpic_diag *out)
 {
     if (!ctx || !out) return fail(ctx, OHPIC_EINVAL, "NULL argument");
