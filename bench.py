@@ -109,10 +109,15 @@ def run_gpu(args, workload=None, timing=True, record=0):
     # which only small grids notice)
     sim.enable_timing(timing)
     per_step = []
+    history = [] if getattr(args, "history", False) else None
     t_start = time.perf_counter()
     for i in range(args.steps):
         ts = time.perf_counter()
         sim.step(1)
+        if history is not None:
+            # kinetic, field, total energy and net charge of this step (one
+            # synchronisation per step, inside the timed loop)
+            history.append([float(v) for v in sim.energy()])
         per_step.append(time.perf_counter() - ts)
         if e_start is None:
             e_start = sim.diagnostics()["total_energy"]
@@ -171,6 +176,9 @@ def run_gpu(args, workload=None, timing=True, record=0):
     }
     if fields is not None:
         res["_fields"] = fields
+    if history is not None:
+        res["energy_history"] = {"columns": ["kinetic", "field", "total", "net_charge"],
+                                 "steps": history}
     return res
 
 
@@ -474,6 +482,9 @@ def main():
     ap.add_argument("--chunks", type=int, default=0)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--history", action="store_true",
+                    help="record the energy history of the timed steps (one host "
+                         "synchronisation per step)")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true")
     ap.add_argument("--field-solve", action="store_true",
