@@ -213,7 +213,8 @@ def run_mixed_pairs(args, headline):
             err["meets_bar"] = worst is not None and worst <= MIXED_TOL
             entry["error_vs_fp64"] = err
             log(f"[bench] {mname} vs {fname}: ratio {entry['vs_fp64']:.3f}, worst rho/E "
-                f"rel L2 after a push {worst:.2e}")
+                f"rel L2 after a push " + (f"{worst:.2e}" if worst is not None else
+                                            "not recorded (fewer than 2 warm-up steps)"))
         out[mname] = entry
         # an FP64 run is kept (host fields) only until its last mixed partner
         if all(p[1] != fname for p in MIXED_PAIRS[MIXED_PAIRS.index((mname, fname)) + 1:]):
