@@ -514,11 +514,12 @@ int do_deposit(ohpic_ctx *ctx)
 // from rho: five axis passes, node E and its energy partials (reduced when
 // reduce_e).
 hipError_t solve_passes(const FftJob *jobs, const InstDev *insts, int n, const int64_t grp[3],
-                        const int rs[3], int max_eblocks, bool reduce_e, hipStream_t st)
+                        const int rs[3], int z256, int max_eblocks, bool reduce_e,
+                        hipStream_t st)
 {
     hipError_t e = fft_pass(kPassR2hcXFromSrc, jobs, n, grp[0], rs[0], st);
     if (e == hipSuccess) e = fft_pass(kPassR2hcY, jobs, n, grp[1], rs[1], st);
-    if (e == hipSuccess) e = fft_pass(kPassPoissonZ, jobs, n, grp[2], rs[2], st);
+    if (e == hipSuccess) e = fft_poisson_z(jobs, n, grp[2], rs[2], z256, st);
     if (e == hipSuccess) e = fft_pass(kPassHc2rY, jobs, n, grp[1], rs[1], st);
     if (e == hipSuccess) e = fft_pass(kPassHc2rXScaled, jobs, n, grp[0], rs[0], st);
     if (e == hipSuccess) e = launch_efield_b(insts, n, max_eblocks, reduce_e, st);
@@ -531,7 +532,7 @@ int do_solve(ohpic_ctx *ctx, bool reduce_e)
 {
     timer_begin(ctx, 1);
     HIPCHK(ctx, solve_passes(ctx->d_job, ctx->d_inst, 1, ctx->fft_grp, ctx->fft_rs,
-                             efield_blocks_of(ctx->n[0], ctx->n[1], ctx->n[2]), reduce_e,
+                             ctx->n[2] == 256 ? kZ256All : kZ256None, efield_blocks_of(ctx->n[0], ctx->n[1], ctx->n[2]), reduce_e,
                              ctx->st));
     timer_end(ctx, 1);
     return OHPIC_OK;
@@ -748,6 +749,7 @@ struct ohpic_group {
     bool dirty = true;
     int64_t max_ng = 0;
     int max_ntiles = 0, max_rows = 0;  // max_rows: largest E-kernel block count
+    int z256 = kZ256None;              // members whose Poisson z pass has 256-point lines
     int64_t grp[3]{};
     int rs[3]{};
     int T[3]{};
@@ -859,6 +861,7 @@ int group_refresh(ohpic_group *g)
         items.reserve((size_t)total);
         g->max_ng = 0;
         g->max_ntiles = g->max_rows = 0;
+        int n256 = 0;
         for (int a = 0; a < 3; ++a) {
             g->grp[a] = 0;
             g->rs[a] = 0;
@@ -874,6 +877,7 @@ int group_refresh(ohpic_group *g)
             g->max_ng = std::max(g->max_ng, m->ng);
             g->max_ntiles = std::max(g->max_ntiles, m->ntiles);
             g->max_rows = std::max(g->max_rows, efield_blocks_of(m->n[0], m->n[1], m->n[2]));
+            n256 += m->n[2] == 256;
             for (int a = 0; a < 3; ++a) {
                 g->grp[a] = std::max(g->grp[a], m->fft_grp[a]);
                 g->rs[a] = std::max(g->rs[a], m->fft_rs[a]);
@@ -881,6 +885,7 @@ int group_refresh(ohpic_group *g)
             g->seen_inst[i] = m->inst_ver;
             g->seen_items[i] = m->items_ver;
         }
+        g->z256 = n256 == 0 ? kZ256None : (n256 == n ? kZ256All : kZ256Some);
         // heavy items first over the whole table: the dispatcher then fills
         // the machine with the largest chunks of every instance
         std::stable_sort(items.begin(), items.end(), [&](const WorkItem &a, const WorkItem &b) {
@@ -935,7 +940,8 @@ hipError_t group_enqueue(ohpic_group *g, int k)
         e = launch_begin_step_b(g->d_insts, g->d_parity, n, g->max_ng, g->max_ntiles,
                                 g->persistent ? g->d_queue : nullptr, g->st);
         if (e == hipSuccess)
-            e = solve_passes(g->d_jobs, g->d_insts, n, g->grp, g->rs, g->max_rows, false, g->st);
+            e = solve_passes(g->d_jobs, g->d_insts, n, g->grp, g->rs, g->z256, g->max_rows, false,
+                             g->st);
         if (e == hipSuccess) {
             if (g->persistent)
                 e = launch_particles_queue(g->precision, g->weight_order, g->T, g->d_insts,

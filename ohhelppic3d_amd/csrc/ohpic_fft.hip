@@ -393,7 +393,7 @@ __device__ __forceinline__ void store_complex_pairs(const cplx *buf, gmem_t<doub
 template <int KIND, int AXIS, int RS, bool FROM_SRC, bool SCALE>
 __global__ void __launch_bounds__(kFftBlock)
     __attribute__((amdgpu_waves_per_eu(OHPIC_FFT_MIN_WAVES, 8)))
-    k_fft_pass(const FftJob *__restrict__ jobs)
+    k_fft_pass(const FftJob *__restrict__ jobs, int skip256)
 {
     __shared__ cplx buf[kFftLdsCplx];
     __shared__ double sLamXY[KIND == kFftPoisson ? 64 : 1];
@@ -402,6 +402,8 @@ __global__ void __launch_bounds__(kFftBlock)
     const int nx = J.n[0], ny = J.n[1], nz = J.n[2];
     const FftAxisDev &ax = J.ax[AXIS];
     const int n = ax.n;
+    // a Poisson job with 256-point z lines is k_poisson_z256's
+    if (KIND == kFftPoisson && skip256 && n == 256) return;
     const int G = fft_group_lines(n);
     const int64_t ngroups = fft_groups(AXIS, G, nx, ny, nz);
     if ((int64_t)blockIdx.x >= ngroups) return;
@@ -500,6 +502,200 @@ __global__ void __launch_bounds__(kFftBlock)
         store_complex_pairs<AXIS>(buf, dst, lb, n, nlines, ax, out_scale);
 }
 
+// ------------------------------------------------ Poisson z pass, n = 256
+// The fused z pass for z lines of 256 points as a 16 x 16 decomposition with
+// the 16-point DFTs in registers: n = 16 n1 + n2, k = k1 + 16 k2.  A 64-thread
+// workgroup holds four complex lines (8 real z lines at consecutive x, packed
+// in pairs), 16 threads per line, 16 points per thread.  Forward: thread n2
+// loads x[16 n1 + n2] (n1 = 0..15), DFT16 over n1, twiddle W256^{n2 k1},
+// transpose through LDS, DFT16 over n2: thread k1 holds X[k1 + 16 k2].  The
+// Green's-function division pairs X[k] with X[256 - k] (read back from LDS),
+// the inverse runs the same steps backwards, and thread n2 stores x[16 n1 +
+// n2].  Three LDS exchanges per visit against ten for the radix-4 stages of
+// k_fft_pass (67 us against 105 us at 256^3).  Rows of the LDS exchange
+// buffer are padded to 17 entries so that 16 lanes of one line cover all
+// banks.
+constexpr int kZ256Lines = 4;                 // complex lines per workgroup
+constexpr int kZ256Threads = 16 * kZ256Lines;
+constexpr int kZ256Row = 17;                  // padded row of 16 entries
+constexpr int kZ256LineStride = 16 * kZ256Row;
+
+// cos and sin of 2 pi j / 16, j = 0..15
+__device__ __forceinline__ cplx w16(int j, int sign)
+{
+    constexpr double c1 = 0.92387953251128675613, s1 = 0.38268343236508977173,
+                     r2 = 0.70710678118654752440;
+    const double C[16] = {1.0, c1, r2, s1, 0.0, -s1, -r2, -c1, -1.0, -c1, -r2, -s1, 0.0, s1, r2, c1};
+    const double S[16] = {0.0, s1, r2, c1, 1.0, c1, r2, s1, 0.0, -s1, -r2, -c1, -1.0, -c1, -r2, -s1};
+    return cplx{C[j & 15], sign < 0 ? -S[j & 15] : S[j & 15]};
+}
+
+// In-place 16-point DFT of a[0..15] with exponent sign SIGN (-1 forward):
+// 4-point DFTs over a[m + 4 j], twiddles w16^{m q}, 4-point DFTs over m;
+// output X[q + 4 r] in a[q + 4 r].
+template <int SIGN>
+__device__ __forceinline__ void dft16(cplx *a)
+{
+    cplx t[4][4];
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+        cplx v[4] = {a[m], a[m + 4], a[m + 8], a[m + 12]};
+        butterfly<SIGN, 4, false>(v, nullptr, 0);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) t[m][q] = (m == 0 || q == 0) ? v[q] : cmul(v[q], w16(m * q, SIGN));
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        cplx v[4] = {t[0][q], t[1][q], t[2][q], t[3][q]};
+        butterfly<SIGN, 4, false>(v, nullptr, 0);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) a[q + 4 * r] = v[r];
+    }
+}
+
+// 1/d for 0 < d < 2^1000 (d = 0 gives inf, which the caller discards): the
+// hardware estimate (about 23 correct bits) refined by two Newton steps, so
+// the error is that of the final rounding.  Replacing the 32 correctly
+// rounded divisions per thread (and their branches) took the pass from 82 to
+// 67 us at 256^3.
+__device__ __forceinline__ double z256_recip(double d)
+{
+    double r = __builtin_amdgcn_rcp(d);
+    r = __fma_rn(__fma_rn(-d, r, 1.0), r, r);
+    return __fma_rn(__fma_rn(-d, r, 1.0), r, r);
+}
+
+__global__ void __launch_bounds__(kZ256Threads) k_poisson_z256(const FftJob *__restrict__ jobs)
+{
+    __shared__ cplx xb[kZ256Lines * kZ256LineStride];
+    __shared__ double2 sTw[256];
+    __shared__ double sLam[2 * kZ256Lines];
+    const FftJob &J = jobs[blockIdx.y];
+    if (J.n[2] != 256) return;
+    const int nx = J.n[0], ny = J.n[1];
+    const int64_t nxy = (int64_t)nx * ny;
+    const int ngroups = (int)((nxy + 2 * kZ256Lines - 1) / (2 * kZ256Lines));
+    if ((int)blockIdx.x >= ngroups) return;
+    // neighbouring groups read the two 64-byte halves of the same 128-byte
+    // lines: keep them on one XCD
+    const int grp = xcd_remap((int)blockIdx.x, ngroups);
+    const int64_t l0 = (int64_t)grp * (2 * kZ256Lines);
+    const int nvalid = (int)min<int64_t>(2 * kZ256Lines, nxy - l0);
+    const int tid = threadIdx.x;
+    const int c = tid >> 4, t = tid & 15;
+    gmem_t<double> *__restrict__ data = as_global(J.data);
+    const FftAxisDev &ax = J.ax[2];
+    for (int i = tid; i < 256; i += kZ256Threads) {
+        const d2v w = gload(reinterpret_cast<const d2v *>(ax.tw), i);
+        sTw[i] = make_double2(w.x, w.y);
+    }
+    if (tid < 2 * kZ256Lines) {
+        // lam_x + lam_y of real line tid; -1 marks the DC line (ix = iy = 0);
+        // lines past the grid's end get 1 (they hold zeros; no 0/0)
+        double v = 1.0;
+        if (tid < nvalid) {
+            const int64_t l = l0 + tid;
+            const int iy = (int)(l / nx), ix = (int)(l - (int64_t)iy * nx);
+            v = (l == 0) ? -1.0 : gload(J.lam[0], ix) + gload(J.lam[1], iy);
+        }
+        sLam[tid] = v;
+    }
+    // real lines 2c (re) and 2c + 1 (im): one 16-byte load per point when
+    // both are present and nx*ny is even (l0 is even, so every pair is then
+    // 16-byte aligned), else two 8-byte loads
+    const bool va = 2 * c < nvalid, vb = 2 * c + 1 < nvalid;
+    const bool v16 = vb && (nxy & 1) == 0;
+    const int64_t la = l0 + 2 * c;
+    cplx a[16];
+#pragma unroll
+    for (int n1 = 0; n1 < 16; ++n1) {
+        const int64_t o = la + (int64_t)(16 * n1 + t) * nxy;
+        if (v16) {
+            const d2v v = ((const gmem_t<d2v> *)data)[o >> 1];
+            a[n1] = cplx{v.x, v.y};
+        } else {
+            a[n1] = cplx{va ? data[o] : 0.0, vb ? data[o + 1] : 0.0};
+        }
+    }
+    __syncthreads();  // sTw, sLam
+    const lds_d2v *tw = (const lds_d2v *)sTw;
+    cplx *xl = xb + c * kZ256LineStride;
+
+    // forward: DFT16 over n1 (thread n2 = t), twiddle, transpose
+    dft16<-1>(a);
+#pragma unroll
+    for (int k1 = 0; k1 < 16; ++k1) {
+        cplx v = a[k1];
+        if (k1 > 0 && t > 0) {
+            const d2v w = tw[t * k1];
+            v = cmul(v, cplx{w.x, w.y});
+        }
+        xl[t * kZ256Row + k1] = v;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int n2 = 0; n2 < 16; ++n2) a[n2] = xl[n2 * kZ256Row + t];
+    // DFT16 over n2 (thread k1 = t): a[k2] = X[t + 16 k2]
+    dft16<-1>(a);
+    __syncthreads();  // transpose reads done before the buffer is reused
+#pragma unroll
+    for (int k2 = 0; k2 < 16; ++k2) xl[k2 * kZ256Row + t] = a[k2];
+    __syncthreads();
+
+    // Green's-function division of the packed spectra (as k_fft_pass):
+    // A_k = (Z_k + conj Z_{n-k}) / 2, B_k = (Z_k - conj Z_{n-k}) / 2i, divided
+    // by their lines' denominators (DC := 0), repacked as A' + i B'
+    const double lam_a = sLam[2 * c], lam_b = sLam[2 * c + 1];
+    const double lxa = lam_a < 0.0 ? 0.0 : lam_a, lxb = lam_b < 0.0 ? 0.0 : lam_b;
+    const gmem_t<double> *__restrict__ lamz = as_global(J.lam[2]);
+#pragma unroll
+    for (int k2 = 0; k2 < 16; ++k2) {
+        const int k = t + 16 * k2;
+        const int m1 = (16 - t) & 15;
+        const int m2 = t == 0 ? ((16 - k2) & 15) : 15 - k2;
+        const cplx zk = a[k2], zm = xl[m2 * kZ256Row + m1];
+        const double lz = lamz[k];
+        const double ra = z256_recip(lxa + lz), rb = z256_recip(lxb + lz);
+        // the DC line's k = 0 mode is set to zero (its denominator is 0)
+        const double sa = (lam_a < 0.0 && k == 0) ? 0.0 : 0.5 * ra;
+        const double sb = (lam_b < 0.0 && k == 0) ? 0.0 : 0.5 * rb;
+        const cplx A = {sa * (zk.re + zm.re), sa * (zk.im - zm.im)};
+        const cplx B = {sb * (zk.im + zm.im), -sb * (zk.re - zm.re)};
+        a[k2] = cplx{A.re - B.im, A.im + B.re};
+    }
+
+    // inverse: DFT16 over k2 (thread k1 = t), twiddle, transpose, DFT16 over k1
+    dft16<+1>(a);
+    __syncthreads();  // pair reads done before the buffer is reused
+#pragma unroll
+    for (int n2 = 0; n2 < 16; ++n2) {
+        cplx v = a[n2];
+        if (n2 > 0 && t > 0) {
+            const d2v w = tw[t * n2];
+            v = cmul(v, cplx{w.x, -w.y});
+        }
+        xl[t * kZ256Row + n2] = v;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k1 = 0; k1 < 16; ++k1) a[k1] = xl[k1 * kZ256Row + t];
+    dft16<+1>(a);
+    // thread n2 = t holds x[16 n1 + n2] in a[n1]
+#pragma unroll
+    for (int n1 = 0; n1 < 16; ++n1) {
+        const int64_t o = la + (int64_t)(16 * n1 + t) * nxy;
+        if (v16) {
+            d2v v;
+            v.x = a[n1].re;
+            v.y = a[n1].im;
+            ((gmem_t<d2v> *)data)[o >> 1] = v;
+        } else {
+            if (va) data[o] = a[n1].re;
+            if (vb) data[o + 1] = a[n1].im;
+        }
+    }
+}
+
 }  // namespace
 
 void fft_plan_divisors(FftAxisDev &ax)
@@ -529,16 +725,32 @@ int fft_radix_set(const FftAxisDev &ax)
 
 template <int KIND, int AXIS, bool FROM_SRC, bool SCALE>
 static hipError_t launch_pass(const FftJob *jobs, int njobs, int64_t max_groups, int rs,
-                              hipStream_t st)
+                              hipStream_t st, int skip256 = 0)
 {
     if (njobs <= 0 || max_groups <= 0) return hipSuccess;
     const dim3 g((unsigned)round_up_xcds(max_groups), (unsigned)njobs), b(kFftBlock);
     if (rs == 0)
-        hipLaunchKernelGGL((k_fft_pass<KIND, AXIS, 0, FROM_SRC, SCALE>), g, b, 0, st, jobs);
+        hipLaunchKernelGGL((k_fft_pass<KIND, AXIS, 0, FROM_SRC, SCALE>), g, b, 0, st, jobs, skip256);
     else if (rs == 1)
-        hipLaunchKernelGGL((k_fft_pass<KIND, AXIS, 1, FROM_SRC, SCALE>), g, b, 0, st, jobs);
+        hipLaunchKernelGGL((k_fft_pass<KIND, AXIS, 1, FROM_SRC, SCALE>), g, b, 0, st, jobs, skip256);
     else
-        hipLaunchKernelGGL((k_fft_pass<KIND, AXIS, 2, FROM_SRC, SCALE>), g, b, 0, st, jobs);
+        hipLaunchKernelGGL((k_fft_pass<KIND, AXIS, 2, FROM_SRC, SCALE>), g, b, 0, st, jobs, skip256);
+    return hipGetLastError();
+}
+
+hipError_t fft_poisson_z(const FftJob *jobs, int njobs, int64_t max_groups, int rs, int z256,
+                         hipStream_t st)
+{
+    if (njobs <= 0 || max_groups <= 0) return hipSuccess;
+    if (z256 != kZ256All) {
+        const hipError_t e = launch_pass<kFftPoisson, 2, false, false>(
+            jobs, njobs, max_groups, rs, st, z256 == kZ256Some ? 1 : 0);
+        if (e != hipSuccess || z256 == kZ256None) return e;
+    }
+    // the groups of a 256-point job are those of k_fft_pass (8 real lines each),
+    // so max_groups covers them
+    const dim3 g((unsigned)round_up_xcds(max_groups), (unsigned)njobs), b(kZ256Threads);
+    hipLaunchKernelGGL(k_poisson_z256, g, b, 0, st, jobs);
     return hipGetLastError();
 }
 

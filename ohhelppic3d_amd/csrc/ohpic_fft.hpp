@@ -101,6 +101,14 @@ enum FftPass : int {
 
 int fft_radix_set(const FftAxisDev &ax);
 
+// Which jobs of a Poisson z pass have 256-point z lines (run by the 16 x 16
+// register kernel; the others by the radix-4 LDS stages of k_fft_pass).
+enum Z256Jobs : int { kZ256None = 0, kZ256All = 1, kZ256Some = 2 };
+
+// The fused Poisson z pass (kPassPoissonZ) for every job of the table.
+hipError_t fft_poisson_z(const FftJob *jobs, int njobs, int64_t max_groups, int rs, int z256,
+                         hipStream_t st);
+
 // One axis pass for every job of the device table `jobs`; max_groups is the
 // largest group count over the jobs, rs the largest radix set.
 hipError_t fft_pass(FftPass pass, const FftJob *jobs, int njobs, int64_t max_groups, int rs,

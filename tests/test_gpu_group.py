@@ -273,3 +273,28 @@ def test_autotune_picks_a_timed_candidate(ohp):
     assert r["tile"] in ((4, 4, 4), (8, 8, 8), (8, 8, 4))
     assert r["chunks"] in (1, 2, 4)
     assert r["particle_steps_per_s"] > 0
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_group_with_256_point_z_lines_equals_alone(ohp, precision):
+    # members whose Poisson z pass runs the 256-point register kernel beside
+    # members on the radix-4 LDS stages, in one group: each equals its run
+    # alone bit for bit
+    prec = ohp.FP64 if precision == "fp64" else ohp.MIXED
+    cases = [((8, 8, 256), 2, 7), ((16, 16, 16), 4, 8), ((4, 12, 256), 3, 9)]
+    alone = []
+    for n, ppc, seed in cases:
+        with make(ohp, n, ppc, seed, prec) as sim:
+            sim.step(3)
+            alone.append(snapshot(ohp, sim))
+    sims = [make(ohp, n, ppc, seed, prec) for n, ppc, seed in cases]
+    try:
+        with ohp.Group() as grp:
+            for s in sims:
+                grp.attach(s)
+            grp.step(3)
+        for i, (a, s) in enumerate(zip(alone, sims)):
+            assert_same(snapshot(ohp, s), a, f"{precision} member {i}")
+    finally:
+        for s in sims:
+            s.close()
