@@ -183,6 +183,19 @@ __host__ __device__ inline int xcd_remap(int b, int count)
 }
 __host__ __device__ inline int64_t round_up_xcds(int64_t n) { return (n + kXcds - 1) / kXcds * kXcds; }
 
+// i mod n for i >= -n: one compare per side when i lies within a period of
+// [0, n) (tile and halo indices of grids at least a tile wide), a division
+// only beyond that
+__device__ __forceinline__ int wrap_near(int i, int n)
+{
+    if (i < 0) i += n;
+    if (i >= n) {
+        i -= n;
+        if (i >= n) i %= n;
+    }
+    return i;
+}
+
 __device__ __forceinline__ int wrap_index(int i, int n)
 {
     // i in [-n, 2n) -> [0, n)

@@ -203,7 +203,8 @@ __global__ void __launch_bounds__(kBlock)
             const double *__restrict__ ez = I->e[2];
             for (int l = tid; l < NE; l += kBlock) {
                 const int lx = l % EX, ly = (l / EX) % EY, lz = l / (EX * EY);
-                const int gx = (lox + lx) % nx, gy = (loy + ly) % ny, gz = (loz + lz) % nz;
+                const int gx = wrap_near(lox + lx, nx), gy = wrap_near(loy + ly, ny),
+                          gz = wrap_near(loz + lz, nz);
                 const int64_t g = gx + (int64_t)nx * (gy + (int64_t)ny * gz);
                 sE[l] = gload(ex, g);
                 sE[NE + l] = gload(ey, g);
@@ -466,9 +467,9 @@ __global__ void __launch_bounds__(kBlock)
                 const unsigned long long v = sR[l];
                 if (v == 0ull) continue;
                 const int lx = l % RX, ly = (l / RX) % RY, lz = l / (RX * RY);
-                const int gx = ((lox - 1 + lx) % nx + nx) % nx;
-                const int gy = ((loy - 1 + ly) % ny + ny) % ny;
-                const int gz = ((loz - 1 + lz) % nz + nz) % nz;
+                const int gx = wrap_near(lox - 1 + lx, nx);
+                const int gy = wrap_near(loy - 1 + ly, ny);
+                const int gz = wrap_near(loz - 1 + lz, nz);
                 gatomic_add(&racc[gx + (int64_t)nx * (gy + (int64_t)ny * gz)], v);
             }
         }
