@@ -150,9 +150,15 @@ __device__ __forceinline__ void cic_wf(float r, float &w0, float &w1)
     }
 }
 
+// g mod n: one add or subtract for a cell within a box length of [0, n)
+// (every step of a particle slower than a box length per step), a division
+// only beyond that
 __device__ __forceinline__ int wrap_cell(int g, int n)
 {
-    if (g < 0 || g >= n) g = ((g % n) + n) % n;
+    if (g < 0 || g >= n) {
+        g += g < 0 ? n : -n;
+        if (g < 0 || g >= n) g = ((g % n) + n) % n;
+    }
     return g;
 }
 
