@@ -479,10 +479,12 @@ def test_errors_are_reported(ohp, orc):
 
 # ------------------------------------------------------------------ edge cases
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
-def test_partial_tiles_match_oracle(ohp, orc, precision):
+@pytest.mark.parametrize("n", [(20, 12, 10), (4, 3, 10), (2, 5, 9)])
+def test_partial_tiles_match_oracle(ohp, orc, precision, n):
     # grid lengths that 8-cell tiles do not divide: the last tile of every
-    # axis is partly outside the box
-    n = (20, 12, 10)
+    # axis is partly outside the box; axes of 2-4 cells put a tile's E halo
+    # and deposit block more than one period past the box (their periodic
+    # indices take the division path)
     sp = two_species(orc, n, 4)
     osim = orc.OracleSim(n, sp, 0.1, (0, 0, 0.5), 0)
     prec = ohp.FP64 if precision == "fp64" else ohp.MIXED
