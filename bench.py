@@ -303,7 +303,26 @@ def run_cpu_baseline(seconds: float):
         used[0] = sim.step_threads(1, 0)
 
     medt, kt = timed(threaded, 0.4 * seconds)
+    # the headline's density (64 ppc per species) on a 64^3 grid, threaded:
+    # 3.4e7 particles, a bounded sample of the 256^3 workload's per-particle
+    # loop (its particles no longer fit the host caches)
+    n2, ppc2 = (64, 64, 64), 64
+    rng2 = orc.Pcg32(42, 52)
+    sp2 = []
+    for qm, vt in ((-1.0, 1.0), (0.01, 0.1)):
+        x, y, z, vx, vy, vz = orc.load_uniform_maxwell(rng2, ppc2 * 64 ** 3, n2, vt)
+        sp2.append(dict(qm=qm, q=(1 if qm > 0 else -1) / ppc2, x=x, y=y, z=z, vx=vx, vy=vy,
+                        vz=vz))
+    sim2 = orc.OracleSim(n2, sp2, 0.1, (0, 0, 0.5), 0)
+    del sp2
+    npart2 = 2 * ppc2 * 64 ** 3
+    med2, k2 = timed(lambda: sim2.step_threads(1, 0), 0.25 * seconds)
+    del sim2
     return {"value": npart / medt, "unit": "particle-steps/s", "cores": used[0],
+            "scaled_headline_density": {
+                "value": npart2 / med2, "sample": f"FP64 oracle step, 64^3 grid x 64 ppc x 2 "
+                f"species ({npart2} particles), {used[0]} threads, {k2} steps, median "
+                f"{med2 * 1e3:.0f} ms"},
             "threads": used[0], "kind": "port", "single_thread_value": npart / med1,
             "cpu_model": _cpu_model(), "host": _cpu_share(), "ranks": 1,
             "compiler": "gcc -O2 -fopenmp",

@@ -54,3 +54,6 @@ def test_cpu_baseline_record(orc):
     host = rec["host"]
     assert host["nproc"] >= 1 and 1 <= host["affinity"] <= host["nproc"]
     assert rec["cpu_model"] and rec["cpu_model"] != "unknown"
+    # the headline's density on a scaled grid beside the 32^3 sample
+    assert rec["scaled_headline_density"]["value"] > 0
+    assert "64^3" in rec["scaled_headline_density"]["sample"]
