@@ -64,7 +64,9 @@ def test_fft_r2hc_matches_oracle_and_roundtrips(ohp, orc, n):
                                (11, 13, 14), (256, 4, 6), (5, 1024, 2), (2, 3, 1024),
                                # 256-point z lines (the 16 x 16 register pass): full
                                # groups, and an odd nx*ny with a partial last group
-                               (8, 4, 256), (16, 16, 256), (5, 3, 256)])
+                               (8, 4, 256), (16, 16, 256), (5, 3, 256),
+                               # an x row longer than one block's 2-point sweep (E)
+                               (1024, 2, 3)])
 def test_field_solve_matches_oracle(ohp, orc, n):
     rng = np.random.default_rng(11)
     rho = rng.standard_normal(n[0] * n[1] * n[2])
