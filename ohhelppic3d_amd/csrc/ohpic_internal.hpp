@@ -96,11 +96,10 @@ constexpr int kKeSlots = 2 * kMaxSpecies;  // per item: ke_old[8], ke_new[8]
 #endif
 constexpr int kEfieldMaxBlocks = OHPIC_EFIELD_MAX_BLOCKS;  // rows of e_partial
 
-// k_efield_b: x rows shorter than 256 nodes share a 256-thread block
-// (256 / nx of them); the instance's block count depends on its grid only,
-// so its energy partial sums do not depend on the other instances.
 // k_efield_b: x points per thread (2, with 16-byte accesses, for even nx)
-// and x rows per 256-thread block
+// and x rows per 256-thread block (rows of fewer than 256 threads share one).
+// An instance's block count depends on its grid only, so its energy partial
+// sums do not depend on the other instances of a group.
 __host__ __device__ inline int efield_vec(int nx) { return (nx & 1) ? 1 : 2; }
 __host__ __device__ inline int efield_rows_per_block(int nx)
 {

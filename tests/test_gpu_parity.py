@@ -34,7 +34,9 @@ def test_fft_impulse_response_reference_kat(ohp):
     a[1] = 1.0
     out = ohp.fft3d_r2r(a, n)
     k = np.arange(63)
-    expected = np.where(k <= 31 - 0, np.cos(k / 63 * 2 * np.pi), np.sin(k / 63 * 2 * np.pi))
+    # R2HC of a unit impulse at index 1: X_k = exp(-2 pi i k / 63), so slot
+    # k <= 31 holds Re X_k = cos(2 pi k / 63) and slot k > 31 holds
+    # Im X_(63-k) = sin(2 pi k / 63)
     expected = np.where(k <= (63 + 1) // 2 - 1, np.cos(k / 63 * 2 * np.pi),
                         np.sin(k / 63 * 2 * np.pi))
     assert np.sum(np.abs(out - expected)) <= 1e-8
