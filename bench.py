@@ -170,7 +170,7 @@ def run_gpu(args, workload=None, timing=True, record=0):
                      "alg_bytes_per_launch": alg_bytes},
         "field_solve_avg_ms": solve_ms / max(solve_n, 1),
         "energy": {"total": diag["total_energy"], "field": diag["field_energy"],
-                   "repacks": diag["repacks"],
+                   "repacks": diag["repacks"], "rebalances": diag["rebalances"],
                    "drift_over_timed_steps": (abs(diag["total_energy"] - e_start) / abs(e_start)
                                               if e_start else None)},
     }

@@ -79,6 +79,7 @@ class Diag(C.Structure):
         ("total_energy", C.c_double),
         ("repacks", C.c_int64),
         ("fast_particles", C.c_int64),
+        ("rebalances", C.c_int64),
     ]
 
 

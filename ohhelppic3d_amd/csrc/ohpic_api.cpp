@@ -33,8 +33,9 @@ namespace {
 std::mutex g_err_mu;
 std::string g_err;
 
-constexpr int kNumTimers = 4;
-const char *kTimerNames[kNumTimers] = {"push_fused", "field_solve", "deposit", "repack"};
+constexpr int kNumTimers = 5;
+const char *kTimerNames[kNumTimers] = {"push_fused", "field_solve", "deposit", "repack",
+                                       "rebalance"};
 
 struct TimerSlot {
     hipEvent_t a = nullptr, b = nullptr;
@@ -92,6 +93,7 @@ struct ohpic_ctx {
     bool binned = true;
     int64_t step = 0;
     int64_t repacks = 0;
+    int64_t rebalances = 0;
     int64_t fast = 0;
     int num_cus = 256;
     bool timing = false;
@@ -143,11 +145,14 @@ int dalloc(ohpic_ctx *ctx, T **p, int64_t count)
 
 int64_t round_up(int64_t v, int64_t m) { return (v + m - 1) / m * m; }
 
+double config_slack(const ohpic_config &c) { return c.slack > 0 ? c.slack : 0.15; }
+
 int64_t species_alloc(const ohpic_config &c, int s, int ntiles)
 {
-    const double slack = c.slack > 0 ? c.slack : 0.15;
+    const double slack = config_slack(c);
     return (int64_t)std::ceil((double)c.capacity[s] * (1.0 + slack)) + (int64_t)ntiles * 80 + 64;
 }
+
 
 // Default tile: 8^3 cells when the grid holds at least 512 such tiles, else
 // 4^3 (fewer, larger tiles leave CUs idle: at 32^3 and 48^3 the 4^3 tiles
@@ -330,16 +335,14 @@ int build_work_items(ohpic_ctx *ctx)
 int set_bins(ohpic_ctx *ctx, int s, const std::vector<int64_t> &counts)
 {
     SpeciesHost &S = ctx->sp[s];
-    const double slack = ctx->cfg.slack > 0 ? ctx->cfg.slack : 0.15;
+    const double slack = config_slack(ctx->cfg);
     const int nt = ctx->ntiles;
     S.base.assign(nt, 0);
     S.cap.assign(nt, 0);
     int64_t off = 0;
     for (int t = 0; t < nt; ++t) {
-        const int64_t c = counts[t];
-        int64_t cap = round_up((int64_t)std::ceil((double)c * (1.0 + slack)) + 16, 64);
-        // the particle kernels address a bin with 32-bit byte offsets
-        if (cap >= (int64_t)1 << 29)
+        const int64_t cap = bin_capacity(counts[t], slack);
+        if (cap >= kBinCapLimit)
             return fail(ctx, OHPIC_ENOMEM, "bin %d needs %lld slots (limit 2^29): use smaller tiles",
                         t, (long long)cap);
         S.base[t] = off;
@@ -442,10 +445,10 @@ int bins_to_staging(ohpic_ctx *ctx, int s, int64_t *nlisted)
     HIPCHK(ctx, hipMemcpyAsync(ctx->d_tmp64, off.data(), nt * sizeof(int64_t),
                                hipMemcpyHostToDevice, ctx->st));
     if (ctx->cfg.precision == OHPIC_MIXED)
-        HIPCHK(ctx, launch_bins_to_list_mx(ctx->d_inst, 0, s, ctx->par, ctx->par ^ 1, nt,
+        HIPCHK(ctx, launch_bins_to_list_mx(ctx->d_inst, 0, s, ctx->par, ctx->par ^ 1, 0, nt,
                                            ctx->d_tmp64, ctx->st));
     else
-        HIPCHK(ctx, launch_bins_to_list(ctx->d_inst, 0, s, ctx->par, ctx->par ^ 1, nt,
+        HIPCHK(ctx, launch_bins_to_list(ctx->d_inst, 0, s, ctx->par, ctx->par ^ 1, 0, nt,
                                         ctx->d_tmp64, ctx->st));
     HIPCHK(ctx, hipStreamSynchronize(ctx->st));
     *nlisted = tot;
@@ -457,6 +460,110 @@ int zero_next_counts(ohpic_ctx *ctx)
     for (int s = 0; s < ctx->cfg.nspecies; ++s)
         HIPCHK(ctx, launch_fill_u64(ctx->h.sp[s].count[ctx->par ^ 1], ctx->ntiles, 0ull, ctx->st));
     return OHPIC_OK;
+}
+
+// Restores the slack of the tight bins of species s between steps (buffer par
+// holds the particles, buffer par^1 is free): plan_rebalance picks windows of
+// bins and their new bases and capacities; only the windows' particles move,
+// compacted into buffer par^1 at the new bin bases and copied back.  *moved:
+// windows re-laid out.  Returns 1 when only the whole buffer fits, so that
+// the caller re-bins the species in full.
+int rebalance_species(ohpic_ctx *ctx, int s, int *moved)
+{
+    SpeciesHost &S = ctx->sp[s];
+    SpecDev &D = ctx->h.sp[s];
+    const int nt = ctx->ntiles;
+    const int par = ctx->par;
+    *moved = 0;
+    if ((int)S.cap.size() != nt) return OHPIC_OK;
+    std::vector<int64_t> F, B;
+    int rc = download_words(ctx, D.count[par], F, B);
+    if (rc) return rc;
+    std::vector<int64_t> used(nt);
+    for (int t = 0; t < nt; ++t) used[t] = F[t] + B[t];
+    std::vector<BinWindow> win;
+    std::vector<int64_t> nbase;
+    std::vector<int32_t> ncap;
+    int big = -1;
+    const int plan = plan_rebalance(nt, S.base, S.cap, S.alloc, used, ctx->h.tight_frac,
+                                    config_slack(ctx->cfg), win, nbase, ncap, &big);
+    if (plan == kPlanNone) return OHPIC_OK;
+    if (plan == kPlanWhole) return 1;
+    if (plan == kPlanTooBig)
+        return fail(ctx, OHPIC_ENOMEM, "bin %d needs %lld slots (limit 2^29): use smaller tiles",
+                    big, (long long)bin_capacity(used[big], config_slack(ctx->cfg)));
+    // the list offsets (new bases) and count words (one front run per bin) of
+    // all windows side by side
+    std::vector<int64_t> off;
+    std::vector<unsigned long long> words;
+    for (const BinWindow &w : win)
+        for (int t = w.a; t <= w.b; ++t) {
+            off.push_back(nbase[t]);
+            words.push_back((unsigned long long)used[t]);
+        }
+    HIPCHK(ctx, hipMemcpyAsync(ctx->d_tmp64, off.data(), off.size() * sizeof(int64_t),
+                               hipMemcpyHostToDevice, ctx->st));
+    const bool mixed = ctx->cfg.precision == OHPIC_MIXED;
+    size_t k0 = 0;
+    for (const BinWindow &w : win) {
+        const int a = w.a, b = w.b, nw = b - a + 1;
+        // compact the window's bins into buffer par^1 at their new bases ...
+        if (mixed)
+            HIPCHK(ctx, launch_bins_to_list_mx(ctx->d_inst, 0, s, par, par ^ 1, a, nw,
+                                               ctx->d_tmp64 + k0, ctx->st));
+        else
+            HIPCHK(ctx, launch_bins_to_list(ctx->d_inst, 0, s, par, par ^ 1, a, nw,
+                                            ctx->d_tmp64 + k0, ctx->st));
+        // ... copy that span back into buffer par ...
+        const int64_t lo = nbase[a], n = nbase[b] + used[b] - lo;
+        auto back = [&](void *dst, const void *src, size_t es) {
+            return hipMemcpyAsync((char *)dst + lo * es, (const char *)src + lo * es, n * es,
+                                  hipMemcpyDeviceToDevice, ctx->st);
+        };
+        for (int c = mixed ? 1 : 0; c < (mixed ? 7 : 6); ++c)
+            HIPCHK(ctx, back(D.buf[par][c], D.buf[par ^ 1][c], mixed ? 4 : 8));
+        if (D.pid[par]) HIPCHK(ctx, back(D.pid[par], D.pid[par ^ 1], 8));
+        // ... and set the window's bases, capacities and counts
+        HIPCHK(ctx, hipMemcpyAsync(D.bin_base + a, nbase.data() + a, nw * sizeof(int64_t),
+                                   hipMemcpyHostToDevice, ctx->st));
+        HIPCHK(ctx, hipMemcpyAsync(D.bin_cap + a, ncap.data() + a, nw * sizeof(int32_t),
+                                   hipMemcpyHostToDevice, ctx->st));
+        HIPCHK(ctx, hipMemcpyAsync(D.count[par] + a, words.data() + k0,
+                                   nw * sizeof(unsigned long long), hipMemcpyHostToDevice,
+                                   ctx->st));
+        k0 += (size_t)nw;
+    }
+    HIPCHK(ctx, hipStreamSynchronize(ctx->st));
+    S.base.swap(nbase);
+    S.cap.swap(ncap);
+    *moved = (int)win.size();
+    return OHPIC_OK;
+}
+
+// After a step that left a bin tight: rebalance every species (a species
+// whose bins only fit as a whole is re-binned in full, which counts as a
+// repack).
+int rebalance_bins(ohpic_ctx *ctx)
+{
+    timer_begin(ctx, 4);
+    bool moved_any = false, full = false;
+    for (int s = 0; s < ctx->cfg.nspecies; ++s) {
+        int moved = 0;
+        int rc = rebalance_species(ctx, s, &moved);
+        if (rc < 0) return rc;
+        if (rc == 1) {
+            int64_t n = 0;
+            if ((rc = bins_to_staging(ctx, s, &n)) || (rc = bin_from_staging(ctx, s, n, 0)))
+                return rc;
+            ctx->repacks += 1;
+            full = true;
+        }
+        moved_any = moved_any || moved > 0;
+    }
+    timer_end(ctx, 4);
+    HIPCHK(ctx, hipStreamSynchronize(ctx->st));
+    if (moved_any) ctx->rebalances += 1;
+    return full ? build_work_items(ctx) : OHPIC_OK;
 }
 
 // Retry of a step whose exchange overflowed a bin: run the kernel again in
@@ -564,7 +671,8 @@ int one_step(ohpic_ctx *ctx)
     HIPCHK(ctx, launch_particles(kModeFused, ctx->cfg.precision, ctx->cfg.weight_order, ctx->T,
                                  ctx->d_inst, ctx->d_items, ctx->nitems, parity_of(ctx), ctx->st));
     timer_end(ctx, 0);
-    HIPCHK(ctx, launch_end_step_b(ctx->d_inst, nullptr, 1, nullptr, ctx->st));
+    // the parity word is only read here (no out: no flip)
+    HIPCHK(ctx, launch_end_step_b(ctx->d_inst, ctx->d_par01 + ctx->par, 1, nullptr, ctx->st));
     if ((rc = read_flags(ctx))) return rc;
     timers_collect(ctx);
     rc = absorb_flags(ctx, ctx->h_flags);
@@ -577,6 +685,9 @@ int one_step(ohpic_ctx *ctx)
     }
     ctx->step += 1;
     ctx->rho_valid = true;
+    // the step left a bin short of free slots: restore its slack now, before
+    // an exchange overflows it
+    if (rc == 0 && ctx->h_flags[kFlagTight]) return rebalance_bins(ctx);
     return OHPIC_OK;
 }
 
@@ -1061,6 +1172,11 @@ int group_step(ohpic_group *g, int nsteps)
                 // leaves E one step behind: the rest run alone, each with its
                 // own field solve
                 for (; r < k && rc == OHPIC_OK; ++r) rc = one_step(m);
+                // a step of the sequence left a bin tight
+                bool tight = false;
+                for (int s = 0; s < k; ++s)
+                    tight = tight || g->h_flags[((size_t)s * n + i) * kFlagSlots + kFlagTight];
+                if (rc == OHPIC_OK && tight) rc = rebalance_bins(m);
             }
             if (rc < 0) {
                 m->failed = true;
@@ -1236,6 +1352,7 @@ int ohpic_create(const ohpic_config *cfg, ohpic_ctx **out)
     h.dt = c.dt;
     h.fix_scale = std::ldexp(1.0, ctx->charge_bits);
     h.fix_inv = std::ldexp(1.0, -ctx->charge_bits);
+    h.tight_frac = tight_fraction(config_slack(c));
     const int nt = ctx->ntiles;
     for (int s = 0; s < c.nspecies; ++s) {
         SpeciesHost &S = ctx->sp[s];
@@ -1995,6 +2112,7 @@ int ohpic_diagnostics(ohpic_ctx *ctx, ohpic_diag *out)
     out->net_charge = ctx->h_diag[kDiagQnet];
     out->total_energy = ke + out->field_energy;
     out->repacks = ctx->repacks;
+    out->rebalances = ctx->rebalances;
     out->fast_particles = ctx->fast;
     return OHPIC_OK;
 }

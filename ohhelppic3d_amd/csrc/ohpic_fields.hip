@@ -296,10 +296,12 @@ __global__ void k_begin_step_b(const InstDev *__restrict__ insts, const int *__r
 
 // End of a step: blocks x < kKeSlots reduce the kinetic-energy slots (as
 // k_reduce_ke_b), block kKeSlots the field-energy partials of k_efield_b (as
-// k_reduce_e_b) and, when out is not null, finishes the instance's step in a
-// group: out[i*kFlagSlots + k] := flags[k] of instance i, and the parity flips
-// unless a bin overflowed (the host then redoes that step in list mode from
-// the unchanged buffer).
+// k_reduce_e_b), then, unless a bin overflowed, looks for tight bins in the
+// step's new buffer (parity ^ 1) and raises kFlagTight for one (the host then
+// restores the slack of its neighbourhood before the next step).  When out is
+// not null it finishes the instance's step in a group: out[i*kFlagSlots + k]
+// := flags[k] of instance i, and parity[i] flips unless a bin overflowed (the
+// host then redoes that step in list mode from the unchanged buffer).
 __global__ void __launch_bounds__(kSumBlock) k_end_step_b(const InstDev *__restrict__ insts,
                                                     int *__restrict__ parity,
                                                     unsigned long long *__restrict__ out)
@@ -317,6 +319,23 @@ __global__ void __launch_bounds__(kSumBlock) k_end_step_b(const InstDev *__restr
         const double v = block_ordered_sum(I->e_partial, count, 3, k, red);
         if (threadIdx.x == 0) I->diag[kDiagWe + k] = v;
     }
+    __shared__ int tight;
+    if (threadIdx.x == 0) tight = 0;
+    __syncthreads();
+    if (I->flags[kFlagOverflow] == 0ull) {
+        const int nb = parity[blockIdx.y] ^ 1;
+        bool t_any = false;
+        for (int s = 0; s < I->nspecies; ++s) {
+            const SpecDev &S = I->sp[s];
+            for (int t = threadIdx.x; t < I->ntiles; t += blockDim.x) {
+                const unsigned long long c = S.count[nb][t];
+                t_any = t_any || bin_tight(S.bin_cap[t], bin_front(c) + bin_back(c), I->tight_frac);
+            }
+        }
+        if (t_any) tight = 1;
+    }
+    __syncthreads();
+    if (tight && threadIdx.x == 0) I->flags[kFlagTight] = 1ull;
     if (out && threadIdx.x == 0) {
         const int i = blockIdx.y;
         const unsigned long long *f = I->flags;

@@ -20,6 +20,8 @@
 
 #include <cstdint>
 
+#include "ohpic_bins.hpp"
+
 namespace ohpic {
 
 constexpr int kMaxSpecies = 8;
@@ -40,7 +42,8 @@ enum FlagSlot : int {
     kFlagOverflow = 0,  // particles that found their destination bin full
     kFlagFast = 1,      // particles whose new position left the deposit halo
     kFlagRange = 2,     // positions outside [0, n) (after the wraps when wrapping)
-    kFlagSlots = 3
+    kFlagTight = 3,     // a step left a bin short of free slots (bin_tight)
+    kFlagSlots = 4
 };
 
 struct SpecDev {
@@ -68,6 +71,7 @@ struct InstDev {
     double b0[3];
     double fix_scale;      // 2^charge_bits
     double fix_inv;        // 2^-charge_bits
+    double tight_frac;     // free-slot fraction of a tight bin (bin_tight)
     SpecDev sp[kMaxSpecies];
     unsigned long long *rho_acc[2];
     double *rho;
@@ -159,6 +163,7 @@ __host__ __device__ __forceinline__ int bin_front(unsigned long long w)
 }
 
 __host__ __device__ __forceinline__ int bin_back(unsigned long long w) { return (int)(w >> 32); }
+
 
 // slot of the v-th particle of a bin with front run F and back run B
 __device__ __forceinline__ int bin_slot(int v, int F, int B, int cap)

@@ -41,7 +41,7 @@ hipError_t launch_bin_count(const InstDev *insts, int inst, int s, int lbuf, int
                             const int T[3], int wrap, hipStream_t st);
 hipError_t launch_bin_scatter(const InstDev *insts, int inst, int s, int lbuf, int64_t n,
                               const int T[3], int wrap, hipStream_t st);
-hipError_t launch_bins_to_list(const InstDev *insts, int inst, int s, int src, int dst,
+hipError_t launch_bins_to_list(const InstDev *insts, int inst, int s, int src, int dst, int t0,
                                int ntiles, const int64_t *list_off, hipStream_t st);
 hipError_t launch_cell_counts(const InstDev *insts, int inst, int s, int par, int ntiles,
                               int32_t *counts, hipStream_t st);
@@ -68,7 +68,7 @@ hipError_t launch_bin_count_mx(const InstDev *insts, int inst, int s, int lbuf, 
                                const int T[3], hipStream_t st);
 hipError_t launch_bin_scatter_mx(const InstDev *insts, int inst, int s, int lbuf, int64_t n,
                                  const int T[3], hipStream_t st);
-hipError_t launch_bins_to_list_mx(const InstDev *insts, int inst, int s, int src, int dst,
+hipError_t launch_bins_to_list_mx(const InstDev *insts, int inst, int s, int src, int dst, int t0,
                                   int ntiles, const int64_t *list_off, hipStream_t st);
 hipError_t launch_cell_counts_mx(const InstDev *insts, int inst, int s, int par, int ntiles,
                                  int32_t *counts, const int T[3], hipStream_t st);
@@ -94,9 +94,10 @@ hipError_t launch_efield_b(const InstDev *insts, int ninst, int max_blocks, bool
 hipError_t launch_begin_step_b(const InstDev *insts, const int *parity, int ninst,
                                int64_t max_ng, int max_ntiles, unsigned int *queue,
                                hipStream_t st);
-// launch_reduce_ke_b, the field-energy reduction of launch_efield_b and (out
-// not null, group steps) flags to out[i * kFlagSlots + k] with a parity flip
-// unless instance i overflowed a bin, in one launch at the end of a step
+// launch_reduce_ke_b, the field-energy reduction of launch_efield_b, the
+// tight-bin check of the new buffer (parity ^ 1) and (out not null, group
+// steps) flags to out[i * kFlagSlots + k] with a parity flip unless instance i
+// overflowed a bin, in one launch at the end of a step
 hipError_t launch_end_step_b(const InstDev *insts, int *parity, int ninst,
                              unsigned long long *out, hipStream_t st);
 // zero the next buffer's bin words and charge grid, the flags, and *queue

@@ -542,20 +542,20 @@ __global__ void k_bin_scatter(const InstDev *__restrict__ insts, int inst, int s
         gatomic_add(&I->flags[kFlagOverflow], 1ull);
 }
 
-// Compact both runs of every bin of buffer src into a contiguous list of
-// buffer dst at list_off[t].  One workgroup per bin.
+// Compact both runs of bins t0, t0 + 1, ... of buffer src into contiguous
+// lists of buffer dst, bin t0 + i at list_off[i].  One workgroup per bin.
 __global__ void k_bins_to_list(const InstDev *__restrict__ insts, int inst, int s, int src,
-                               int dst, const int64_t *__restrict__ list_off)
+                               int dst, int t0, const int64_t *__restrict__ list_off)
 {
     const InstDev *I = insts + inst;
     const SpecDev &S = I->sp[s];
-    const int t = blockIdx.x;
+    const int t = t0 + blockIdx.x;
     const unsigned long long w = S.count[src][t];
     const int cap = S.bin_cap[t];
     const int F = min(bin_front(w), cap);
     const int B = min(bin_back(w), cap - F);
     const int64_t ib = S.bin_base[t];
-    const int64_t ob = list_off[t];
+    const int64_t ob = list_off[blockIdx.x];
     const bool ids = I->track_ids != 0;
     for (int v = threadIdx.x; v < F + B; v += blockDim.x) {
         const int64_t o = ib + bin_slot(v, F, B, cap);
@@ -734,12 +734,12 @@ hipError_t launch_bin_scatter(const InstDev *insts, int inst, int s, int lbuf, i
     return hipGetLastError();
 }
 
-hipError_t launch_bins_to_list(const InstDev *insts, int inst, int s, int src, int dst,
+hipError_t launch_bins_to_list(const InstDev *insts, int inst, int s, int src, int dst, int t0,
                                int ntiles, const int64_t *list_off, hipStream_t st)
 {
     if (ntiles <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_bins_to_list, dim3(ntiles), dim3(256), 0, st, insts, inst, s, src, dst,
-                       list_off);
+                       t0, list_off);
     return hipGetLastError();
 }
 

@@ -50,6 +50,7 @@ module m_ohpic_c
         real(c_double) :: total_energy
         integer(c_int64_t) :: repacks
         integer(c_int64_t) :: fast_particles
+        integer(c_int64_t) :: rebalances
     end type
 
     interface
