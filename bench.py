@@ -171,6 +171,7 @@ def run_gpu(args, workload=None, timing=True, record=0):
         "field_solve_avg_ms": solve_ms / max(solve_n, 1),
         "energy": {"total": diag["total_energy"], "field": diag["field_energy"],
                    "repacks": diag["repacks"], "rebalances": diag["rebalances"],
+                   "compactions": diag["compactions"],
                    "drift_over_timed_steps": (abs(diag["total_energy"] - e_start) / abs(e_start)
                                               if e_start else None)},
     }

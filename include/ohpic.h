@@ -89,7 +89,9 @@ typedef struct {
     double total_energy;                   /* (ke_old+ke_new)/2 summed + field_energy */
     int64_t repacks;                       /* full re-bins so far */
     int64_t fast_particles;                /* particles that moved > 1 cell in a step */
-    int64_t rebalances;                    /* re-layouts of bins short of slack so far */
+    int64_t rebalances;                    /* passes that moved bins short of slack */
+    int64_t compactions;                   /* compactions of every species' bins (a
+                                              relocation pool ran out) */
 } ohpic_diag;
 
 /* --- context --------------------------------------------------------------- */

@@ -80,6 +80,7 @@ class Diag(C.Structure):
         ("repacks", C.c_int64),
         ("fast_particles", C.c_int64),
         ("rebalances", C.c_int64),
+        ("compactions", C.c_int64),
     ]
 
 

@@ -33,9 +33,9 @@ namespace {
 std::mutex g_err_mu;
 std::string g_err;
 
-constexpr int kNumTimers = 5;
+constexpr int kNumTimers = 6;
 const char *kTimerNames[kNumTimers] = {"push_fused", "field_solve", "deposit", "repack",
-                                       "rebalance"};
+                                       "rebalance", "compact"};
 
 struct TimerSlot {
     hipEvent_t a = nullptr, b = nullptr;
@@ -50,6 +50,7 @@ struct SpeciesHost {
     int64_t n = 0;         // particles loaded
     std::vector<int64_t> base;
     std::vector<int32_t> cap;
+    int64_t pool = 0;      // first free element of the relocation pool
 };
 
 }  // namespace
@@ -94,6 +95,7 @@ struct ohpic_ctx {
     int64_t step = 0;
     int64_t repacks = 0;
     int64_t rebalances = 0;
+    int64_t compactions = 0;
     int64_t fast = 0;
     int num_cus = 256;
     bool timing = false;
@@ -147,10 +149,13 @@ int64_t round_up(int64_t v, int64_t m) { return (v + m - 1) / m * m; }
 
 double config_slack(const ohpic_config &c) { return c.slack > 0 ? c.slack : 0.15; }
 
+// Elements per buffer component: the bins of capacity particles with their
+// slack and per-bin round-ups, and the relocation pool of tight bins.
 int64_t species_alloc(const ohpic_config &c, int s, int ntiles)
 {
     const double slack = config_slack(c);
-    return (int64_t)std::ceil((double)c.capacity[s] * (1.0 + slack)) + (int64_t)ntiles * 80 + 64;
+    return (int64_t)std::ceil((double)c.capacity[s] * (1.0 + slack * (1.0 + kPoolSlackShare))) +
+           (int64_t)ntiles * 80 + 64;
 }
 
 
@@ -210,7 +215,7 @@ int64_t estimate_bytes(const ohpic_config &c, int num_cus)
     b += ng * (2 * 8 + 5 * 8 + 4);           // int64 charge x2, rho, phi, E, cell counts
     b += nt * (8 + 8 + 4 + 4) + 8;           // host-transfer scratch, cursor
     b += 3 * kEfieldMaxBlocks * 8;           // field-energy partials
-    b += kDiagSlots * 8 + kFlagSlots * 8 + (int64_t)sizeof(InstDev) + 2 * 4 +
+    b += kDiagSlots * 8 + kFlagSlots * 8 + kTightListCap * 8 + (int64_t)sizeof(InstDev) + 2 * 4 +
          (int64_t)sizeof(FftJob);
     b += (int64_t)(c.nx + c.ny + c.nz) * (16 + 8);  // twiddles, Green's-function tables
     b += max_work_items(c, nt, num_cus) * ((int64_t)sizeof(WorkItem) + kKeSlots * 8);
@@ -353,6 +358,7 @@ int set_bins(ohpic_ctx *ctx, int s, const std::vector<int64_t> &counts)
         return fail(ctx, OHPIC_ENOMEM,
                     "species %d: bins need %lld elements, buffer holds %lld (raise capacity)", s,
                     (long long)off, (long long)S.alloc);
+    S.pool = off;
     HIPCHK(ctx, hipMemcpyAsync(ctx->h.sp[s].bin_base, S.base.data(), nt * sizeof(int64_t),
                                hipMemcpyHostToDevice, ctx->st));
     HIPCHK(ctx, hipMemcpyAsync(ctx->h.sp[s].bin_cap, S.cap.data(), nt * sizeof(int32_t),
@@ -445,10 +451,10 @@ int bins_to_staging(ohpic_ctx *ctx, int s, int64_t *nlisted)
     HIPCHK(ctx, hipMemcpyAsync(ctx->d_tmp64, off.data(), nt * sizeof(int64_t),
                                hipMemcpyHostToDevice, ctx->st));
     if (ctx->cfg.precision == OHPIC_MIXED)
-        HIPCHK(ctx, launch_bins_to_list_mx(ctx->d_inst, 0, s, ctx->par, ctx->par ^ 1, 0, nt,
+        HIPCHK(ctx, launch_bins_to_list_mx(ctx->d_inst, 0, s, ctx->par, ctx->par ^ 1, nullptr, 0, nt,
                                            ctx->d_tmp64, ctx->st));
     else
-        HIPCHK(ctx, launch_bins_to_list(ctx->d_inst, 0, s, ctx->par, ctx->par ^ 1, 0, nt,
+        HIPCHK(ctx, launch_bins_to_list(ctx->d_inst, 0, s, ctx->par, ctx->par ^ 1, nullptr, 0, nt,
                                         ctx->d_tmp64, ctx->st));
     HIPCHK(ctx, hipStreamSynchronize(ctx->st));
     *nlisted = tot;
@@ -462,108 +468,160 @@ int zero_next_counts(ohpic_ctx *ctx)
     return OHPIC_OK;
 }
 
-// Restores the slack of the tight bins of species s between steps (buffer par
-// holds the particles, buffer par^1 is free): plan_rebalance picks windows of
-// bins and their new bases and capacities; only the windows' particles move,
-// compacted into buffer par^1 at the new bin bases and copied back.  *moved:
-// windows re-laid out.  Returns 1 when only the whole buffer fits, so that
-// the caller re-bins the species in full.
-int rebalance_species(ohpic_ctx *ctx, int s, int *moved)
+// Moves the listed bins of species s (cands) into the relocation pool with
+// fresh slack (plan_relocation), or only its tight ones when the pool cannot
+// take them all: their particles are compacted into the pool slots within
+// buffer par (from their old slots, which stay unused until the next full
+// re-bin), then the bins take their new bases, capacities and counts.
+// Returns 1 when the pool cannot take even the tight bins, so that the caller
+// re-bins the species in full.
+int relocate_bins(ohpic_ctx *ctx, int s, std::vector<BinCand> cands)
 {
     SpeciesHost &S = ctx->sp[s];
-    SpecDev &D = ctx->h.sp[s];
-    const int nt = ctx->ntiles;
-    const int par = ctx->par;
-    *moved = 0;
-    if ((int)S.cap.size() != nt) return OHPIC_OK;
-    std::vector<int64_t> F, B;
-    int rc = download_words(ctx, D.count[par], F, B);
-    if (rc) return rc;
-    std::vector<int64_t> used(nt);
-    for (int t = 0; t < nt; ++t) used[t] = F[t] + B[t];
-    std::vector<BinWindow> win;
-    std::vector<int64_t> nbase;
-    std::vector<int32_t> ncap;
+    std::vector<BinMove> moves;
+    int64_t pool = S.pool;
     int big = -1;
-    const int plan = plan_rebalance(nt, S.base, S.cap, S.alloc, used, ctx->h.tight_frac,
-                                    config_slack(ctx->cfg), win, nbase, ncap, &big);
-    if (plan == kPlanNone) return OHPIC_OK;
+    const double slack = config_slack(ctx->cfg);
+    int plan = plan_relocation(cands, slack, &pool, S.alloc, moves, &big);
+    if (plan == kPlanWhole) {
+        cands.erase(std::remove_if(cands.begin(), cands.end(),
+                                   [&](const BinCand &c) {
+                                       return !bin_tight(S.cap[c.tile], c.used,
+                                                         ctx->h.tight_frac);
+                                   }),
+                    cands.end());
+        if (!cands.empty()) plan = plan_relocation(cands, slack, &pool, S.alloc, moves, &big);
+        if (cands.empty()) return OHPIC_OK;
+    }
     if (plan == kPlanWhole) return 1;
     if (plan == kPlanTooBig)
-        return fail(ctx, OHPIC_ENOMEM, "bin %d needs %lld slots (limit 2^29): use smaller tiles",
-                    big, (long long)bin_capacity(used[big], config_slack(ctx->cfg)));
-    // the list offsets (new bases) and count words (one front run per bin) of
-    // all windows side by side
-    std::vector<int64_t> off;
-    std::vector<unsigned long long> words;
-    for (const BinWindow &w : win)
-        for (int t = w.a; t <= w.b; ++t) {
-            off.push_back(nbase[t]);
-            words.push_back((unsigned long long)used[t]);
-        }
-    HIPCHK(ctx, hipMemcpyAsync(ctx->d_tmp64, off.data(), off.size() * sizeof(int64_t),
-                               hipMemcpyHostToDevice, ctx->st));
-    const bool mixed = ctx->cfg.precision == OHPIC_MIXED;
-    size_t k0 = 0;
-    for (const BinWindow &w : win) {
-        const int a = w.a, b = w.b, nw = b - a + 1;
-        // compact the window's bins into buffer par^1 at their new bases ...
-        if (mixed)
-            HIPCHK(ctx, launch_bins_to_list_mx(ctx->d_inst, 0, s, par, par ^ 1, a, nw,
-                                               ctx->d_tmp64 + k0, ctx->st));
-        else
-            HIPCHK(ctx, launch_bins_to_list(ctx->d_inst, 0, s, par, par ^ 1, a, nw,
-                                            ctx->d_tmp64 + k0, ctx->st));
-        // ... copy that span back into buffer par ...
-        const int64_t lo = nbase[a], n = nbase[b] + used[b] - lo;
-        auto back = [&](void *dst, const void *src, size_t es) {
-            return hipMemcpyAsync((char *)dst + lo * es, (const char *)src + lo * es, n * es,
-                                  hipMemcpyDeviceToDevice, ctx->st);
-        };
-        for (int c = mixed ? 1 : 0; c < (mixed ? 7 : 6); ++c)
-            HIPCHK(ctx, back(D.buf[par][c], D.buf[par ^ 1][c], mixed ? 4 : 8));
-        if (D.pid[par]) HIPCHK(ctx, back(D.pid[par], D.pid[par ^ 1], 8));
-        // ... and set the window's bases, capacities and counts
-        HIPCHK(ctx, hipMemcpyAsync(D.bin_base + a, nbase.data() + a, nw * sizeof(int64_t),
-                                   hipMemcpyHostToDevice, ctx->st));
-        HIPCHK(ctx, hipMemcpyAsync(D.bin_cap + a, ncap.data() + a, nw * sizeof(int32_t),
-                                   hipMemcpyHostToDevice, ctx->st));
-        HIPCHK(ctx, hipMemcpyAsync(D.count[par] + a, words.data() + k0,
-                                   nw * sizeof(unsigned long long), hipMemcpyHostToDevice,
-                                   ctx->st));
-        k0 += (size_t)nw;
+        return fail(ctx, OHPIC_ENOMEM, "bin %d needs 2^29 slots or more: use smaller tiles", big);
+    const int nm = (int)moves.size();
+    std::vector<int32_t> tiles(nm), cap(nm);
+    std::vector<int64_t> base(nm), used(nm);
+    for (int k = 0; k < nm; ++k) {
+        tiles[k] = moves[k].tile;
+        base[k] = moves[k].base;
+        cap[k] = moves[k].cap;
+        used[k] = moves[k].used;
     }
+    HIPCHK(ctx, hipMemcpyAsync(ctx->d_tmp32a, tiles.data(), nm * sizeof(int32_t),
+                               hipMemcpyHostToDevice, ctx->st));
+    HIPCHK(ctx, hipMemcpyAsync(ctx->d_tmp64, base.data(), nm * sizeof(int64_t),
+                               hipMemcpyHostToDevice, ctx->st));
+    HIPCHK(ctx, hipMemcpyAsync(ctx->d_tmp32b, cap.data(), nm * sizeof(int32_t),
+                               hipMemcpyHostToDevice, ctx->st));
+    HIPCHK(ctx, hipMemcpyAsync(ctx->d_tmp64b, used.data(), nm * sizeof(int64_t),
+                               hipMemcpyHostToDevice, ctx->st));
+    const int par = ctx->par;
+    // the particles first (the compaction reads the old bases), then the layout
+    if (ctx->cfg.precision == OHPIC_MIXED)
+        HIPCHK(ctx, launch_bins_to_list_mx(ctx->d_inst, 0, s, par, par, ctx->d_tmp32a, 0, nm,
+                                           ctx->d_tmp64, ctx->st));
+    else
+        HIPCHK(ctx, launch_bins_to_list(ctx->d_inst, 0, s, par, par, ctx->d_tmp32a, 0, nm,
+                                        ctx->d_tmp64, ctx->st));
+    HIPCHK(ctx, launch_set_bins(ctx->d_inst, 0, s, par, ctx->d_tmp32a, ctx->d_tmp64,
+                                ctx->d_tmp32b, ctx->d_tmp64b, nm, ctx->st));
     HIPCHK(ctx, hipStreamSynchronize(ctx->st));
-    S.base.swap(nbase);
-    S.cap.swap(ncap);
-    *moved = (int)win.size();
+    for (const BinMove &m : moves) {
+        S.base[m.tile] = m.base;
+        S.cap[m.tile] = m.cap;
+    }
+    S.pool = pool;
     return OHPIC_OK;
 }
 
-// After a step that left a bin tight: rebalance every species (a species
-// whose bins only fit as a whole is re-binned in full, which counts as a
-// repack).
-int rebalance_bins(ohpic_ctx *ctx)
+// Compacts every species into buffer par^1, each bin laid out as a full
+// re-bin lays it out for its count (one read and one write of every particle;
+// no particle changes bin), then makes that buffer the current one: the
+// charge grids swap with it, and every relocation pool starts empty.
+int compact_bins(ohpic_ctx *ctx)
+{
+    timer_begin(ctx, 5);
+    const int nt = ctx->ntiles, par = ctx->par;
+    const double slack = config_slack(ctx->cfg);
+    int rc;
+    for (int s = 0; s < ctx->cfg.nspecies; ++s) {
+        if ((int)ctx->sp[s].cap.size() != nt) continue;
+        std::vector<int64_t> F, B;
+        if ((rc = download_words(ctx, ctx->h.sp[s].count[par], F, B))) return rc;
+        std::vector<int64_t> used(nt), off(nt);
+        std::vector<unsigned long long> words(nt);
+        int64_t o = 0;
+        for (int t = 0; t < nt; ++t) {
+            used[t] = F[t] + B[t];
+            words[t] = (unsigned long long)used[t];  // one front run
+            off[t] = o;
+            o += bin_capacity(used[t], slack);
+        }
+        HIPCHK(ctx, hipMemcpyAsync(ctx->d_tmp64, off.data(), nt * sizeof(int64_t),
+                                   hipMemcpyHostToDevice, ctx->st));
+        // the copy reads the current layout; set_bins then uploads the new one
+        // (the same bases as off)
+        if (ctx->cfg.precision == OHPIC_MIXED)
+            HIPCHK(ctx, launch_bins_to_list_mx(ctx->d_inst, 0, s, par, par ^ 1, nullptr, 0, nt,
+                                               ctx->d_tmp64, ctx->st));
+        else
+            HIPCHK(ctx, launch_bins_to_list(ctx->d_inst, 0, s, par, par ^ 1, nullptr, 0, nt,
+                                            ctx->d_tmp64, ctx->st));
+        if ((rc = set_bins(ctx, s, used))) return rc;
+        HIPCHK(ctx, hipMemcpyAsync(ctx->h.sp[s].count[par ^ 1], words.data(),
+                                   nt * sizeof(unsigned long long), hipMemcpyHostToDevice,
+                                   ctx->st));
+        HIPCHK(ctx, hipStreamSynchronize(ctx->st));
+    }
+    ctx->par ^= 1;
+    std::swap(ctx->h.rho_acc[0], ctx->h.rho_acc[1]);
+    timer_end(ctx, 5);
+    ctx->compactions += 1;
+    if ((rc = sync_upload_inst(ctx))) return rc;
+    return build_work_items(ctx);
+}
+
+// After a step that left a bin tight: move the bins k_end_step_b listed
+// (`listed` of them, from tight_list; when more than the list holds, the
+// listed bins are found from every species' counts).  When a species' pool
+// cannot take its tight bins, every species is compacted instead.
+int rebalance_bins(ohpic_ctx *ctx, unsigned long long listed)
 {
     timer_begin(ctx, 4);
-    bool moved_any = false, full = false;
-    for (int s = 0; s < ctx->cfg.nspecies; ++s) {
-        int moved = 0;
-        int rc = rebalance_species(ctx, s, &moved);
-        if (rc < 0) return rc;
-        if (rc == 1) {
-            int64_t n = 0;
-            if ((rc = bins_to_staging(ctx, s, &n)) || (rc = bin_from_staging(ctx, s, n, 0)))
-                return rc;
-            ctx->repacks += 1;
-            full = true;
+    const int ns = ctx->cfg.nspecies, nt = ctx->ntiles;
+    std::vector<std::vector<BinCand>> cands(ns);
+    int rc;
+    if (listed <= (unsigned long long)kTightListCap) {
+        std::vector<unsigned long long> e(listed);
+        HIPCHK(ctx, hipMemcpyAsync(e.data(), ctx->h.tight_list, listed * sizeof(e[0]),
+                                   hipMemcpyDeviceToHost, ctx->st));
+        HIPCHK(ctx, hipStreamSynchronize(ctx->st));
+        for (unsigned long long v : e)
+            cands[tight_entry_species(v)].push_back({tight_entry_tile(v), tight_entry_used(v)});
+        // tile order: the pool layout does not depend on the scan's order
+        for (auto &c : cands)
+            std::sort(c.begin(), c.end(),
+                      [](const BinCand &a, const BinCand &b) { return a.tile < b.tile; });
+    } else {
+        for (int s = 0; s < ns; ++s) {
+            if ((int)ctx->sp[s].cap.size() != nt) continue;
+            std::vector<int64_t> F, B;
+            if ((rc = download_words(ctx, ctx->h.sp[s].count[ctx->par], F, B))) return rc;
+            for (int t = 0; t < nt; ++t)
+                if (bin_listed(ctx->sp[s].cap[t], F[t] + B[t], ctx->h.tight_frac))
+                    cands[s].push_back({t, F[t] + B[t]});
         }
-        moved_any = moved_any || moved > 0;
+    }
+    bool full = false;
+    for (int s = 0; s < ns && !full; ++s) {
+        if (cands[s].empty()) continue;
+        rc = relocate_bins(ctx, s, cands[s]);
+        if (rc < 0) return rc;
+        full = rc == 1;
     }
     timer_end(ctx, 4);
     HIPCHK(ctx, hipStreamSynchronize(ctx->st));
-    if (moved_any) ctx->rebalances += 1;
-    return full ? build_work_items(ctx) : OHPIC_OK;
+    ctx->rebalances += 1;
+    // a relocation pool ran out: compact every species (fresh pools)
+    return full ? compact_bins(ctx) : OHPIC_OK;
 }
 
 // Retry of a step whose exchange overflowed a bin: run the kernel again in
@@ -687,7 +745,8 @@ int one_step(ohpic_ctx *ctx)
     ctx->rho_valid = true;
     // the step left a bin short of free slots: restore its slack now, before
     // an exchange overflows it
-    if (rc == 0 && ctx->h_flags[kFlagTight]) return rebalance_bins(ctx);
+    if (rc == 0 && ctx->h_flags[kFlagTight])
+        return rebalance_bins(ctx, ctx->h_flags[kFlagListed]);
     return OHPIC_OK;
 }
 
@@ -1172,11 +1231,15 @@ int group_step(ohpic_group *g, int nsteps)
                 // leaves E one step behind: the rest run alone, each with its
                 // own field solve
                 for (; r < k && rc == OHPIC_OK; ++r) rc = one_step(m);
-                // a step of the sequence left a bin tight
+                // every step of the sequence went through and one left a bin
+                // tight: move the bins the last step listed (a member that
+                // finished alone or was re-binned has already seen to its bins)
                 bool tight = false;
                 for (int s = 0; s < k; ++s)
                     tight = tight || g->h_flags[((size_t)s * n + i) * kFlagSlots + kFlagTight];
-                if (rc == OHPIC_OK && tight) rc = rebalance_bins(m);
+                if (rc == OHPIC_OK && tight && ok == k)
+                    rc = rebalance_bins(
+                        m, g->h_flags[((size_t)(k - 1) * n + i) * kFlagSlots + kFlagListed]);
             }
             if (rc < 0) {
                 m->failed = true;
@@ -1400,6 +1463,7 @@ int ohpic_create(const ohpic_config *cfg, ohpic_ctx **out)
     CREATE_CHK(dalloc(ctx, &h.diag, kDiagSlots));
     CREATE_HIP(launch_fill_f64(h.diag, kDiagSlots, 0.0, ctx->st));
     CREATE_CHK(dalloc(ctx, &h.flags, kFlagSlots));
+    CREATE_CHK(dalloc(ctx, &h.tight_list, kTightListCap));
     CREATE_HIP(launch_fill_u64(h.flags, kFlagSlots, 0ull, ctx->st));
     CREATE_CHK(dalloc(ctx, &ctx->d_tmp64, nt));
     CREATE_CHK(dalloc(ctx, &ctx->d_tmp64b, nt));
@@ -2113,6 +2177,7 @@ int ohpic_diagnostics(ohpic_ctx *ctx, ohpic_diag *out)
     out->total_energy = ke + out->field_energy;
     out->repacks = ctx->repacks;
     out->rebalances = ctx->rebalances;
+    out->compactions = ctx->compactions;
     out->fast_particles = ctx->fast;
     return OHPIC_OK;
 }

@@ -319,8 +319,15 @@ __global__ void __launch_bounds__(kSumBlock) k_end_step_b(const InstDev *__restr
         const double v = block_ordered_sum(I->e_partial, count, 3, k, red);
         if (threadIdx.x == 0) I->diag[kDiagWe + k] = v;
     }
+    // the step's new bins (buffer parity ^ 1): the listed ones into
+    // tight_list, their number into kFlagListed, and kFlagTight when one is
+    // tight (the host then moves them before the next step)
     __shared__ int tight;
-    if (threadIdx.x == 0) tight = 0;
+    __shared__ unsigned int listed;
+    if (threadIdx.x == 0) {
+        tight = 0;
+        listed = 0u;
+    }
     __syncthreads();
     if (I->flags[kFlagOverflow] == 0ull) {
         const int nb = parity[blockIdx.y] ^ 1;
@@ -329,13 +336,20 @@ __global__ void __launch_bounds__(kSumBlock) k_end_step_b(const InstDev *__restr
             const SpecDev &S = I->sp[s];
             for (int t = threadIdx.x; t < I->ntiles; t += blockDim.x) {
                 const unsigned long long c = S.count[nb][t];
-                t_any = t_any || bin_tight(S.bin_cap[t], bin_front(c) + bin_back(c), I->tight_frac);
+                const int cap = S.bin_cap[t], used = bin_front(c) + bin_back(c);
+                if (!bin_listed(cap, used, I->tight_frac)) continue;
+                t_any = t_any || bin_tight(cap, used, I->tight_frac);
+                const unsigned int k = atomicAdd(&listed, 1u);
+                if (k < (unsigned int)kTightListCap) I->tight_list[k] = tight_entry(s, t, used);
             }
         }
         if (t_any) tight = 1;
     }
     __syncthreads();
-    if (tight && threadIdx.x == 0) I->flags[kFlagTight] = 1ull;
+    if (threadIdx.x == 0) {
+        I->flags[kFlagListed] = listed;
+        if (tight) I->flags[kFlagTight] = 1ull;
+    }
     if (out && threadIdx.x == 0) {
         const int i = blockIdx.y;
         const unsigned long long *f = I->flags;

@@ -43,8 +43,19 @@ enum FlagSlot : int {
     kFlagFast = 1,      // particles whose new position left the deposit halo
     kFlagRange = 2,     // positions outside [0, n) (after the wraps when wrapping)
     kFlagTight = 3,     // a step left a bin short of free slots (bin_tight)
-    kFlagSlots = 4
+    kFlagListed = 4,    // entries the step wrote to tight_list (bin_listed bins)
+    kFlagSlots = 5
 };
+
+// tight_list entries: the bin's particle count, species and tile
+constexpr int kTightListCap = 4096;
+__host__ __device__ inline unsigned long long tight_entry(int s, int t, int64_t used)
+{
+    return (unsigned long long)used << 32 | (unsigned long long)s << 27 | (unsigned long long)t;
+}
+__host__ __device__ inline int tight_entry_species(unsigned long long e) { return (int)(e >> 27) & 31; }
+__host__ __device__ inline int tight_entry_tile(unsigned long long e) { return (int)(e & 0x7ffffffull); }
+__host__ __device__ inline int64_t tight_entry_used(unsigned long long e) { return (int64_t)(e >> 32); }
 
 struct SpecDev {
     void *buf[2][7];       // component arrays of the two particle buffers
@@ -72,6 +83,7 @@ struct InstDev {
     double fix_scale;      // 2^charge_bits
     double fix_inv;        // 2^-charge_bits
     double tight_frac;     // free-slot fraction of a tight bin (bin_tight)
+    unsigned long long *tight_list;  // [kTightListCap] bins k_end_step_b found listed
     SpecDev sp[kMaxSpecies];
     unsigned long long *rho_acc[2];
     double *rho;

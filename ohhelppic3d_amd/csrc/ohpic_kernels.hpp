@@ -41,8 +41,14 @@ hipError_t launch_bin_count(const InstDev *insts, int inst, int s, int lbuf, int
                             const int T[3], int wrap, hipStream_t st);
 hipError_t launch_bin_scatter(const InstDev *insts, int inst, int s, int lbuf, int64_t n,
                               const int T[3], int wrap, hipStream_t st);
-hipError_t launch_bins_to_list(const InstDev *insts, int inst, int s, int src, int dst, int t0,
-                               int ntiles, const int64_t *list_off, hipStream_t st);
+hipError_t launch_bins_to_list(const InstDev *insts, int inst, int s, int src, int dst,
+                               const int *tiles, int t0, int ntiles, const int64_t *list_off,
+                               hipStream_t st);
+// bins tiles[i] of species s: base[i], cap[i], and used[i] particles as one
+// front run of buffer par
+hipError_t launch_set_bins(const InstDev *insts, int inst, int s, int par, const int *tiles,
+                           const int64_t *base, const int32_t *cap, const int64_t *used, int n,
+                           hipStream_t st);
 hipError_t launch_cell_counts(const InstDev *insts, int inst, int s, int par, int ntiles,
                               int32_t *counts, hipStream_t st);
 
@@ -68,8 +74,9 @@ hipError_t launch_bin_count_mx(const InstDev *insts, int inst, int s, int lbuf, 
                                const int T[3], hipStream_t st);
 hipError_t launch_bin_scatter_mx(const InstDev *insts, int inst, int s, int lbuf, int64_t n,
                                  const int T[3], hipStream_t st);
-hipError_t launch_bins_to_list_mx(const InstDev *insts, int inst, int s, int src, int dst, int t0,
-                                  int ntiles, const int64_t *list_off, hipStream_t st);
+hipError_t launch_bins_to_list_mx(const InstDev *insts, int inst, int s, int src, int dst,
+                                  const int *tiles, int t0, int ntiles, const int64_t *list_off,
+                                  hipStream_t st);
 hipError_t launch_cell_counts_mx(const InstDev *insts, int inst, int s, int par, int ntiles,
                                  int32_t *counts, const int T[3], hipStream_t st);
 

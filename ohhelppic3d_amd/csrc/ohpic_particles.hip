@@ -542,14 +542,16 @@ __global__ void k_bin_scatter(const InstDev *__restrict__ insts, int inst, int s
         gatomic_add(&I->flags[kFlagOverflow], 1ull);
 }
 
-// Compact both runs of bins t0, t0 + 1, ... of buffer src into contiguous
-// lists of buffer dst, bin t0 + i at list_off[i].  One workgroup per bin.
+// Compact both runs of bins tiles[i] (tiles null: t0 + i) of buffer src into
+// contiguous lists of buffer dst at list_off[i].  One workgroup per bin.  With
+// src == dst each list must lie outside every bin's current slots.
 __global__ void k_bins_to_list(const InstDev *__restrict__ insts, int inst, int s, int src,
-                               int dst, int t0, const int64_t *__restrict__ list_off)
+                               int dst, const int *__restrict__ tiles, int t0,
+                  const int64_t *__restrict__ list_off)
 {
     const InstDev *I = insts + inst;
     const SpecDev &S = I->sp[s];
-    const int t = t0 + blockIdx.x;
+    const int t = tiles ? tiles[blockIdx.x] : t0 + (int)blockIdx.x;
     const unsigned long long w = S.count[src][t];
     const int cap = S.bin_cap[t];
     const int F = min(bin_front(w), cap);
@@ -734,12 +736,39 @@ hipError_t launch_bin_scatter(const InstDev *insts, int inst, int s, int lbuf, i
     return hipGetLastError();
 }
 
-hipError_t launch_bins_to_list(const InstDev *insts, int inst, int s, int src, int dst, int t0,
-                               int ntiles, const int64_t *list_off, hipStream_t st)
+// Bins tiles[i] of species s take base[i] and cap[i], and used[i] particles
+// as one front run of buffer par (after their particles moved there).
+__global__ void k_set_bins(const InstDev *__restrict__ insts, int inst, int s, int par,
+                           const int *__restrict__ tiles, const int64_t *__restrict__ base,
+                           const int32_t *__restrict__ cap, const int64_t *__restrict__ used,
+                           int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const SpecDev &S = insts[inst].sp[s];
+    const int t = tiles[i];
+    S.bin_base[t] = base[i];
+    S.bin_cap[t] = cap[i];
+    S.count[par][t] = (unsigned long long)used[i];
+}
+
+hipError_t launch_set_bins(const InstDev *insts, int inst, int s, int par, const int *tiles,
+                           const int64_t *base, const int32_t *cap, const int64_t *used, int n,
+                           hipStream_t st)
+{
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_set_bins, dim3((n + 255) / 256), dim3(256), 0, st, insts, inst, s, par,
+                       tiles, base, cap, used, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_bins_to_list(const InstDev *insts, int inst, int s, int src, int dst,
+                               const int *tiles, int t0, int ntiles, const int64_t *list_off,
+                               hipStream_t st)
 {
     if (ntiles <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_bins_to_list, dim3(ntiles), dim3(256), 0, st, insts, inst, s, src, dst,
-                       t0, list_off);
+                       tiles, t0, list_off);
     return hipGetLastError();
 }
 

@@ -51,6 +51,7 @@ module m_ohpic_c
         integer(c_int64_t) :: repacks
         integer(c_int64_t) :: fast_particles
         integer(c_int64_t) :: rebalances
+        integer(c_int64_t) :: compactions
     end type
 
     interface

@@ -224,7 +224,8 @@ class Simulation:
         return dict(step=d.step, ke_old=np.array(d.ke_old[:ns]), ke_new=np.array(d.ke_new[:ns]),
                     field_energy=d.field_energy, field_energy_e2=d.field_energy_e2,
                     net_charge=d.net_charge, total_energy=d.total_energy, repacks=d.repacks,
-                    fast_particles=d.fast_particles, rebalances=d.rebalances)
+                    fast_particles=d.fast_particles, rebalances=d.rebalances,
+                    compactions=d.compactions)
 
     def energy(self) -> np.ndarray:
         """[kinetic, field, total, net charge] of the last step (ohpic_energy)."""

@@ -1,5 +1,5 @@
-// Property check of ohpic::plan_rebalance (ohpic_bins.hpp) on random bin
-// layouts: run by tests/test_bins_plan.py.  Exit status 0 when every case
+// Property check of the bin relocation planning (ohpic_bins.hpp) on random
+// bin layouts: run by tests/test_bins_plan.py.  Exit status 0 when every case
 // holds; otherwise prints the failing case.
 #include <cstdio>
 #include <cstdlib>
@@ -10,87 +10,81 @@
 using namespace ohpic;
 
 static int failures = 0;
-#define CHECK(c, ...)                                  \
-    do {                                               \
-        if (!(c)) {                                    \
-            std::printf("case %d: ", icase);           \
-            std::printf(__VA_ARGS__);                  \
-            std::printf("\n");                         \
-            ++failures;                                \
-            return;                                    \
-        }                                              \
+#define CHECK(c, ...)                        \
+    do {                                     \
+        if (!(c)) {                          \
+            std::printf("case %d: ", icase); \
+            std::printf(__VA_ARGS__);        \
+            std::printf("\n");               \
+            ++failures;                      \
+            return;                          \
+        }                                    \
     } while (0)
 
 static void one_case(int icase, std::mt19937_64 &rng, int counts[4])
 {
-    std::uniform_int_distribution<int> nt_d(1, 300);
-    const int nt = nt_d(rng);
+    const int nt = std::uniform_int_distribution<int>(1, 300)(rng);
     const double slack = std::uniform_real_distribution<double>(0.01, 0.5)(rng);
     const double frac = tight_fraction(slack);
-    std::vector<int64_t> base(nt), used(nt);
+    std::vector<int64_t> used(nt);
     std::vector<int32_t> cap(nt);
     std::uniform_int_distribution<int> c_d(0, rng() % 2 ? 200 : 20000);
-    int64_t o = 0;
+    int64_t o = 0, total = 0;
     for (int t = 0; t < nt; ++t) {
         const int64_t c0 = c_d(rng);
+        total += c0;
         cap[t] = (int32_t)bin_capacity(c0, slack);
-        base[t] = o;
         o += cap[t];
-        if (rng() % 7 == 0) o += 64 * (int64_t)(rng() % 4);  // a gap after the bin
         // the counts drift: most bins a little, some up to their capacity
-        const double g = std::uniform_real_distribution<double>(-0.3, 0.3)(rng);
+        const double g = std::uniform_real_distribution<double>(-0.3, 0.1)(rng);
         int64_t u = (int64_t)(c0 * (1.0 + g));
-        if (rng() % 5 == 0) u = cap[t] - (int64_t)(rng() % 20);
+        if (rng() % 40 == 0) u = cap[t] - (int64_t)(rng() % 20);
         used[t] = std::max<int64_t>(0, std::min<int64_t>(u, cap[t]));
     }
-    const int64_t alloc = o + (int64_t)(rng() % 3) * 64 * (int64_t)(rng() % 50);
-    std::vector<BinWindow> win;
-    std::vector<int64_t> nbase;
-    std::vector<int32_t> ncap;
-    int big = -1;
-    const int r = plan_rebalance(nt, base, cap, alloc, used, frac, slack, win, nbase, ncap, &big);
-    counts[r] += 1;
-    bool any_tight = false;
+    // the pool of a buffer of this capacity, sometimes half in use already
+    const int64_t alloc = o + (int64_t)std::ceil(total * slack * kPoolSlackShare) +
+                          (int64_t)(rng() % 3) * 64;
+    const int64_t pool0 = o + (rng() % 3 == 0 ? (alloc - o) / 2 : 0);
+    // the candidates k_end_step_b lists
+    std::vector<BinCand> cands;
     int64_t need = 0;
     for (int t = 0; t < nt; ++t) {
-        any_tight = any_tight || bin_tight(cap[t], used[t], frac);
-        need += bin_capacity(used[t], slack);
-    }
-    if (r == kPlanNone) {
-        CHECK(!any_tight, "kPlanNone with a tight bin");
-        return;
-    }
-    CHECK(any_tight, "plan without a tight bin");
-    if (r == kPlanWhole) {
-        CHECK(need > alloc - base[0], "kPlanWhole although the whole buffer fits");
-        return;
-    }
-    CHECK(r == kPlanWindows, "unexpected result %d", r);
-    std::vector<char> in(nt, 0);
-    int prev_b = -1;
-    for (const BinWindow &w : win) {
-        CHECK(w.a > prev_b && w.a <= w.b && w.b < nt, "windows out of order or overlapping");
-        prev_b = w.b;
-        bool has_tight = false;
-        CHECK(nbase[w.a] == base[w.a], "window start moved");
-        const int64_t end = w.b + 1 < nt ? base[w.b + 1] : alloc;
-        for (int t = w.a; t <= w.b; ++t) {
-            in[t] = 1;
-            has_tight = has_tight || bin_tight(cap[t], used[t], frac);
-            CHECK(ncap[t] % 64 == 0 && ncap[t] >= bin_capacity(used[t], slack) &&
-                      ncap[t] < kBinCapLimit,
-                  "bin %d: capacity %d for %lld particles", t, ncap[t], (long long)used[t]);
-            CHECK(!bin_tight(ncap[t], used[t], frac), "bin %d still tight", t);
-            if (t > w.a)
-                CHECK(nbase[t] == nbase[t - 1] + ncap[t - 1], "bin %d not contiguous", t);
+        if (bin_tight(cap[t], used[t], frac))
+            CHECK(bin_listed(cap[t], used[t], frac), "tight bin %d not listed", t);
+        if (bin_listed(cap[t], used[t], frac)) {
+            cands.push_back({t, used[t]});
+            need += bin_capacity(used[t], 2.0 * slack);
         }
-        CHECK(has_tight, "window without a tight bin");
-        CHECK(nbase[w.b] + ncap[w.b] <= end, "window overruns its span");
     }
-    for (int t = 0; t < nt; ++t) {
-        if (bin_tight(cap[t], used[t], frac)) CHECK(in[t], "tight bin %d not in a window", t);
-        if (!in[t]) CHECK(nbase[t] == base[t] && ncap[t] == cap[t], "bin %d outside moved", t);
+    if (cands.empty()) {
+        counts[0] += 1;
+        return;
     }
+    int64_t pool = pool0;
+    std::vector<BinMove> moves;
+    int big = -1;
+    const int r = plan_relocation(cands, slack, &pool, alloc, moves, &big);
+    counts[r] += 1;
+    if (r == kPlanWhole) {
+        CHECK(pool0 + need > alloc && moves.empty() && pool == pool0,
+              "kPlanWhole although the pool holds the listed bins");
+        return;
+    }
+    CHECK(r == kPlanMoves, "unexpected result %d", r);
+    CHECK(moves.size() == cands.size(), "%zu moves for %zu bins", moves.size(), cands.size());
+    int64_t p = pool0;
+    for (size_t k = 0; k < moves.size(); ++k) {
+        const BinMove &m = moves[k];
+        CHECK(m.tile == cands[k].tile && m.used == cands[k].used, "move %zu is not bin %d", k,
+              cands[k].tile);
+        CHECK(m.base == p, "bin %d not packed in the pool", m.tile);
+        CHECK(m.cap % 64 == 0 && m.cap >= bin_capacity(m.used, slack) && m.cap < kBinCapLimit,
+              "bin %d: capacity %d for %lld particles", m.tile, m.cap, (long long)m.used);
+        CHECK(!bin_listed(m.cap, m.used, frac), "bin %d still listed after its move", m.tile);
+        p += m.cap;
+    }
+    CHECK(p == pool && pool <= alloc, "pool end %lld, alloc %lld", (long long)pool,
+          (long long)alloc);
 }
 
 int main(int argc, char **argv)
@@ -99,14 +93,17 @@ int main(int argc, char **argv)
     std::mt19937_64 rng(12345);
     int counts[4] = {0, 0, 0, 0};
     for (int i = 0; i < n; ++i) one_case(i, rng, counts);
-    std::printf("none %d windows %d whole %d toobig %d failures %d\n", counts[0], counts[1],
+    std::printf("none %d moves %d whole %d toobig %d failures %d\n", counts[0], counts[1],
                 counts[2], counts[3], failures);
-    // bin_capacity leaves a fresh bin untight at every count
+    // a bin whose capacity was just set is neither tight nor listed, with the
+    // slack (a full re-bin) or twice the slack (a move)
     for (int64_t c = 0; c < 200000; c += 7)
-        for (double s : {0.01, 0.05, 0.15, 0.6})
-            if (bin_tight(bin_capacity(c, s), c, tight_fraction(s))) {
-                std::printf("fresh bin of %lld at slack %g is tight\n", (long long)c, s);
-                return 1;
-            }
+        for (double s : {0.01, 0.05, 0.15, 0.3, 0.6})
+            for (double k : {1.0, 2.0})
+                if (bin_listed(bin_capacity(c, k * s), c, tight_fraction(s))) {
+                    std::printf("fresh bin of %lld at slack %g x%g is listed\n", (long long)c, s,
+                                k);
+                    return 1;
+                }
     return failures ? 1 : 0;
 }

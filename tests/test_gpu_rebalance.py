@@ -1,12 +1,14 @@
-"""Bins short of free slots are re-laid out between steps instead of
-overflowing into a full re-bin.
+"""Bins short of free slots move between steps instead of overflowing into
+a full re-bin.
 
 An expanding Gaussian hump of nearly free-streaming particles (q/m 1e-6)
 fills the bins around it a little every step.  At slack 0.1 the bins there
-get tight: the run rebalances (only the neighbourhood of each tight bin is
-re-laid out) and never repacks, and its fields, cell counts and particles
-equal, bit for bit, those of the same run at slack 0.6, where no bin gets
-tight, and the FP64 oracle's.  Bin layout is storage only: the reference has
+get tight: the run rebalances (each tight bin moves, with fresh slack, into
+the relocation pool at the end of the particle buffers) and never repacks,
+and its fields, cell counts and particles equal, bit for bit, those of the
+same run at slack 0.6, where no bin gets tight, and the FP64 oracle's.  Over
+160 steps the moved bins fill the pool, and the run compacts every species
+into the other buffer instead of re-binning it.  Bin layout is storage only: the reference has
 no counterpart (its particles live in per-process arrays that
 ohhelppic3d.f90:311-358 steps in place)."""
 import numpy as np
@@ -92,9 +94,32 @@ def test_tight_bins_are_rebalanced_not_repacked(ohp, orc, precision):
         assert np.array_equal(tight["counts"], ref)
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_pool_exhaustion_compacts_instead_of_repacking(ohp, orc, precision):
+    # over 160 steps the moved bins fill the relocation pool: every species is
+    # then compacted into the other buffer (fresh pools), never re-binned
+    prec = ohp.FP64 if precision == "fp64" else ohp.MIXED
+    steps = 160
+    sp = expanding_hump(orc)
+    tight = run(ohp, sp, prec, SLACK, steps)
+    wide = run(ohp, sp, prec, 0.6, steps)
+    d = tight["diag"]
+    print(precision, "rebalances", d["rebalances"], "compactions", d["compactions"],
+          "repacks", d["repacks"])
+    assert d["compactions"] >= 1 and d["repacks"] == 0 and d["step"] == steps
+    assert_same(tight, wide)
+    if prec == ohp.FP64:
+        osim = orc.OracleSim(N, sp, 0.1, (0.0, 0.0, 0.0), 1)
+        osim.step(steps)
+        assert rel_l2(tight["fields"][0], osim.rho) <= TOL
+        for s in range(2):
+            for k in ("x", "vx"):
+                assert rel_l2(tight["parts"][s][k], osim.sp[s][k]) <= TOL, (s, k)
+
+
 def test_group_member_rebalances_between_sequences(ohp, orc):
     # the tight bins are found inside graph-replayed sequences of 4 steps and
-    # re-laid out after each sequence; the member equals the run alone
+    # moved after each sequence; the member equals the run alone
     sp = expanding_hump(orc)
     alone = run(ohp, sp, ohp.FP64, SLACK)
     other = [dict(d) for d in sp]
