@@ -215,7 +215,7 @@ int64_t estimate_bytes(const ohpic_config &c, int num_cus)
     b += ng * (2 * 8 + 5 * 8 + 4);           // int64 charge x2, rho, phi, E, cell counts
     b += nt * (8 + 8 + 4 + 4) + 8;           // host-transfer scratch, cursor
     b += 3 * kEfieldMaxBlocks * 8;           // field-energy partials
-    b += kDiagSlots * 8 + kFlagSlots * 8 + kTightListCap * 8 + (int64_t)sizeof(InstDev) + 2 * 4 +
+    b += kDiagSlots * 8 + kFlagWords * 8 + kTightListCap * 8 + (int64_t)sizeof(InstDev) + 2 * 4 +
          (int64_t)sizeof(FftJob);
     b += (int64_t)(c.nx + c.ny + c.nz) * (16 + 8);  // twiddles, Green's-function tables
     b += max_work_items(c, nt, num_cus) * ((int64_t)sizeof(WorkItem) + kKeSlots * 8);
@@ -376,7 +376,7 @@ int read_flags(ohpic_ctx *ctx)
 
 int clear_flags(ohpic_ctx *ctx)
 {
-    HIPCHK(ctx, launch_fill_u64(ctx->h.flags, kFlagSlots, 0ull, ctx->st));
+    HIPCHK(ctx, launch_fill_u64(ctx->h.flags, kFlagWords, 0ull, ctx->st));
     return OHPIC_OK;
 }
 
@@ -1462,9 +1462,9 @@ int ohpic_create(const ohpic_config *cfg, ohpic_ctx **out)
         CREATE_HIP(launch_fill_f64(p, ctx->ng, 0.0, ctx->st));
     CREATE_CHK(dalloc(ctx, &h.diag, kDiagSlots));
     CREATE_HIP(launch_fill_f64(h.diag, kDiagSlots, 0.0, ctx->st));
-    CREATE_CHK(dalloc(ctx, &h.flags, kFlagSlots));
+    CREATE_CHK(dalloc(ctx, &h.flags, kFlagWords));
     CREATE_CHK(dalloc(ctx, &h.tight_list, kTightListCap));
-    CREATE_HIP(launch_fill_u64(h.flags, kFlagSlots, 0ull, ctx->st));
+    CREATE_HIP(launch_fill_u64(h.flags, kFlagWords, 0ull, ctx->st));
     CREATE_CHK(dalloc(ctx, &ctx->d_tmp64, nt));
     CREATE_CHK(dalloc(ctx, &ctx->d_tmp64b, nt));
     CREATE_CHK(dalloc(ctx, &ctx->d_tmp32a, nt));

@@ -264,7 +264,7 @@ __global__ void k_prepare_next_b(const InstDev *__restrict__ insts, const int *_
         unsigned long long *__restrict__ w = I->sp[s].count[nb];
         for (int64_t t = t0; t < I->ntiles; t += stride) w[t] = 0ull;
     }
-    if (t0 < kFlagSlots) I->flags[t0] = 0ull;
+    if (t0 < kFlagWords) I->flags[t0] = 0ull;
 }
 
 // Start of a step: rho := the current charge grid * 2^-charge_bits, and the
@@ -291,17 +291,18 @@ __global__ void k_begin_step_b(const InstDev *__restrict__ insts, const int *__r
         unsigned long long *__restrict__ w = I->sp[s].count[par ^ 1];
         for (int64_t t = t0; t < I->ntiles; t += stride) w[t] = 0ull;
     }
-    if (t0 < kFlagSlots) I->flags[t0] = 0ull;
+    if (t0 < kFlagWords) I->flags[t0] = 0ull;
 }
 
 // End of a step: blocks x < kKeSlots reduce the kinetic-energy slots (as
 // k_reduce_ke_b), block kKeSlots the field-energy partials of k_efield_b (as
-// k_reduce_e_b), then, unless a bin overflowed, looks for tight bins in the
-// step's new buffer (parity ^ 1) and raises kFlagTight for one (the host then
-// restores the slack of its neighbourhood before the next step).  When out is
-// not null it finishes the instance's step in a group: out[i*kFlagSlots + k]
-// := flags[k] of instance i, and parity[i] flips unless a bin overflowed (the
-// host then redoes that step in list mode from the unchanged buffer).
+// k_reduce_e_b); unless a bin overflowed, all of them then scan a share of
+// the step's new bins (buffer parity ^ 1): listed bins go to tight_list (their
+// number to kFlagListed) and a tight one raises kFlagTight (the host then
+// moves them before the next step).  When out is not null the last block to
+// finish completes the instance's step in a group: out[i*kFlagSlots + k] :=
+// flags[k] of instance i, and parity[i] flips unless a bin overflowed (the host
+// then redoes that step in list mode from the unchanged buffer).
 __global__ void __launch_bounds__(kSumBlock) k_end_step_b(const InstDev *__restrict__ insts,
                                                     int *__restrict__ parity,
                                                     unsigned long long *__restrict__ out)
@@ -312,48 +313,48 @@ __global__ void __launch_bounds__(kSumBlock) k_end_step_b(const InstDev *__restr
         const int k = blockIdx.x;
         const double v = ke_slot_sum(I, k, red);
         if (threadIdx.x == 0) I->diag[kDiagKeOld + k] = v;
-        return;
+    } else {
+        const int count = efield_blocks_of(I->n[0], I->n[1], I->n[2]);
+        for (int k = 0; k < 3; ++k) {
+            const double v = block_ordered_sum(I->e_partial, count, 3, k, red);
+            if (threadIdx.x == 0) I->diag[kDiagWe + k] = v;
+        }
     }
-    const int count = efield_blocks_of(I->n[0], I->n[1], I->n[2]);
-    for (int k = 0; k < 3; ++k) {
-        const double v = block_ordered_sum(I->e_partial, count, 3, k, red);
-        if (threadIdx.x == 0) I->diag[kDiagWe + k] = v;
-    }
-    // the step's new bins (buffer parity ^ 1): the listed ones into
-    // tight_list, their number into kFlagListed, and kFlagTight when one is
-    // tight (the host then moves them before the next step)
-    __shared__ int tight;
-    __shared__ unsigned int listed;
-    if (threadIdx.x == 0) {
-        tight = 0;
-        listed = 0u;
-    }
-    __syncthreads();
-    if (I->flags[kFlagOverflow] == 0ull) {
+    unsigned long long *flags = I->flags;
+    if (flags[kFlagOverflow] == 0ull) {
         const int nb = parity[blockIdx.y] ^ 1;
-        bool t_any = false;
+        const int t0 = blockIdx.x * blockDim.x + threadIdx.x;
+        const int stride = gridDim.x * blockDim.x;
         for (int s = 0; s < I->nspecies; ++s) {
             const SpecDev &S = I->sp[s];
-            for (int t = threadIdx.x; t < I->ntiles; t += blockDim.x) {
+            for (int t = t0; t < I->ntiles; t += stride) {
                 const unsigned long long c = S.count[nb][t];
                 const int cap = S.bin_cap[t], used = bin_front(c) + bin_back(c);
                 if (!bin_listed(cap, used, I->tight_frac)) continue;
-                t_any = t_any || bin_tight(cap, used, I->tight_frac);
-                const unsigned int k = atomicAdd(&listed, 1u);
-                if (k < (unsigned int)kTightListCap) I->tight_list[k] = tight_entry(s, t, used);
+                const unsigned long long k = atomicAdd(&flags[kFlagListed], 1ull);
+                if (k < (unsigned long long)kTightListCap) I->tight_list[k] = tight_entry(s, t, used);
+                if (bin_tight(cap, used, I->tight_frac)) atomicOr(&flags[kFlagTight], 1ull);
+                __threadfence();  // visible before this block counts itself done
             }
         }
-        if (t_any) tight = 1;
     }
+    if (!out) return;
+    // the flags are device-scope atomics (a thread that listed a bin fenced
+    // its writes before the barrier), so the counter needs no fence of its
+    // own: device-scope fences write back the L2 and cost microseconds per
+    // block on this multi-XCD device
+    __shared__ bool last;
     __syncthreads();
-    if (threadIdx.x == 0) {
-        I->flags[kFlagListed] = listed;
-        if (tight) I->flags[kFlagTight] = 1ull;
-    }
-    if (out && threadIdx.x == 0) {
+    if (threadIdx.x == 0)
+        last = atomicAdd(&flags[kFlagEndBlocks], 1ull) == (unsigned long long)gridDim.x - 1ull;
+    __syncthreads();
+    if (last && threadIdx.x == 0) {
         const int i = blockIdx.y;
-        const unsigned long long *f = I->flags;
-        for (int k = 0; k < kFlagSlots; ++k) out[i * kFlagSlots + k] = f[k];
+        unsigned long long f[kFlagSlots];
+        for (int k = 0; k < kFlagSlots; ++k) {
+            f[k] = __hip_atomic_load(&flags[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            out[i * kFlagSlots + k] = f[k];
+        }
         if (f[kFlagOverflow] == 0ull) parity[i] ^= 1;
     }
 }

@@ -46,6 +46,10 @@ enum FlagSlot : int {
     kFlagListed = 4,    // entries the step wrote to tight_list (bin_listed bins)
     kFlagSlots = 5
 };
+// flags[kFlagEndBlocks]: blocks of k_end_step_b that finished (a counter,
+// not a reported flag); the start of a step clears every word
+constexpr int kFlagEndBlocks = kFlagSlots;
+constexpr int kFlagWords = kFlagSlots + 1;
 
 // tight_list entries: the bin's particle count, species and tile
 constexpr int kTightListCap = 4096;
