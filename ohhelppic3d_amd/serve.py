@@ -281,6 +281,7 @@ class ServingLoop:
         pool = ThreadPoolExecutor(self.creators) if clock == "wall" and self.creators > 0 else None
         creating: Dict[int, tuple] = {}
         t0 = time.perf_counter()
+        last_say = t0
         nsteps = 0
         active_sum = 0
         pi = 0
@@ -378,7 +379,9 @@ class ServingLoop:
                     entry[2] -= taken
                     if entry[2] <= 0:
                         finish(rid, "done")
-                if progress and nsteps // 200 != (nsteps - taken) // 200:
+                if progress and (nsteps // 200 != (nsteps - taken) // 200 or
+                                 time.perf_counter() - last_say >= 10.0):
+                    last_say = time.perf_counter()
                     say(f"[serve] step {nsteps}: {len(active)} active, "
                         f"{len(sched.waiting)} waiting, {sched.used / 1e9:.1f} GB admitted")
         finally:

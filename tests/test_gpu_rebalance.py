@@ -137,3 +137,29 @@ def test_group_member_rebalances_between_sequences(ohp, orc):
     assert got["diag"]["rebalances"] >= 1 and got["diag"]["repacks"] == 0
     assert_same(got, alone)
     assert_same(wide, alone)
+
+
+def test_group_member_compacts_between_sequences(ohp, orc):
+    # 160 steps in sequences of 4: the member's pool runs out and it is
+    # compacted between sequences (its parity and charge grids flip outside
+    # the group's graph); it equals the run alone, which compacts after single
+    # steps
+    steps = 160
+    sp = expanding_hump(orc)
+    alone = run(ohp, sp, ohp.MIXED, SLACK, steps)
+    with sim_for(sp, N, b0=(0.0, 0.0, 0.0), weight_order=1, precision=ohp.MIXED,
+                 slack=SLACK) as a, \
+            sim_for([dict(d) for d in sp], N, b0=(0.0, 0.0, 0.0), weight_order=1,
+                    precision=ohp.MIXED, slack=0.6) as b:
+        with ohp.Group(use_graph=True, persistent=True, batch=4) as grp:
+            grp.attach(a)
+            grp.attach(b)
+            grp.step(steps)
+        got = snapshot(ohp, a)
+        wide = snapshot(ohp, b)
+    d = got["diag"]
+    print("group member rebalances", d["rebalances"], "compactions", d["compactions"],
+          "repacks", d["repacks"])
+    assert d["step"] == steps and d["compactions"] >= 1 and d["repacks"] == 0
+    assert_same(got, alone)
+    assert_same(wide, alone)
