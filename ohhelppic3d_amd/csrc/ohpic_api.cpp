@@ -771,7 +771,8 @@ int make_axis(ohpic_ctx *ctx, int n, FftAxisDev &ax)
     double2 *d = nullptr;
     int rc = dalloc(ctx, &d, n);
     if (rc) return rc;
-    HIPCHK(ctx, hipMemcpy(d, tw.data(), n * sizeof(double2), hipMemcpyHostToDevice));
+    HIPCHK(ctx, hipMemcpyAsync(d, tw.data(), n * sizeof(double2), hipMemcpyHostToDevice, ctx->st));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->st));
     ax.tw = d;
     return OHPIC_OK;
 }
@@ -934,6 +935,7 @@ struct ohpic_group {
     hipGraph_t graph_b = nullptr;     // `batch` steps
     hipGraphExec_t exec_b = nullptr;
     int64_t steps = 0, captures = 0;
+    int64_t capture_failures = 0;  // sequences launched directly after a failed capture
     std::string err;
 };
 
@@ -1176,8 +1178,14 @@ int group_step(ohpic_group *g, int nsteps)
         // whole sequences of `batch` steps while that many remain, then single
         // steps: one host synchronisation per sequence
         const int k = (g->batch > 1 && nsteps - done >= g->batch) ? g->batch : 1;
-        if (g->use_graph) {
-            if (!(k == 1 ? g->exec : g->exec_b) && (rc = group_capture(g, k))) return rc;
+        if (g->use_graph && !(k == 1 ? g->exec : g->exec_b) && group_capture(g, k) != OHPIC_OK) {
+            // a capture that failed (another thread's allocation or free can
+            // invalidate it) enqueued nothing: this sequence launches its
+            // kernels directly and the next one captures again
+            (void)hipGetLastError();
+            g->capture_failures += 1;
+            GHIP(g, group_enqueue(g, k));
+        } else if (g->use_graph) {
             GHIP(g, hipGraphLaunch(k == 1 ? g->exec : g->exec_b, g->st));
         } else {
             GHIP(g, group_enqueue(g, k));
@@ -1475,7 +1483,8 @@ int ohpic_create(const ohpic_config *cfg, ohpic_ctx **out)
     CREATE_CHK(dalloc(ctx, &ctx->d_par01, 2));
     {
         const int p01[2] = {0, 1};
-        CREATE_HIP(hipMemcpy(ctx->d_par01, p01, sizeof p01, hipMemcpyHostToDevice));
+        CREATE_HIP(hipMemcpyAsync(ctx->d_par01, p01, sizeof p01, hipMemcpyHostToDevice, ctx->st));
+        CREATE_HIP(hipStreamSynchronize(ctx->st));
     }
     // FFT plans and the Green's-function denominators per axis
     for (int a = 0; a < 3; ++a) {
@@ -1490,7 +1499,9 @@ int ohpic_create(const ohpic_config *cfg, ohpic_ctx **out)
             lam[k] = wn * wn;
         }
         CREATE_CHK(dalloc(ctx, &ctx->d_lam[a], n));
-        CREATE_HIP(hipMemcpy(ctx->d_lam[a], lam.data(), n * sizeof(double), hipMemcpyHostToDevice));
+        CREATE_HIP(hipMemcpyAsync(ctx->d_lam[a], lam.data(), n * sizeof(double),
+                                  hipMemcpyHostToDevice, ctx->st));
+        CREATE_HIP(hipStreamSynchronize(ctx->st));
     }
     {
         FftJob &J = ctx->job;
@@ -1505,7 +1516,8 @@ int ohpic_create(const ohpic_config *cfg, ohpic_ctx **out)
         }
         J.inv_n = 1.0 / (double)ctx->ng;
         CREATE_CHK(dalloc(ctx, &ctx->d_job, 1));
-        CREATE_HIP(hipMemcpy(ctx->d_job, &J, sizeof J, hipMemcpyHostToDevice));
+        CREATE_HIP(hipMemcpyAsync(ctx->d_job, &J, sizeof J, hipMemcpyHostToDevice, ctx->st));
+        CREATE_HIP(hipStreamSynchronize(ctx->st));
     }
     CREATE_HIP(hipHostMalloc((void **)&ctx->h_flags, kFlagSlots * sizeof(unsigned long long),
                              hipHostMallocDefault));

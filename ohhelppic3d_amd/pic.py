@@ -337,6 +337,12 @@ class Group:
         """Steps per launch sequence (1 to 16, default 8)."""
         self._chk(self._lib.ohpic_group_set_batch(self._g, int(steps)))
 
+    def last_error(self) -> str:
+        """The group's last error message (also of a failed graph capture that
+        it recovered from by launching that sequence directly); "" if none."""
+        m = self._lib.ohpic_group_last_error(self._g)
+        return m.decode() if isinstance(m, bytes) else (m or "")
+
     def stats(self) -> dict:
         s, c, i = C.c_int64(), C.c_int64(), C.c_int64()
         self._chk(self._lib.ohpic_group_stats(self._g, C.byref(s), C.byref(c), C.byref(i)))

@@ -274,11 +274,15 @@ class ServingLoop:
             raise ValueError("request ids must be unique")
         sched = Scheduler(self.budget, self.max_active, self.footprint)
         active: Dict[int, list] = {}
-        grp = Group(self.device, self.use_graph, batch=self.batch)
         # wall clock: runs are created by worker threads while the group steps
         # (the library's calls release the interpreter lock); steps clock:
         # created in line, so the mix is reproducible
         pool = ThreadPoolExecutor(self.creators) if clock == "wall" and self.creators > 0 else None
+        # a graph capture is not safe while other threads make HIP calls: a
+        # capture they invalidated was left open on the group's stream, and
+        # every later call on it failed.  With creator threads the launch
+        # sequences are enqueued directly.
+        grp = Group(self.device, self.use_graph and pool is None, batch=self.batch)
         creating: Dict[int, tuple] = {}
         t0 = time.perf_counter()
         last_say = t0
@@ -369,6 +373,15 @@ class ServingLoop:
                 except OhpicError as e:
                     err = str(e)
                 taken = grp.stats()["steps"] - before
+                if err is not None and taken == 0 and all(grp.contains(e[1])
+                                                          for e in active.values()):
+                    # the group failed before any step and left every member
+                    # attached: the same call would fail again, so the active
+                    # runs fail with its error
+                    say(f"[serve] group step failed: {err}")
+                    for rid in list(active):
+                        finish(rid, "failed", err)
+                    continue
                 nsteps += taken
                 active_sum += len(active) * taken
                 for rid in list(active):
@@ -397,6 +410,8 @@ class ServingLoop:
                 pool.shutdown(wait=True)
             for rid in list(active):
                 finish(rid, "failed", "serving loop stopped")
+            if grp.last_error():
+                say(f"[serve] group's last error: {grp.last_error()}")
             grp.close()
         out = [outcomes[r.rid] for r in pending]
         first = min((o.arrival for o in out), default=0.0)
