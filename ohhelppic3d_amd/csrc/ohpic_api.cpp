@@ -15,6 +15,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <numeric>
@@ -96,6 +97,10 @@ struct ohpic_ctx {
     int64_t repacks = 0;
     int64_t rebalances = 0;
     int64_t compactions = 0;
+    // listed bins read from the device list (at most kTightListCap); beyond
+    // it the bins are found from the downloaded counts.  OHPIC_TIGHT_LIST_LIMIT
+    // lowers it, so that tests reach that path at small sizes.
+    unsigned long long list_limit = kTightListCap;
     int64_t fast = 0;
     int num_cus = 256;
     bool timing = false;
@@ -580,8 +585,8 @@ int compact_bins(ohpic_ctx *ctx)
 }
 
 // After a step that left a bin tight: move the bins k_end_step_b listed
-// (`listed` of them, from tight_list; when more than the list holds, the
-// listed bins are found from every species' counts).  When a species' pool
+// (`listed` of them, from tight_list; when more than list_limit, the listed
+// bins are found from every species' counts).  When a species' pool
 // cannot take its tight bins, every species is compacted instead.
 int rebalance_bins(ohpic_ctx *ctx, unsigned long long listed)
 {
@@ -589,7 +594,7 @@ int rebalance_bins(ohpic_ctx *ctx, unsigned long long listed)
     const int ns = ctx->cfg.nspecies, nt = ctx->ntiles;
     std::vector<std::vector<BinCand>> cands(ns);
     int rc;
-    if (listed <= (unsigned long long)kTightListCap) {
+    if (listed <= ctx->list_limit) {
         std::vector<unsigned long long> e(listed);
         HIPCHK(ctx, hipMemcpyAsync(e.data(), ctx->h.tight_list, listed * sizeof(e[0]),
                                    hipMemcpyDeviceToHost, ctx->st));
@@ -1424,6 +1429,10 @@ int ohpic_create(const ohpic_config *cfg, ohpic_ctx **out)
     h.fix_scale = std::ldexp(1.0, ctx->charge_bits);
     h.fix_inv = std::ldexp(1.0, -ctx->charge_bits);
     h.tight_frac = tight_fraction(config_slack(c));
+    if (const char *e = std::getenv("OHPIC_TIGHT_LIST_LIMIT")) {
+        const long long v = std::atoll(e);
+        if (v >= 0 && v < (long long)kTightListCap) ctx->list_limit = (unsigned long long)v;
+    }
     const int nt = ctx->ntiles;
     for (int s = 0; s < c.nspecies; ++s) {
         SpeciesHost &S = ctx->sp[s];

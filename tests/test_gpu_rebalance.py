@@ -95,6 +95,24 @@ def test_tight_bins_are_rebalanced_not_repacked(ohp, orc, precision):
 
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_bins_beyond_the_device_list_are_found_from_the_counts(ohp, orc, monkeypatch,
+                                                               precision):
+    # OHPIC_TIGHT_LIST_LIMIT=0: every step's listed bins count as more than the
+    # device list holds, so each relocation finds them from the downloaded bin
+    # counts instead; the bins it moves, and so the run, are the same
+    prec = ohp.FP64 if precision == "fp64" else ohp.MIXED
+    sp = expanding_hump(orc)
+    listed = run(ohp, sp, prec, SLACK)
+    monkeypatch.setenv("OHPIC_TIGHT_LIST_LIMIT", "0")
+    counted = run(ohp, sp, prec, SLACK)
+    monkeypatch.delenv("OHPIC_TIGHT_LIST_LIMIT")
+    for key in ("rebalances", "compactions", "repacks", "step"):
+        assert counted["diag"][key] == listed["diag"][key], key
+    assert counted["diag"]["rebalances"] >= 1 and counted["diag"]["repacks"] == 0
+    assert_same(counted, listed)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_pool_exhaustion_compacts_instead_of_repacking(ohp, orc, precision):
     # over 160 steps the moved bins fill the relocation pool: every species is
     # then compacted into the other buffer (fresh pools), never re-binned
