@@ -263,6 +263,12 @@ int ohpic_group_use_graph(ohpic_group *g, int enable);
 int ohpic_group_use_persistent(ohpic_group *g, int enable);
 /* steps taken, graph captures, work items of the current table */
 int ohpic_group_stats(const ohpic_group *g, int64_t *steps, int64_t *captures, int64_t *items);
+/* launch sequences replayed from a captured graph, and sequences enqueued
+ * launch by launch (graphs disabled, or another thread was inside a library
+ * call when the sequence needed a capture: a capture is only taken while no
+ * other thread makes HIP calls through the library, since such calls can
+ * invalidate it) */
+int ohpic_group_sequences(const ohpic_group *g, int64_t *replayed, int64_t *direct);
 const char *ohpic_group_last_error(const ohpic_group *g);
 
 /* --- autotuner ---------------------------------------------------------------

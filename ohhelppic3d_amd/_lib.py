@@ -36,6 +36,7 @@ EXPORTS = (
     "ohpic_group_create", "ohpic_group_destroy", "ohpic_group_attach", "ohpic_group_detach",
     "ohpic_group_size", "ohpic_group_contains", "ohpic_group_step", "ohpic_group_use_graph",
     "ohpic_group_use_persistent", "ohpic_group_set_batch", "ohpic_group_stats",
+    "ohpic_group_sequences",
     "ohpic_group_last_error", "ohpic_autotune",
     "ohpic_snapshot_open", "ohpic_snapshot_write", "ohpic_snapshot_write_host",
     "ohpic_snapshot_close", "ohpic_snapshot_last_error", "ohpic_snapshot_read",
@@ -170,6 +171,7 @@ def lib():
         "ohpic_group_use_persistent": [vp, C.c_int],
         "ohpic_group_set_batch": [vp, C.c_int],
         "ohpic_group_stats": [vp, i64p, i64p, i64p],
+        "ohpic_group_sequences": [vp, i64p, i64p],
         "ohpic_autotune": [C.POINTER(Config), C.c_int, C.c_int, C.c_int, C.POINTER(TuneResult)],
         "ohpic_snapshot_open": [C.c_char_p, C.c_char_p, i64p, i64p, C.POINTER(vp)],
         "ohpic_snapshot_write": [vp, C.c_char_p, vp, C.c_int],

@@ -19,6 +19,7 @@
 #include <cstring>
 #include <mutex>
 #include <numeric>
+#include <shared_mutex>
 #include <string>
 #include <vector>
 
@@ -33,6 +34,53 @@ namespace {
 
 std::mutex g_err_mu;
 std::string g_err;
+
+// A group captures its launch sequence into a hipGraph on its own stream.
+// Another thread's HIP calls during that capture can invalidate it (a free or
+// a synchronous copy synchronises with every stream of the device) and then
+// fail themselves.  Every entry point that makes HIP calls holds this gate
+// shared for its duration; a group takes it exclusively for a capture only
+// when no other thread holds it (try_lock, so a capture never waits) and
+// otherwise launches that sequence directly.  Entry points wait at most for
+// one capture (well under a millisecond: enqueueing, no GPU work).
+std::shared_mutex g_gate;
+thread_local bool t_gate_held = false;
+
+struct GateShared {
+    bool owner;
+    GateShared() : owner(!t_gate_held)
+    {
+        if (owner) {
+            g_gate.lock_shared();
+            t_gate_held = true;
+        }
+    }
+    ~GateShared()
+    {
+        if (owner) {
+            t_gate_held = false;
+            g_gate.unlock_shared();
+        }
+    }
+    GateShared(const GateShared &) = delete;
+    GateShared &operator=(const GateShared &) = delete;
+};
+
+// exclusive hold for a capture, trading the caller's shared hold for it;
+// false (the shared hold kept) when another thread is inside an entry point
+bool gate_capture_begin()
+{
+    if (t_gate_held) g_gate.unlock_shared();
+    if (g_gate.try_lock()) return true;
+    if (t_gate_held) g_gate.lock_shared();
+    return false;
+}
+
+void gate_capture_end()
+{
+    g_gate.unlock();
+    if (t_gate_held) g_gate.lock_shared();
+}
 
 constexpr int kNumTimers = 6;
 const char *kTimerNames[kNumTimers] = {"push_fused", "field_solve", "deposit", "repack",
@@ -941,6 +989,8 @@ struct ohpic_group {
     hipGraphExec_t exec_b = nullptr;
     int64_t steps = 0, captures = 0;
     int64_t capture_failures = 0;  // sequences launched directly after a failed capture
+    int64_t replayed = 0;          // launch sequences replayed from a graph
+    int64_t direct = 0;            // launch sequences enqueued kernel by kernel
     std::string err;
 };
 
@@ -1183,17 +1233,25 @@ int group_step(ohpic_group *g, int nsteps)
         // whole sequences of `batch` steps while that many remain, then single
         // steps: one host synchronisation per sequence
         const int k = (g->batch > 1 && nsteps - done >= g->batch) ? g->batch : 1;
-        if (g->use_graph && !(k == 1 ? g->exec : g->exec_b) && group_capture(g, k) != OHPIC_OK) {
-            // a capture that failed (another thread's allocation or free can
-            // invalidate it) enqueued nothing: this sequence launches its
+        bool have = g->use_graph && (k == 1 ? g->exec : g->exec_b);
+        if (g->use_graph && !have && gate_capture_begin()) {
+            // no other thread is inside the library: capture now.  A capture
+            // that failed enqueued nothing; this sequence then launches its
             // kernels directly and the next one captures again
-            (void)hipGetLastError();
-            g->capture_failures += 1;
-            GHIP(g, group_enqueue(g, k));
-        } else if (g->use_graph) {
+            have = group_capture(g, k) == OHPIC_OK;
+            gate_capture_end();
+            if (!have) {
+                (void)hipGetLastError();
+                g->capture_failures += 1;
+            }
+        }
+        if (have) {
             GHIP(g, hipGraphLaunch(k == 1 ? g->exec : g->exec_b, g->st));
+            g->replayed += 1;
         } else {
+            // no graph (disabled, or another thread was making HIP calls)
             GHIP(g, group_enqueue(g, k));
+            g->direct += 1;
         }
         GHIP(g, hipStreamSynchronize(g->st));
         // every member's steps are accounted for before a failure is reported:
@@ -1327,6 +1385,7 @@ const char *ohpic_last_error(const ohpic_ctx *ctx)
 
 int ohpic_estimate_footprint(const ohpic_config *cfg, int64_t *bytes)
 {
+    GateShared gate;
     if (!cfg || !bytes) return fail(nullptr, OHPIC_EINVAL, "NULL argument");
     if (cfg->chunks < 0 || cfg->chunks > kMaxChunks)
         return fail(nullptr, OHPIC_EINVAL, "chunks must be in [0, %d] (0 = automatic)", kMaxChunks);
@@ -1342,6 +1401,7 @@ int ohpic_estimate_footprint(const ohpic_config *cfg, int64_t *bytes)
 
 int ohpic_create(const ohpic_config *cfg, ohpic_ctx **out)
 {
+    GateShared gate;
     if (!cfg || !out) return fail(nullptr, OHPIC_EINVAL, "NULL argument");
     *out = nullptr;
     const ohpic_config &c = *cfg;
@@ -1546,6 +1606,7 @@ int ohpic_create(const ohpic_config *cfg, ohpic_ctx **out)
 
 int ohpic_destroy(ohpic_ctx *ctx)
 {
+    GateShared gate;
     release(ctx);
     return OHPIC_OK;
 }
@@ -1565,6 +1626,7 @@ int ohpic_load_particles(ohpic_ctx *ctx, int ispec, int64_t n, const double *x, 
                          const double *z, const double *vx, const double *vy, const double *vz,
                          const int64_t *pid)
 {
+    GateShared gate;
     if (!ctx) return fail(nullptr, OHPIC_EINVAL, "ctx is NULL");
     if (ispec < 0 || ispec >= ctx->cfg.nspecies)
         return fail(ctx, OHPIC_EINVAL, "species %d out of range", ispec);
@@ -1620,6 +1682,7 @@ int ohpic_load_particles(ohpic_ctx *ctx, int ispec, int64_t n, const double *x, 
 
 int ohpic_particle_count(ohpic_ctx *ctx, int ispec, int64_t *n)
 {
+    GateShared gate;
     if (!ctx || !n) return fail(ctx, OHPIC_EINVAL, "NULL argument");
     if (ispec < 0 || ispec >= ctx->cfg.nspecies)
         return fail(ctx, OHPIC_EINVAL, "species %d out of range", ispec);
@@ -1631,6 +1694,7 @@ int ohpic_store_particles(ohpic_ctx *ctx, int ispec, int64_t cap, int64_t *n, do
                           double *y, double *z, double *vx, double *vy, double *vz,
                           int64_t *pid)
 {
+    GateShared gate;
     if (!ctx || !n) return fail(ctx, OHPIC_EINVAL, "NULL argument");
     if (ispec < 0 || ispec >= ctx->cfg.nspecies)
         return fail(ctx, OHPIC_EINVAL, "species %d out of range", ispec);
@@ -1681,6 +1745,7 @@ int ohpic_store_particles(ohpic_ctx *ctx, int ispec, int64_t cap, int64_t *n, do
 
 int ohpic_load_oh_particles(ohpic_ctx *ctx, int64_t n, const ohpic_oh_particle *rec)
 {
+    GateShared gate;
     if (!ctx) return fail(nullptr, OHPIC_EINVAL, "ctx is NULL");
     if (n > 0 && !rec) return fail(ctx, OHPIC_EINVAL, "records are NULL");
     const int ns = ctx->cfg.nspecies;
@@ -1716,6 +1781,7 @@ int ohpic_load_oh_particles(ohpic_ctx *ctx, int64_t n, const ohpic_oh_particle *
 
 int ohpic_store_oh_particles(ohpic_ctx *ctx, int64_t cap, int64_t *n, ohpic_oh_particle *rec)
 {
+    GateShared gate;
     if (!ctx || !n) return fail(ctx, OHPIC_EINVAL, "NULL argument");
     int64_t tot = 0;
     for (int s = 0; s < ctx->cfg.nspecies; ++s) tot += ctx->sp[s].n;
@@ -1751,6 +1817,7 @@ int ohpic_store_oh_particles(ohpic_ctx *ctx, int64_t cap, int64_t *n, ohpic_oh_p
 int ohpic_seed_uniform_maxwell(ohpic_ctx *ctx, uint64_t initstate, uint64_t initseq,
                                const int64_t *np, const double *vth)
 {
+    GateShared gate;
     if (!ctx || !np || !vth) return fail(ctx, OHPIC_EINVAL, "NULL argument");
     HIPCHK(ctx, hipSetDevice(ctx->device));
     // pcg_setseq_64_srandom_r (pcg_variants.h:790-798) on the host
@@ -1792,6 +1859,7 @@ int ohpic_seed_hump_maxwell(ohpic_ctx *ctx, uint64_t initstate, uint64_t initseq
                             const int64_t *np, const double *vth, double amp, double sigma,
                             uint64_t budget)
 {
+    GateShared gate;
     if (!ctx || !np || !vth) return fail(ctx, OHPIC_EINVAL, "NULL argument");
     if (budget < 8) return fail(ctx, OHPIC_EINVAL, "budget must be >= 8 draws");
     HIPCHK(ctx, hipSetDevice(ctx->device));
@@ -1837,6 +1905,7 @@ int ohpic_seed_hump_maxwell(ohpic_ctx *ctx, uint64_t initstate, uint64_t initseq
 
 int ohpic_deposit(ohpic_ctx *ctx)
 {
+    GateShared gate;
     if (!ctx) return fail(nullptr, OHPIC_EINVAL, "ctx is NULL");
     if (!ctx->binned)
         return fail(ctx, OHPIC_ESTATE, "deposit needs binned particles: call boundary first");
@@ -1851,6 +1920,7 @@ int ohpic_deposit(ohpic_ctx *ctx)
 
 int ohpic_field_solve(ohpic_ctx *ctx)
 {
+    GateShared gate;
     if (!ctx) return fail(nullptr, OHPIC_EINVAL, "ctx is NULL");
     HIPCHK(ctx, hipSetDevice(ctx->device));
     int rc = do_solve(ctx, true);
@@ -1862,6 +1932,7 @@ int ohpic_field_solve(ohpic_ctx *ctx)
 
 int ohpic_gather_push(ohpic_ctx *ctx)
 {
+    GateShared gate;
     if (!ctx) return fail(nullptr, OHPIC_EINVAL, "ctx is NULL");
     if (!ctx->binned)
         return fail(ctx, OHPIC_ESTATE, "gather_push needs binned particles: call boundary");
@@ -1878,6 +1949,7 @@ int ohpic_gather_push(ohpic_ctx *ctx)
 
 int ohpic_boundary(ohpic_ctx *ctx)
 {
+    GateShared gate;
     if (!ctx) return fail(nullptr, OHPIC_EINVAL, "ctx is NULL");
     HIPCHK(ctx, hipSetDevice(ctx->device));
     int rc = zero_next_counts(ctx);
@@ -1902,6 +1974,7 @@ int ohpic_boundary(ohpic_ctx *ctx)
 
 int ohpic_step(ohpic_ctx *ctx, int nsteps)
 {
+    GateShared gate;
     if (!ctx) return fail(nullptr, OHPIC_EINVAL, "ctx is NULL");
     if (nsteps < 0) return fail(ctx, OHPIC_EINVAL, "nsteps < 0");
     if (ctx->failed) return fail(ctx, OHPIC_ESTATE, "an earlier step failed: %s", ctx->err.c_str());
@@ -1920,6 +1993,7 @@ int ohpic_step(ohpic_ctx *ctx, int nsteps)
 
 int ohpic_synchronize(ohpic_ctx *ctx)
 {
+    GateShared gate;
     if (!ctx) return fail(nullptr, OHPIC_EINVAL, "ctx is NULL");
     HIPCHK(ctx, hipStreamSynchronize(ctx->st));
     timers_collect(ctx);
@@ -1941,6 +2015,7 @@ static int field_ptr(ohpic_ctx *ctx, int which, double **p)
 int ohpic_get_field(ohpic_ctx *ctx, int which, double *values, int nelem, const int32_t lb[3],
                     const int32_t ub[3], int comp)
 {
+    GateShared gate;
     if (!ctx || !values || !lb || !ub) return fail(ctx, OHPIC_EINVAL, "NULL argument");
     const bool eb = which == OHPIC_FIELD_EB;
     if (nelem < (eb ? 6 : 1) || (!eb && (comp < 0 || comp >= nelem)))
@@ -1982,6 +2057,7 @@ int ohpic_get_field(ohpic_ctx *ctx, int which, double *values, int nelem, const 
 int ohpic_put_field(ohpic_ctx *ctx, int which, const double *values, int nelem,
                     const int32_t lb[3], const int32_t ub[3], int comp)
 {
+    GateShared gate;
     if (!ctx || !values || !lb || !ub) return fail(ctx, OHPIC_EINVAL, "NULL argument");
     if (which == OHPIC_FIELD_EB) return fail(ctx, OHPIC_EINVAL, "put EB: put EX/EY/EZ instead");
     if (nelem < 1 || comp < 0 || comp >= nelem) return fail(ctx, OHPIC_EINVAL, "bad nelem/comp");
@@ -2011,6 +2087,7 @@ int ohpic_put_field(ohpic_ctx *ctx, int which, const double *values, int nelem,
 int ohpic_fft3d_r2r(int device, int nx, int ny, int nz, int backward, const double *in,
                     double *out)
 {
+    GateShared gate;
     if (!in || !out) return fail(nullptr, OHPIC_EINVAL, "NULL argument");
     if (nx < 1 || ny < 1 || nz < 1) return fail(nullptr, OHPIC_EINVAL, "sizes must be >= 1");
     ohpic_ctx tmp;  // holds allocations and the error string only
@@ -2101,6 +2178,7 @@ static __device__ __align__(256) unsigned char scratch[sizeof(NoallocArea)];
 int ohpic_fft3d_r2r_noalloc(int device, int nx, int ny, int nz, int backward, const double *in,
                             double *out)
 {
+    GateShared gate;
     if (!in || !out) return fail(nullptr, OHPIC_EINVAL, "NULL argument");
     if (nx < 1 || ny < 1 || nz < 1) return fail(nullptr, OHPIC_EINVAL, "sizes must be >= 1");
     const int64_t ng = (int64_t)nx * ny * nz;
@@ -2179,6 +2257,7 @@ int ohpic_fft3d_r2r_noalloc(int device, int nx, int ny, int nz, int backward, co
 
 int ohpic_diagnostics(ohpic_ctx *ctx, ohpic_diag *out)
 {
+    GateShared gate;
     if (!ctx || !out) return fail(ctx, OHPIC_EINVAL, "NULL argument");
     HIPCHK(ctx, hipSetDevice(ctx->device));
     HIPCHK(ctx, hipMemcpyAsync(ctx->h_diag, ctx->h.diag, kDiagSlots * sizeof(double),
@@ -2205,6 +2284,7 @@ int ohpic_diagnostics(ohpic_ctx *ctx, ohpic_diag *out)
 
 int ohpic_energy(ohpic_ctx *ctx, double out[4])
 {
+    GateShared gate;
     if (!ctx || !out) return fail(ctx, OHPIC_EINVAL, "NULL argument");
     ohpic_diag d;
     const int rc = ohpic_diagnostics(ctx, &d);
@@ -2218,6 +2298,7 @@ int ohpic_energy(ohpic_ctx *ctx, double out[4])
 
 int ohpic_cell_counts(ohpic_ctx *ctx, int ispec, int32_t *counts)
 {
+    GateShared gate;
     if (!ctx || !counts) return fail(ctx, OHPIC_EINVAL, "NULL argument");
     if (ispec >= ctx->cfg.nspecies) return fail(ctx, OHPIC_EINVAL, "species out of range");
     if (!ctx->binned) return fail(ctx, OHPIC_ESTATE, "particles pushed but not re-binned");
@@ -2244,6 +2325,7 @@ int ohpic_cell_counts(ohpic_ctx *ctx, int ispec, int32_t *counts)
 
 int ohpic_enable_timing(ohpic_ctx *ctx, int enable)
 {
+    GateShared gate;
     if (!ctx) return fail(nullptr, OHPIC_EINVAL, "ctx is NULL");
     timers_collect(ctx);
     ctx->timing = enable != 0;
@@ -2257,6 +2339,7 @@ int ohpic_enable_timing(ohpic_ctx *ctx, int enable)
 int ohpic_kernel_times(ohpic_ctx *ctx, int n, char names[][32], double *total_ms,
                        int64_t *launches, int *count)
 {
+    GateShared gate;
     if (!ctx || !count) return fail(ctx, OHPIC_EINVAL, "NULL argument");
     HIPCHK(ctx, hipStreamSynchronize(ctx->st));
     timers_collect(ctx);
@@ -2275,6 +2358,7 @@ int ohpic_kernel_times(ohpic_ctx *ctx, int n, char names[][32], double *total_ms
 
 int ohpic_memory_footprint(ohpic_ctx *ctx, int64_t *bytes)
 {
+    GateShared gate;
     if (!ctx || !bytes) return fail(ctx, OHPIC_EINVAL, "NULL argument");
     *bytes = ctx->bytes;
     return OHPIC_OK;
@@ -2283,6 +2367,7 @@ int ohpic_memory_footprint(ohpic_ctx *ctx, int64_t *bytes)
 // ------------------------------------------------------------------ groups
 int ohpic_group_create(int device, ohpic_group **out)
 {
+    GateShared gate;
     if (!out) return fail(nullptr, OHPIC_EINVAL, "NULL argument");
     *out = nullptr;
     int ndev = 0;
@@ -2313,6 +2398,7 @@ int ohpic_group_create(int device, ohpic_group **out)
 
 int ohpic_group_destroy(ohpic_group *g)
 {
+    GateShared gate;
     if (!g) return OHPIC_OK;
     (void)hipSetDevice(g->device);
     while (!g->members.empty()) (void)group_detach(g, g->members.back());
@@ -2332,6 +2418,7 @@ int ohpic_group_destroy(ohpic_group *g)
 
 int ohpic_group_attach(ohpic_group *g, ohpic_ctx *ctx)
 {
+    GateShared gate;
     if (!g || !ctx) return fail(nullptr, OHPIC_EINVAL, "NULL argument");
     if (ctx->group) return gfail(g, OHPIC_ESTATE, "context already belongs to a group");
     if (ctx->device != g->device)
@@ -2367,6 +2454,7 @@ int ohpic_group_attach(ohpic_group *g, ohpic_ctx *ctx)
 
 int ohpic_group_detach(ohpic_group *g, ohpic_ctx *ctx)
 {
+    GateShared gate;
     if (!g || !ctx) return fail(nullptr, OHPIC_EINVAL, "NULL argument");
     return group_detach(g, ctx);
 }
@@ -2387,6 +2475,7 @@ int ohpic_group_size(const ohpic_group *g, int *n)
 
 int ohpic_group_step(ohpic_group *g, int nsteps)
 {
+    GateShared gate;
     if (!g) return fail(nullptr, OHPIC_EINVAL, "group is NULL");
     if (nsteps < 0) return gfail(g, OHPIC_EINVAL, "nsteps < 0");
     return group_step(g, nsteps);
@@ -2394,6 +2483,7 @@ int ohpic_group_step(ohpic_group *g, int nsteps)
 
 int ohpic_group_use_graph(ohpic_group *g, int enable)
 {
+    GateShared gate;
     if (!g) return fail(nullptr, OHPIC_EINVAL, "group is NULL");
     g->use_graph = enable != 0;
     (void)hipSetDevice(g->device);
@@ -2404,6 +2494,7 @@ int ohpic_group_use_graph(ohpic_group *g, int enable)
 
 int ohpic_group_set_batch(ohpic_group *g, int steps)
 {
+    GateShared gate;
     if (!g) return fail(nullptr, OHPIC_EINVAL, "group is NULL");
     if (steps < 1 || steps > kGroupMaxBatch)
         return gfail(g, OHPIC_EINVAL, "batch of %d steps: 1 to %d", steps, kGroupMaxBatch);
@@ -2418,6 +2509,7 @@ int ohpic_group_set_batch(ohpic_group *g, int steps)
 
 int ohpic_group_use_persistent(ohpic_group *g, int enable)
 {
+    GateShared gate;
     if (!g) return fail(nullptr, OHPIC_EINVAL, "group is NULL");
     g->persistent = enable != 0;
     (void)hipSetDevice(g->device);
@@ -2435,6 +2527,14 @@ int ohpic_group_stats(const ohpic_group *g, int64_t *steps, int64_t *captures, i
     return OHPIC_OK;
 }
 
+int ohpic_group_sequences(const ohpic_group *g, int64_t *replayed, int64_t *direct)
+{
+    if (!g) return fail(nullptr, OHPIC_EINVAL, "group is NULL");
+    if (replayed) *replayed = g->replayed;
+    if (direct) *direct = g->direct;
+    return OHPIC_OK;
+}
+
 const char *ohpic_group_last_error(const ohpic_group *g)
 {
     if (g) return g->err.c_str();
@@ -2444,6 +2544,7 @@ const char *ohpic_group_last_error(const ohpic_group *g)
 // ---------------------------------------------------------------- autotuner
 int ohpic_autotune(const ohpic_config *cfgs, int n, int sample, int steps, ohpic_tune_result *out)
 {
+    GateShared gate;
     if (!cfgs || !out || n <= 0) return fail(nullptr, OHPIC_EINVAL, "bad arguments");
     if (steps < 1) steps = 3;
     if (sample < 1 || sample > n) sample = n;

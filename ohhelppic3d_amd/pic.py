@@ -346,7 +346,10 @@ class Group:
     def stats(self) -> dict:
         s, c, i = C.c_int64(), C.c_int64(), C.c_int64()
         self._chk(self._lib.ohpic_group_stats(self._g, C.byref(s), C.byref(c), C.byref(i)))
-        return {"steps": s.value, "captures": c.value, "items": i.value}
+        r, d = C.c_int64(), C.c_int64()
+        self._chk(self._lib.ohpic_group_sequences(self._g, C.byref(r), C.byref(d)))
+        return {"steps": s.value, "captures": c.value, "items": i.value,
+                "replayed": r.value, "direct": d.value}
 
 
 def autotune(cfgs: Sequence[SimConfig], sample: int = 16, steps: int = 3) -> dict:

@@ -5,8 +5,10 @@ of one run) arrive over time.  The loop keeps every admitted run's particles
 and fields resident on the device and advances all of them together in
 ``Group`` launch sequences of a few steps, so runs that arrive while others
 are on the GPU join at the next sequence and their work overlaps on the
-device; new runs are created by worker threads while the group steps.
-Admission is FIFO
+device; new runs are created by worker threads while the group steps.  Each
+launch sequence is a replayed hipGraph, captured again when the membership
+changes; a capture waits for no creator (when one is inside a library call
+then, that sequence is enqueued launch by launch).  Admission is FIFO
 against a device-memory budget (the library's footprint estimate per run) and
 a slot limit; a run that finishes frees its slot and memory at once.  A run
 whose footprint exceeds the whole budget is rejected; a run that fails
@@ -193,6 +195,8 @@ class ServeReport:
     mean_active: float
     peak_bytes: int
     clock: str
+    graph_sequences: int = 0      # launch sequences replayed from a captured graph
+    direct_sequences: int = 0     # launch sequences enqueued launch by launch
 
     def summary(self) -> dict:
         done = [o for o in self.outcomes if o.status == "done"]
@@ -209,6 +213,7 @@ class ServeReport:
             "particle_steps_per_s": psteps / self.makespan if self.makespan > 0 else None,
             "makespan": self.makespan, "group_steps": self.group_steps,
             "mean_active": self.mean_active, "peak_admitted_bytes": self.peak_bytes,
+            "graph_sequences": self.graph_sequences, "direct_sequences": self.direct_sequences,
         }
 
 
@@ -220,7 +225,8 @@ class ServingLoop:
     with the wall clock an arrival waits at most one such call.  With the wall
     clock, admitted runs are created (allocated, seeded, binned on their own
     streams) by `creators` worker threads while the group keeps stepping, and
-    join the group at the first step after they are ready."""
+    join the group at the first step after they are ready.  The report counts
+    the launch sequences replayed from graphs and those enqueued directly."""
 
     def __init__(self, budget_bytes: int = HBM_BUDGET, max_active: int = 256,
                  tile=(8, 8, 8), chunks: int = 1, precision: int = L.MIXED,
@@ -278,11 +284,10 @@ class ServingLoop:
         # (the library's calls release the interpreter lock); steps clock:
         # created in line, so the mix is reproducible
         pool = ThreadPoolExecutor(self.creators) if clock == "wall" and self.creators > 0 else None
-        # a graph capture is not safe while other threads make HIP calls: a
-        # capture they invalidated was left open on the group's stream, and
-        # every later call on it failed.  With creator threads the launch
-        # sequences are enqueued directly.
-        grp = Group(self.device, self.use_graph and pool is None, batch=self.batch)
+        # the group captures a launch sequence only while no creator thread is
+        # inside a library call (their HIP calls could invalidate the
+        # capture), and otherwise enqueues that sequence directly
+        grp = Group(self.device, self.use_graph, batch=self.batch)
         creating: Dict[int, tuple] = {}
         t0 = time.perf_counter()
         last_say = t0
@@ -412,10 +417,12 @@ class ServingLoop:
                 finish(rid, "failed", "serving loop stopped")
             if grp.last_error():
                 say(f"[serve] group's last error: {grp.last_error()}")
+            gstats = grp.stats()
             grp.close()
         out = [outcomes[r.rid] for r in pending]
         first = min((o.arrival for o in out), default=0.0)
         last = max((o.finished for o in out if o.finished is not None), default=first)
         return ServeReport(outcomes=out, makespan=last - first, group_steps=nsteps,
                            mean_active=active_sum / max(nsteps, 1), peak_bytes=sched.peak_used,
-                           clock=clock)
+                           clock=clock, graph_sequences=gstats["replayed"],
+                           direct_sequences=gstats["direct"])

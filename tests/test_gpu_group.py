@@ -129,6 +129,70 @@ def test_group_step_sequences_equal_alone(ohp, use_graph):
         assert_same(g, a, f"member {i}")
 
 
+def test_group_captures_while_other_threads_create_runs(ohp):
+    # the group steps, capturing again before every sequence (a member leaves
+    # and rejoins), while two threads create, seed, step and destroy contexts;
+    # a capture is only taken while no other thread is inside a library call,
+    # so no capture fails, no other thread's call fails, and the members equal
+    # the same runs stepped alone
+    import threading
+    import time
+    stop = threading.Event()
+    errors, made = [], [0, 0]
+
+    def churn(i):
+        # a pause after each context leaves moments with no thread inside the
+        # library, in which captures go through
+        while not stop.is_set():
+            try:
+                with make(ohp, (16, 16, 24), 4, 100 + i, ohp.MIXED) as s:
+                    s.step(1)
+                made[i] += 1
+            except Exception as e:  # reported below, not raised in the thread
+                errors.append(repr(e))
+                return
+            time.sleep(0.01)
+
+    sims = [make(ohp, n, ppc, seed, ohp.MIXED) for n, ppc, seed in CASES]
+    threads = [threading.Thread(target=churn, args=(i,)) for i in range(2)]
+    try:
+        with ohp.Group(batch=2) as grp:
+            for s in sims:
+                grp.attach(s)
+            for t in threads:
+                t.start()
+            seqs = 0
+            # at least 10 sequences, and until each thread has made 3 contexts
+            while seqs < 200 and (seqs < 10 or min(made) < 3) and not errors:
+                grp.detach(sims[0])
+                grp.attach(sims[0])
+                grp.step(2)
+                seqs += 1
+            stop.set()
+            for t in threads:
+                t.join()
+            stats = grp.stats()
+            last = grp.last_error()
+            grouped = [snapshot(ohp, s) for s in sims]
+    finally:
+        stop.set()
+        for t in threads:
+            if t.is_alive():
+                t.join()
+        for s in sims:
+            s.close()
+    print("contexts made by the other threads", made, "| sequences replayed",
+          stats["replayed"], "direct", stats["direct"], "captures", stats["captures"])
+    assert not errors, errors
+    assert last == "", last
+    assert min(made) >= 1
+    assert stats["steps"] == 2 * seqs and stats["replayed"] + stats["direct"] == seqs
+    assert stats["captures"] == stats["replayed"]  # every capture is replayed once
+    alone = run_alone(ohp, ohp.MIXED, 2 * seqs)
+    for i, (a, g) in enumerate(zip(alone, grouped)):
+        assert_same(g, a, f"member {i}")
+
+
 def drifting_pair(ohp):
     # every particle starts in tile 0 and moves one cell into tile 1, whose
     # bin capacity is the minimum: the step overflows and is redone as a list

@@ -51,6 +51,8 @@ def test_serving_mix_equals_alone(ohp):
     summ = rep.summary()
     assert summ["done"] == len(reqs), [(o.rid, o.status, o.error) for o in rep.outcomes]
     assert rep.mean_active > 1.5           # runs overlapped on the device
+    # no other thread makes HIP calls: every launch sequence is a graph replay
+    assert summ["graph_sequences"] >= 1 and summ["direct_sequences"] == 0
     assert rep.peak_bytes <= budget
     waited = [o for o in rep.outcomes if o.admitted > o.arrival]
     assert waited                          # some run waited for memory
@@ -84,7 +86,9 @@ def test_wall_clock_serving_creates_runs_while_stepping(ohp):
     mix = {}
     rep = serve.ServingLoop(creators=2, **kw).run(
         reqs, clock="wall", on_complete=lambda r, sim: mix.__setitem__(r.rid, snapshot(ohp, sim)))
-    assert rep.summary()["done"] == len(reqs), [(o.rid, o.status, o.error) for o in rep.outcomes]
+    summ = rep.summary()
+    assert summ["done"] == len(reqs), [(o.rid, o.status, o.error) for o in rep.outcomes]
+    print("sequences replayed", summ["graph_sequences"], "direct", summ["direct_sequences"])
     for r in reqs:
         one = serve.Request(**{**r.to_dict(), "arrival": 0.0, "n": tuple(r.n),
                                "ppc": tuple(r.ppc)})
