@@ -42,7 +42,8 @@ std::string g_err;
 // shared for its duration; a group takes it exclusively for a capture only
 // when no other thread holds it (try_lock, so a capture never waits) and
 // otherwise launches that sequence directly.  Entry points wait at most for
-// one capture (well under a millisecond: enqueueing, no GPU work).
+// one capture (enqueueing the sequence and instantiating its graph, no GPU
+// work).
 std::shared_mutex g_gate;
 thread_local bool t_gate_held = false;
 
