@@ -528,7 +528,7 @@ int zero_next_counts(ohpic_ctx *ctx)
 // buffer par (from their old slots, which stay unused until the next full
 // re-bin), then the bins take their new bases, capacities and counts.
 // Returns 1 when the pool cannot take even the tight bins, so that the caller
-// re-bins the species in full.
+// compacts every species into the other buffer (fresh pools).
 int relocate_bins(ohpic_ctx *ctx, int s, std::vector<BinCand> cands)
 {
     SpeciesHost &S = ctx->sp[s];
