@@ -102,6 +102,10 @@ def test_group_graph_equals_direct_launches(ohp):
     with_graph, st_g = run_group(ohp, ohp.MIXED, 4, use_graph=True, tile=(8, 8, 8), chunks=2)
     direct, st_d = run_group(ohp, ohp.MIXED, 4, use_graph=False, tile=(8, 8, 8), chunks=2)
     assert st_g["captures"] >= 1 and st_d["captures"] == 0
+    # one thread: every sequence of the graph group is a replay, none of the
+    # other's is
+    assert st_g["replayed"] >= 1 and st_g["direct"] == 0
+    assert st_d["replayed"] == 0 and st_d["direct"] >= 1
     for i, (a, b) in enumerate(zip(with_graph, direct)):
         assert_same(a, b, f"member {i}")
 
@@ -191,6 +195,52 @@ def test_group_captures_while_other_threads_create_runs(ohp):
     alone = run_alone(ohp, ohp.MIXED, 2 * seqs)
     for i, (a, g) in enumerate(zip(alone, grouped)):
         assert_same(g, a, f"member {i}")
+
+
+def test_two_groups_stepped_from_two_threads(ohp):
+    # two groups, each needing a capture before every sequence, stepped at
+    # once from two threads: a capture is only taken while the other thread
+    # is outside the library (with both stepping back to back, that is
+    # seldom, and those sequences are enqueued directly), so both groups'
+    # members equal the runs alone
+    import threading
+    halves = [CASES[:2], CASES[2:]]
+    seqs = 8
+    errors = []
+    out = [None, None]
+
+    def drive(h):
+        sims = [make(ohp, n, ppc, seed, ohp.MIXED) for n, ppc, seed in halves[h]]
+        try:
+            with ohp.Group(batch=2) as grp:
+                for s in sims:
+                    grp.attach(s)
+                for _ in range(seqs):
+                    grp.detach(sims[0])
+                    grp.attach(sims[0])
+                    grp.step(2)
+                out[h] = ([snapshot(ohp, s) for s in sims], grp.stats(), grp.last_error())
+        except Exception as e:  # reported below, not raised in the thread
+            errors.append(repr(e))
+        finally:
+            for s in sims:
+                s.close()
+
+    threads = [threading.Thread(target=drive, args=(h,)) for h in range(2)]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    assert not errors, errors
+    alone = run_alone(ohp, ohp.MIXED, 2 * seqs)
+    for h in range(2):
+        snaps, stats, last = out[h]
+        print("group", h, "sequences replayed", stats["replayed"], "direct", stats["direct"])
+        assert last == "", last
+        assert stats["steps"] == 2 * seqs and stats["replayed"] + stats["direct"] == seqs
+        assert stats["captures"] == stats["replayed"]
+        for i, g in enumerate(snaps):
+            assert_same(g, alone[2 * h + i], f"group {h} member {i}")
 
 
 def drifting_pair(ohp):
